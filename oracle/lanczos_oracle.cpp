@@ -1,0 +1,242 @@
+// CPU restatement of the reference's truncated Lanczos driver -- TEST INFRASTRUCTURE.
+//
+// sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548) on top of the oracle's
+// H.v and sparse-vector operations (sfci_oracle.cpp).  The Nl x Nl generalized
+// eigenproblem that the reference hands to Eigen's GeneralizedSelfAdjointEigenSolver
+// (src/sparse_solver.cpp:450-454) is solved here by Cholesky of S followed by cyclic
+// Jacobi on L^-1 K L^-T.  Eigenvector sign: the first component with |c_i| > 0 is made
+// positive (the reference takes whatever sign Eigen returns).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+extern "C" {
+typedef struct oracle_ham oracle_ham;
+int oracle_hv(const oracle_ham*, const uint64_t*, const double*, int64_t, int64_t, double, double, double,
+              int64_t, uint64_t*, double*, int64_t*, double*, int, int, int64_t*);
+double oracle_dot(const uint64_t*, const double*, int64_t, const uint64_t*, const double*, int64_t);
+double oracle_norm(const double*, int64_t);
+int64_t oracle_add_two(const uint64_t*, const double*, int64_t, const uint64_t*, const double*, int64_t,
+                       double, int64_t, uint64_t*, double*);
+int64_t oracle_shrink(const uint64_t*, const double*, int64_t, int64_t, uint64_t*, double*);
+int64_t oracle_lincomb(const uint64_t*, const double*, const int64_t*, int, const double*, int64_t,
+                       uint64_t*, double*, double*);
+double oracle_energy(const oracle_ham*, const uint64_t*, const double*, int64_t, int);
+}
+
+namespace {
+
+struct Vec {
+    std::vector<uint64_t> k;
+    std::vector<double> v;
+    int64_t len = 0;
+    int64_t cap = 0;
+    explicit Vec(int64_t c = 0) : k(c), v(c), cap(c) {}
+    void copy_from(const Vec& s) {
+        if (s.len > (int64_t)k.size()) { k.resize(s.len); v.resize(s.len); }
+        std::copy(s.k.begin(), s.k.begin() + s.len, k.begin());
+        std::copy(s.v.begin(), s.v.begin() + s.len, v.begin());
+        len = s.len;
+    }
+};
+
+// Generalized symmetric eigenproblem K x = lambda S x (n x n, row-major).
+// Returns 0, or -1 when S is not positive definite.
+int gsaes(int n, const std::vector<double>& K, const std::vector<double>& S,
+          std::vector<double>& evals, std::vector<double>& evecs) {
+    std::vector<double> L(n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = S[i * n + j];
+            for (int k = 0; k < j; ++k) s -= L[i * n + k] * L[j * n + k];
+            if (i == j) {
+                if (!(s > 0.0)) return -1;
+                L[i * n + i] = std::sqrt(s);
+            } else {
+                L[i * n + j] = s / L[j * n + j];
+            }
+        }
+    // C = L^-1 K L^-T : first Y = L^-1 K (forward substitution per column)
+    std::vector<double> Y(n * n), C(n * n);
+    for (int c = 0; c < n; ++c)
+        for (int i = 0; i < n; ++i) {
+            double s = K[i * n + c];
+            for (int k = 0; k < i; ++k) s -= L[i * n + k] * Y[k * n + c];
+            Y[i * n + c] = s / L[i * n + i];
+        }
+    // C = Y L^-T  <=>  C^T = L^-1 Y^T
+    for (int r = 0; r < n; ++r)
+        for (int i = 0; i < n; ++i) {
+            double s = Y[r * n + i];
+            for (int k = 0; k < i; ++k) s -= L[i * n + k] * C[r * n + k];
+            C[r * n + i] = s / L[i * n + i];
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            double a = 0.5 * (C[i * n + j] + C[j * n + i]);
+            C[i * n + j] = C[j * n + i] = a;
+        }
+    // cyclic Jacobi
+    std::vector<double> V(n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += C[i * n + j] * C[i * n + j];
+                if (i != j) off += C[i * n + j] * C[i * n + j];
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = C[p * n + q];
+                if (apq == 0.0) continue;
+                double app = C[p * n + p], aqq = C[q * n + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double ckp = C[k * n + p], ckq = C[k * n + q];
+                    C[k * n + p] = c * ckp - s * ckq;
+                    C[k * n + q] = s * ckp + c * ckq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double cpk = C[p * n + k], cqk = C[q * n + k];
+                    C[p * n + k] = c * cpk - s * cqk;
+                    C[q * n + k] = s * cpk + c * cqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return C[a * n + a] < C[b * n + b]; });
+    evals.assign(n, 0.0);
+    evecs.assign(n * n, 0.0);   // column j = eigenvector j
+    for (int jj = 0; jj < n; ++jj) {
+        int j = order[jj];
+        evals[jj] = C[j * n + j];
+        // x = L^-T y
+        std::vector<double> x(n);
+        for (int i = n - 1; i >= 0; --i) {
+            double s = V[i * n + j];
+            for (int k = i + 1; k < n; ++k) s -= L[k * n + i] * x[k];
+            x[i] = s / L[i * n + i];
+        }
+        double sg = 1.0;
+        for (int i = 0; i < n; ++i)
+            if (x[i] != 0.0) { sg = x[i] > 0 ? 1.0 : -1.0; break; }
+        for (int i = 0; i < n; ++i) evecs[i * n + jj] = sg * x[i];
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_gsaes(int n, const double* K, const double* S, double* evals, double* evecs) {
+    std::vector<double> k(K, K + n * n), s(S, S + n * n), e, v;
+    int rc = gsaes(n, k, s, e, v);
+    if (rc) return rc;
+    std::copy(e.begin(), e.end(), evals);
+    std::copy(v.begin(), v.end(), evecs);
+    return 0;
+}
+
+// sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548).
+// Start vector: init_det with coefficient 1.  Writes per-restart Ritz values and
+// <v|H|v> after truncation (up to max_restart each); returns the number of restarts run
+// or a negative error.  Final vector written to out_k/out_v (capacity nd).
+int oracle_lanczos(const oracle_ham* H, uint64_t init_det, int64_t nd, int nl, int max_restart,
+                   int64_t hpsi_cap, double cut, int nthreads,
+                   double* ritz, double* e_trunc, double* e_final,
+                   uint64_t* out_k, double* out_v, int64_t* out_len) {
+    const double deltaE = 1e-5;
+    double previousE = 0.0, nearestE = -1000000.0;
+    std::vector<Vec> K(nl, Vec(nd));
+    Vec v(nd);
+    v.k[0] = init_det; v.v[0] = 1.0; v.len = 1;
+    std::vector<double> KM(nl * nl, 0.0), S(nl * nl, 0.0);
+    int res = 0;
+    for (res = 0; res < max_restart; ++res) {
+        K[0].copy_from(v);
+        Vec u(hpsi_cap), temp(hpsi_cap), temp_u(hpsi_cap), v_next(nd);
+        double alpha;
+        int rc = oracle_hv(H, v.k.data(), v.v.data(), v.len, nd, 0.0, 1.0, cut, u.cap,
+                           u.k.data(), u.v.data(), &u.len, &alpha, nthreads, 0, nullptr);
+        if (rc) return -2;
+        KM[0] = alpha;
+        double beta;
+        for (int i = 1; i < nl; ++i) {
+            temp.copy_from(u);
+            temp.len = oracle_add_two(u.k.data(), u.v.data(), u.len, v.k.data(), v.v.data(), v.len,
+                                      -alpha, temp.cap, temp.k.data(), temp.v.data());
+            v_next.len = oracle_shrink(temp.k.data(), temp.v.data(), temp.len, v_next.cap,
+                                       v_next.k.data(), v_next.v.data());
+            double g = oracle_norm(v_next.v.data(), v_next.len);
+            beta = g;
+            double inv = 1.0 / g;
+            for (int64_t t = 0; t < v_next.len; ++t) v_next.v[t] *= inv;
+            rc = oracle_hv(H, v_next.k.data(), v_next.v.data(), v_next.len, nd, 0.0, 1.0, cut, temp_u.cap,
+                           temp_u.k.data(), temp_u.v.data(), &temp_u.len, &alpha, nthreads, 0, nullptr);
+            if (rc) return -2;
+            KM[i * nl + i] = alpha;
+            for (int j = 0; j < i; ++j) {
+                double d = oracle_dot(K[j].k.data(), K[j].v.data(), K[j].len,
+                                      temp_u.k.data(), temp_u.v.data(), temp_u.len);
+                KM[j * nl + i] = KM[i * nl + j] = d;
+            }
+            Vec t2(hpsi_cap);
+            t2.len = oracle_add_two(temp_u.k.data(), temp_u.v.data(), temp_u.len, v.k.data(), v.v.data(), v.len,
+                                    -beta, t2.cap, t2.k.data(), t2.v.data());
+            u.copy_from(t2);
+            v.copy_from(v_next);
+            K[i].copy_from(v);
+        }
+        for (int i = 0; i < nl; ++i)
+            for (int j = i; j < nl; ++j) {
+                double d = oracle_dot(K[i].k.data(), K[i].v.data(), K[i].len, K[j].k.data(), K[j].v.data(), K[j].len);
+                S[i * nl + j] = S[j * nl + i] = d;
+            }
+        std::vector<double> ev, evec;
+        if (gsaes(nl, KM, S, ev, evec)) return -3;
+        int nearest = 0;
+        double maxGap = 0;
+        for (int i = 0; i < nl; ++i) {
+            double gap = std::fabs(nearestE - ev[i]);
+            if (i == 0 || gap < maxGap) { maxGap = gap; nearest = i; }
+        }
+        nearestE = ev[nearest];
+        std::vector<double> c(nl);
+        for (int i = 0; i < nl; ++i) c[i] = evec[i * nl + nearest];
+        std::vector<int64_t> offs(nl + 1, 0);
+        for (int i = 0; i < nl; ++i) offs[i + 1] = offs[i] + K[i].len;
+        std::vector<uint64_t> ck(offs[nl]);
+        std::vector<double> cv(offs[nl]);
+        for (int i = 0; i < nl; ++i) {
+            std::copy(K[i].k.begin(), K[i].k.begin() + K[i].len, ck.begin() + offs[i]);
+            std::copy(K[i].v.begin(), K[i].v.begin() + K[i].len, cv.begin() + offs[i]);
+        }
+        v.len = oracle_lincomb(ck.data(), cv.data(), offs.data(), nl, c.data(), v.cap, v.k.data(), v.v.data(), nullptr);
+        double et = oracle_energy(H, v.k.data(), v.v.data(), v.len, nthreads);
+        ritz[res] = ev[nearest];
+        e_trunc[res] = et;
+        std::fill(KM.begin(), KM.end(), 0.0);
+        for (auto& kk : K) kk.len = 0;
+        if (previousE - ev[nearest] < deltaE) { ++res; break; }
+        previousE = ev[nearest];
+    }
+    *e_final = oracle_energy(H, v.k.data(), v.v.data(), v.len, nthreads);
+    for (int64_t t = 0; t < v.len; ++t) { out_k[t] = v.k[t]; out_v[t] = v.v[t]; }
+    *out_len = v.len;
+    return res;
+}
+
+}  // extern "C"
