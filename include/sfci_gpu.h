@@ -1,0 +1,162 @@
+/*
+ * sfci_gpu.h -- C-ABI of the MI355X (gfx950) SFCI H.v library, libsfci_hip.so.
+ *
+ * The reference (wlj89/SFCI) is one C++ binary with no plugin or FFI layer; its hot
+ * path is the in-process call
+ *     void SpMVMtply_bucket(dynVec<u64,double>& in, dynVec<u64,double>& out,
+ *                           double lambda, double coeff, double& diag);
+ * (declared src/all.h:642, defined src/functions.cpp:511-804) made by the Lanczos
+ * driver (src/sparse_solver.cpp:306, :358), plus the dynVec methods the driver uses
+ * (src/all.h:150-625) and calc_energy (src/functions.cpp:1272-1313).  This header is
+ * the drop-in for that call boundary: plain pointers and sizes, device-resident sparse
+ * vectors, integer return codes.
+ *
+ * Conventions
+ *   - Every function returns SFCI_OK (0) or a negative SFCI_E_* code; the message of
+ *     the last failure is available from sfci_last_error().  No C++ exception crosses
+ *     the ABI.
+ *   - One context = one GPU = one HIP stream.  A context is not thread-safe.
+ *   - Sparse vectors hold (key, coefficient) pairs sorted by ascending key, like
+ *     dynVec; keys are determinant bit strings (bit p = spin orbital p, the numbering
+ *     of the reference's _IN files after ReadElements' "-2", src/functions.cpp:329-352).
+ *   - Every |c| truncation keeps the largest |c|; among equal |c| the smaller key.
+ *   - Sums are exact (128-bit fixed point, unit 2^-84) and rounded once, so results do
+ *     not depend on thread counts or summation order.
+ */
+#ifndef SFCI_GPU_H
+#define SFCI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFCI_OK 0
+#define SFCI_E_ARG (-1)      /* invalid argument, malformed input file or vector */
+#define SFCI_E_HIP (-2)      /* HIP runtime error (allocation, launch) */
+#define SFCI_E_OVERFLOW (-3) /* device staging overflow that regrowing did not fix */
+#define SFCI_E_CAPACITY (-4) /* problem does not fit the workspace limit */
+#define SFCI_E_RANGE (-5)    /* a term left the fixed-point range (|x| >= 2^42) */
+#define SFCI_E_IO (-6)       /* integral file could not be opened */
+
+typedef struct sfci_ctx sfci_ctx;
+typedef struct sfci_vec sfci_vec;
+
+typedef struct {
+    int n_so;             /* spin orbitals (num_orb) */
+    int n_elec;           /* electrons (num_e), half of each spin */
+    uint64_t string_dim;  /* C(n_so/2, n_elec/2): the reference's blockDim (src/main.cpp:183) */
+    uint64_t n_conn;      /* connections per determinant including the diagonal (U + 1) */
+    double e_core;        /* constant block of the _IN file */
+} sfci_info_t;
+
+typedef struct {
+    uint64_t n_src;         /* input determinants used (top-Nd of the input) */
+    uint64_t n_candidates;  /* connections generated = n_src * n_conn */
+    uint64_t n_unique;      /* distinct target determinants before the cut */
+    uint64_t n_out;         /* output length */
+    uint64_t n_passes;      /* target-range passes */
+    double ms_generate, ms_sort, ms_reduce, ms_select, ms_total;
+} sfci_hv_stats_t;
+
+/* Context from a reference _IN integral file: replaces load_integrals/ReadElements
+ * (src/functions.cpp:297-418) and the parameter block of main (src/main.cpp:183-223).
+ * n_so <= 64, n_elec even.  device: HIP device ordinal. */
+int sfci_create(const char* in_path, int n_so, int n_elec, int device, sfci_ctx** out);
+
+/* Same from entries already in memory, with the _IN numbering (coeff, m, n, p, q; the
+ * constant as 0 0 0 0, one-body as m n 0 0): idx holds 4 int32 per entry. */
+int sfci_create_from_entries(int n_so, int n_elec, int64_t n_entries, const double* coeff, const int32_t* idx,
+                             int device, sfci_ctx** out);
+
+void sfci_destroy(sfci_ctx* ctx);
+const char* sfci_last_error(const sfci_ctx* ctx);
+int sfci_info(const sfci_ctx* ctx, sfci_info_t* info);
+
+/* |y| > cut filter of H.v output; default 1e-5 (SpMSpVFinalResCut, src/main.cpp:196). */
+int sfci_set_cut(sfci_ctx* ctx, double cut);
+/* Device bytes the H.v candidate buffers of one target pass may use. */
+int sfci_set_workspace(sfci_ctx* ctx, uint64_t bytes);
+int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
+int sfci_sync(sfci_ctx* ctx);
+
+/* Device sparse vector with fixed capacity: dynVec(cap) (src/all.h:170-179). */
+int sfci_vec_create(sfci_ctx* ctx, int64_t cap, sfci_vec** out);
+void sfci_vec_destroy(sfci_vec* v);
+int64_t sfci_vec_len(const sfci_vec* v);
+int64_t sfci_vec_cap(const sfci_vec* v);
+/* Host -> device; keys strictly ascending, n <= cap. */
+int sfci_vec_upload(sfci_vec* v, const uint64_t* keys, const double* vals, int64_t n);
+/* Device -> host; copies min(len, max_n) entries. */
+int sfci_vec_download(const sfci_vec* v, uint64_t* keys, double* vals, int64_t max_n, int64_t* n_out);
+/* Device pointers of the key and value arrays (for zero-copy interop). */
+int sfci_vec_device_ptrs(sfci_vec* v, uint64_t** keys, double** vals);
+/* Declare the length after writing through the device pointers (keys must be sorted). */
+int sfci_vec_set_len(sfci_vec* v, int64_t len);
+/* dynVec::naiveCopy (src/all.h:610-623): dst <- src, src.len <= dst.cap. */
+int sfci_vec_copy(sfci_vec* dst, const sfci_vec* src);
+/* dynVec::setZeroVec (src/all.h:244-247). */
+int sfci_vec_clear(sfci_vec* v);
+
+/* SpMVMtply_bucket (src/functions.cpp:511-804):
+ *   kept = top-nd of in by |c|
+ *   out  = top-{out.cap} of { t : |y_t| > cut }, y = coeff * (H - lambda) * kept, key-sorted
+ *   diag = <kept| coeff (H - lambda) |kept>, exact, before the cut. */
+int sfci_hv(sfci_ctx* ctx, const sfci_vec* in, sfci_vec* out, int64_t nd, double lambda, double coeff,
+            double* diag);
+
+/* calc_energy (src/functions.cpp:1272-1313): <v|H|v>. */
+int sfci_energy(sfci_ctx* ctx, const sfci_vec* v, double* e);
+
+/* DiagCal_bit (src/bitstr.cpp:433-476): <D|H|D> of one determinant. */
+int sfci_diag_element(sfci_ctx* ctx, uint64_t det, double* e);
+
+/* dynVec::dot (src/all.h:317-351). */
+int sfci_dot(sfci_ctx* ctx, const sfci_vec* a, const sfci_vec* b, double* out);
+/* dynVec::norm (src/all.h:249-258). */
+int sfci_norm(sfci_ctx* ctx, const sfci_vec* a, double* out);
+/* dynVec::scalarMtply (src/all.h:260-265). */
+int sfci_scale(sfci_ctx* ctx, sfci_vec* a, double s);
+/* dynVec::addTwo (src/all.h:407-480): v <- top-{v.cap}(v + a*w), zeros kept, key-sorted. */
+int sfci_add_two(sfci_ctx* ctx, sfci_vec* v, const sfci_vec* w, double a);
+/* dynVec::shrinkFrom (src/all.h:281-299): dst <- top-{dst.cap}(src); src is left as is. */
+int sfci_shrink_from(sfci_ctx* ctx, sfci_vec* dst, const sfci_vec* src);
+/* Restart recombination (src/sparse_solver.cpp:492-516): out <- top-{out.cap}(sum_i c_i v_i),
+ * duplicates merged.  out may be one of the inputs. */
+int sfci_lincomb(sfci_ctx* ctx, const sfci_vec* const* vecs, const double* c, int n, sfci_vec* out);
+
+/* ---- Ground-state driver ----------------------------------------------------------
+ * sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548) in host C++ on top of
+ * the calls above: truncated Lanczos with full projection onto the stored Krylov
+ * vectors, Nl x Nl generalized eigenproblem, restart from the Ritz vector nearest the
+ * previous Ritz value (the first restart takes the lowest), stop when the Ritz value
+ * drops by less than delta_e. */
+typedef struct {
+    int64_t nd;          /* Nd: determinants kept per Krylov vector */
+    int nl;              /* Nl: Krylov vectors per restart */
+    int num_restart;     /* maximum restarts */
+    int64_t hpsi_cap;    /* capacity of H.v results (HPsiCap, src/main.cpp:194): 60,000,000 */
+    double delta_e;      /* convergence threshold on the Ritz value (src/sparse_solver.cpp:269): 1e-5 */
+    int verbose;         /* 1: print progress lines like the reference's driver */
+} sfci_lanczos_params_t;
+
+typedef struct {
+    int n_restarts;      /* restarts run */
+    double e_final;      /* <v|H|v> of the final vector ("final e:", src/sparse_solver.cpp:545) */
+    double ms_total;     /* wall time of the solve */
+    double ms_hv;        /* wall time spent in sfci_hv */
+    int64_t n_hv;        /* number of sfci_hv calls */
+} sfci_lanczos_result_t;
+
+/* Runs from the single determinant init_det (coefficient 1).  ritz and e_trunc receive
+ * one value per restart (capacity num_restart): the selected Ritz value and <v|H|v>
+ * after truncation.  v_out (optional, capacity >= nd) receives the final vector. */
+int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_lanczos_params_t* params, double* ritz,
+                      double* e_trunc, sfci_lanczos_result_t* result, sfci_vec* v_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFCI_GPU_H */

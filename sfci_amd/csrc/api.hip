@@ -1,0 +1,472 @@
+// C-ABI of libsfci_hip.so: context, integral loading, device vectors, operations.
+#include <cmath>
+#include <cstdio>
+#include <exception>
+
+#include "../../include/sfci_gpu.h"
+#include "internal.h"
+
+using namespace sfci;
+
+struct sfci_ctx {
+    Ctx c;
+};
+struct sfci_vec {
+    Vec v;
+};
+
+namespace {
+
+uint64_t binom_u64(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    if (k > n - k) k = n - k;
+    unsigned __int128 r = 1;
+    for (int i = 0; i < k; ++i) r = r * (unsigned)(n - i) / (unsigned)(i + 1);
+    return (uint64_t)r;
+}
+
+struct Entry {
+    double c;
+    int m, n, p, q;
+};
+
+// ReadElements (src/functions.cpp:297-395): up to 7 blocks of "count" + count lines of
+// "coeff m n p q"; stops early after a block of exactly one line.
+int read_in_file(const char* path, std::vector<Entry>& out) {
+    FILE* fp = fopen(path, "r");
+    if (!fp) return SFCI_E_IO;
+    for (int blk = 0; blk < 7; ++blk) {
+        int num = 0;
+        if (fscanf(fp, "%d", &num) != 1) break;
+        for (int i = 0; i < num; ++i) {
+            Entry e;
+            if (fscanf(fp, "%lf %d %d %d %d", &e.c, &e.m, &e.n, &e.p, &e.q) != 5) {
+                fclose(fp);
+                return SFCI_E_ARG;
+            }
+            out.push_back(e);
+        }
+        if (num == 1) break;
+    }
+    fclose(fp);
+    return SFCI_OK;
+}
+
+void set_err(Ctx* c, const std::string& m) {
+    if (c) c->last_error = m;
+}
+
+int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, sfci_ctx** out, std::string& err) {
+    if (n_so < 2 || n_so > 64 || (n_so & 1) || n_elec < 2 || (n_elec & 1) || n_elec / 2 > n_so / 2 ||
+        n_elec / 2 > SFCI_MAX_K) {
+        err = "unsupported n_so / n_elec (n_so even <= 64, n_elec even)";
+        return SFCI_E_ARG;
+    }
+    const int N = n_so / 2;
+    const int k = n_elec / 2;
+    const size_t N2 = (size_t)N * N, N4 = N2 * N2;
+    std::vector<double> h(2 * N2, 0.0), g(4 * N4, 0.0);
+    double constant = 0.0;
+    for (const Entry& e : ent) {
+        if (e.p == 0 && e.q == 0) {
+            if (e.m == 0 && e.n == 0) {
+                constant = e.c;
+                continue;
+            }
+            int a = e.m - 2, b = e.n - 2;
+            if (a < 0 || b < 0 || a >= n_so || b >= n_so) {
+                err = "one-body index out of range";
+                return SFCI_E_ARG;
+            }
+            if ((a ^ b) & 1) {
+                err = "spin-forbidden one-body entry";
+                return SFCI_E_ARG;
+            }
+            h[(size_t)(a & 1) * N2 + (size_t)(a >> 1) * N + (b >> 1)] += e.c;
+        } else {
+            int a = e.m - 2, b = e.n - 2, p = e.p - 2, q = e.q - 2;
+            if (a < 0 || b < 0 || p < 0 || q < 0 || a >= n_so || b >= n_so || p >= n_so || q >= n_so) {
+                err = "two-body index out of range";
+                return SFCI_E_ARG;
+            }
+            if (((a ^ b) & 1) || ((p ^ q) & 1)) {
+                err = "spin-forbidden two-body entry";
+                return SFCI_E_ARG;
+            }
+            size_t blk = (size_t)(((a & 1) << 1) | (p & 1));
+            g[blk * N4 + (((size_t)(a >> 1) * N + (b >> 1)) * N + (p >> 1)) * N + (q >> 1)] += e.c;
+        }
+    }
+    sfci_ctx* X = new sfci_ctx();
+    Ctx& c = X->c;
+    try {
+        HIPCHK(hipSetDevice(device));
+        c.device = device;
+        HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        c.n_so = n_so;
+        c.n_orb = N;
+        c.n_elec = n_elec;
+        c.h_binom.assign(65 * 33, 0);
+        for (int n = 0; n <= 64; ++n)
+            for (int t = 0; t <= 32; ++t) c.h_binom[n * 33 + t] = binom_u64(n, t);
+        const int v = N - k;
+        std::vector<uint16_t> op, vp;
+        for (int i = 0; i < k; ++i)
+            for (int j = i + 1; j < k; ++j) op.push_back((uint16_t)(i | (j << 8)));
+        for (int i = 0; i < v; ++i)
+            for (int j = i + 1; j < v; ++j) vp.push_back((uint16_t)(i | (j << 8)));
+        if (op.empty()) op.push_back(0);
+        if (vp.empty()) vp.push_back(0);
+        HIPCHK(hipMalloc(&c.d_h, h.size() * 8));
+        HIPCHK(hipMalloc(&c.d_g, g.size() * 8));
+        HIPCHK(hipMalloc(&c.d_binom, c.h_binom.size() * 8));
+        HIPCHK(hipMalloc(&c.d_occ_pairs, op.size() * 2));
+        HIPCHK(hipMalloc(&c.d_vir_pairs, vp.size() * 2));
+        HIPCHK(hipMalloc(&c.d_acc, 16 * 8));
+        HIPCHK(hipMalloc(&c.d_flags, 16));
+        HIPCHK(hipMalloc(&c.d_counter, 8 * 8));
+        HIPCHK(hipMemcpy(c.d_h, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.d_g, g.data(), g.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.d_binom, c.h_binom.data(), c.h_binom.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.d_occ_pairs, op.data(), op.size() * 2, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.d_vir_pairs, vp.data(), vp.size() * 2, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(c.d_acc, 0, 16 * 8));
+        HIPCHK(hipMemset(c.d_flags, 0, 16));
+        HIPCHK(hipMemset(c.d_counter, 0, 8 * 8));
+        HamDev& H = c.ham;
+        H.n_orb = N;
+        H.k = k;
+        H.v = v;
+        H.dim = binom_u64(N, k);
+        H.h = c.d_h;
+        H.g = c.d_g;
+        H.binom = c.d_binom;
+        H.occ_pairs = c.d_occ_pairs;
+        H.vir_pairs = c.d_vir_pairs;
+        H.constant = constant;
+        H.n_sa = (uint64_t)k * v;
+        H.n_opair = (uint32_t)(k * (k - 1) / 2);
+        H.n_vpair = (uint32_t)(v * (v - 1) / 2);
+        H.n_daa = (uint64_t)H.n_opair * H.n_vpair;
+        H.n_dab = H.n_sa * H.n_sa;
+        H.n_conn = 1 + 2 * H.n_sa + 2 * H.n_daa + H.n_dab;
+        if (H.n_vpair == 0) H.n_vpair = 1;   // divisor guard; n_daa is 0 then
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        c.ws_limit = (size_t)(free_b * 0.55);
+    } catch (const HipError& e) {
+        err = e.what();
+        delete X;
+        return e.code;
+    }
+    *out = X;
+    return SFCI_OK;
+}
+
+template <class F>
+int guarded(Ctx* c, F&& f) {
+    try {
+        if (c) HIPCHK(hipSetDevice(c->device));
+        f();
+        return SFCI_OK;
+    } catch (const HipError& e) {
+        set_err(c, e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(c, "host allocation failed");
+        return SFCI_E_HIP;
+    } catch (const std::exception& e) {
+        set_err(c, e.what());
+        return SFCI_E_ARG;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfci_create(const char* in_path, int n_so, int n_elec, int device, sfci_ctx** out) {
+    if (!in_path || !out) return SFCI_E_ARG;
+    *out = nullptr;
+    std::vector<Entry> ent;
+    int rc = read_in_file(in_path, ent);
+    if (rc) return rc;
+    std::string err;
+    rc = build_ctx(n_so, n_elec, ent, device, out, err);
+    if (rc) fprintf(stderr, "sfci_create: %s\n", err.c_str());
+    return rc;
+}
+
+int sfci_create_from_entries(int n_so, int n_elec, int64_t n_entries, const double* coeff, const int32_t* idx,
+                             int device, sfci_ctx** out) {
+    if (!out || n_entries < 0 || (n_entries && (!coeff || !idx))) return SFCI_E_ARG;
+    *out = nullptr;
+    std::vector<Entry> ent(n_entries);
+    for (int64_t i = 0; i < n_entries; ++i)
+        ent[i] = Entry{coeff[i], idx[4 * i], idx[4 * i + 1], idx[4 * i + 2], idx[4 * i + 3]};
+    std::string err;
+    int rc = build_ctx(n_so, n_elec, ent, device, out, err);
+    if (rc) fprintf(stderr, "sfci_create_from_entries: %s\n", err.c_str());
+    return rc;
+}
+
+void sfci_destroy(sfci_ctx* X) {
+    if (!X) return;
+    Ctx& c = X->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    DevBuf* bufs[] = {&c.cand_k[0], &c.cand_k[1], &c.cand_v[0], &c.cand_v[1], &c.src_cnt, &c.src_off,
+                      &c.src_k, &c.src_v, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
+                      &c.scratch_off, &c.scratch_small, &c.rowhist};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& b : c.scratch_scan) b.release();
+    void* ptrs[] = {c.d_h, c.d_g, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_acc, c.d_flags, c.d_counter};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete X;
+}
+
+const char* sfci_last_error(const sfci_ctx* X) { return X ? X->c.last_error.c_str() : "null context"; }
+
+int sfci_info(const sfci_ctx* X, sfci_info_t* info) {
+    if (!X || !info) return SFCI_E_ARG;
+    const Ctx& c = X->c;
+    info->n_so = c.n_so;
+    info->n_elec = c.n_elec;
+    info->string_dim = c.ham.dim;
+    info->n_conn = c.ham.n_conn;
+    info->e_core = c.ham.constant;
+    return SFCI_OK;
+}
+
+int sfci_set_cut(sfci_ctx* X, double cut) {
+    if (!X || !(cut >= 0.0)) return SFCI_E_ARG;
+    X->c.cut = cut;
+    return SFCI_OK;
+}
+
+int sfci_set_workspace(sfci_ctx* X, uint64_t bytes) {
+    if (!X || bytes < (1u << 20)) return SFCI_E_ARG;
+    X->c.ws_limit = bytes;
+    return SFCI_OK;
+}
+
+int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
+    if (!X || !s) return SFCI_E_ARG;
+    const auto& t = X->c.st;
+    s->n_src = t.n_src;
+    s->n_candidates = t.n_candidates;
+    s->n_unique = t.n_unique;
+    s->n_out = t.n_out;
+    s->n_passes = t.n_passes;
+    s->ms_generate = t.ms_generate;
+    s->ms_sort = t.ms_sort;
+    s->ms_reduce = t.ms_reduce;
+    s->ms_select = t.ms_select;
+    s->ms_total = t.ms_total;
+    return SFCI_OK;
+}
+
+int sfci_sync(sfci_ctx* X) {
+    if (!X) return SFCI_E_ARG;
+    return guarded(&X->c, [&] { HIPCHK(hipStreamSynchronize(X->c.stream)); });
+}
+
+int sfci_vec_create(sfci_ctx* X, int64_t cap, sfci_vec** out) {
+    if (!X || !out || cap < 0) return SFCI_E_ARG;
+    *out = nullptr;
+    sfci_vec* V = new sfci_vec();
+    V->v.ctx = &X->c;
+    V->v.cap = cap;
+    int rc = guarded(&X->c, [&] {
+        size_t n = cap > 0 ? (size_t)cap : 1;
+        HIPCHK(hipMalloc(&V->v.k, n * 8));
+        HIPCHK(hipMalloc(&V->v.v, n * 8));
+    });
+    if (rc) {
+        if (V->v.k) (void)hipFree(V->v.k);
+        if (V->v.v) (void)hipFree(V->v.v);
+        delete V;
+        return rc;
+    }
+    *out = V;
+    return SFCI_OK;
+}
+
+void sfci_vec_destroy(sfci_vec* V) {
+    if (!V) return;
+    (void)hipSetDevice(V->v.ctx->device);
+    (void)hipStreamSynchronize(V->v.ctx->stream);
+    if (V->v.k) (void)hipFree(V->v.k);
+    if (V->v.v) (void)hipFree(V->v.v);
+    delete V;
+}
+
+int64_t sfci_vec_len(const sfci_vec* V) { return V ? V->v.len : -1; }
+int64_t sfci_vec_cap(const sfci_vec* V) { return V ? V->v.cap : -1; }
+
+int sfci_vec_upload(sfci_vec* V, const uint64_t* keys, const double* vals, int64_t n) {
+    if (!V || n < 0 || n > V->v.cap || (n && (!keys || !vals))) return SFCI_E_ARG;
+    Ctx* c = V->v.ctx;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && keys[i] <= keys[i - 1]) {
+            c->last_error = "sfci_vec_upload: keys not strictly ascending";
+            return SFCI_E_ARG;
+        }
+        uint64_t d = keys[i];
+        if (popc64(d & 0x5555555555555555ull) != c->ham.k || popc64(d & 0xaaaaaaaaaaaaaaaaull) != c->ham.k ||
+            (c->n_so < 64 && (d >> c->n_so))) {
+            c->last_error = "sfci_vec_upload: key " + std::to_string(d) + " does not match n_so / n_elec";
+            return SFCI_E_ARG;
+        }
+    }
+    return guarded(c, [&] {
+        if (n) {
+            HIPCHK(hipMemcpyAsync(V->v.k, keys, n * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(V->v.v, vals, n * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        V->v.len = n;
+    });
+}
+
+int sfci_vec_download(const sfci_vec* V, uint64_t* keys, double* vals, int64_t max_n, int64_t* n_out) {
+    if (!V || max_n < 0) return SFCI_E_ARG;
+    Ctx* c = V->v.ctx;
+    int64_t n = V->v.len < max_n ? V->v.len : max_n;
+    if (n && (!keys || !vals)) return SFCI_E_ARG;
+    return guarded(c, [&] {
+        if (n) {
+            HIPCHK(hipMemcpyAsync(keys, V->v.k, n * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(vals, V->v.v, n * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (n_out) *n_out = n;
+    });
+}
+
+int sfci_vec_device_ptrs(sfci_vec* V, uint64_t** keys, double** vals) {
+    if (!V || !keys || !vals) return SFCI_E_ARG;
+    *keys = V->v.k;
+    *vals = V->v.v;
+    return SFCI_OK;
+}
+
+int sfci_vec_set_len(sfci_vec* V, int64_t len) {
+    if (!V || len < 0 || len > V->v.cap) return SFCI_E_ARG;
+    V->v.len = len;
+    return SFCI_OK;
+}
+
+int sfci_vec_copy(sfci_vec* D, const sfci_vec* S) {
+    if (!D || !S || D->v.ctx != S->v.ctx || S->v.len > D->v.cap) return SFCI_E_ARG;
+    Ctx* c = D->v.ctx;
+    return guarded(c, [&] {
+        dev_copy(c, D->v.k, D->v.v, S->v.k, S->v.v, (size_t)S->v.len);
+        D->v.len = S->v.len;
+    });
+}
+
+int sfci_vec_clear(sfci_vec* V) {
+    if (!V) return SFCI_E_ARG;
+    V->v.len = 0;
+    return SFCI_OK;
+}
+
+int sfci_hv(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, int64_t nd, double lambda, double coeff, double* diag) {
+    if (!X || !in || !out || nd < 0 || in == out || in->v.ctx != &X->c || out->v.ctx != &X->c) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        double d = 0.0;
+        size_t n = dev_hv(c, in->v.k, in->v.v, (size_t)in->v.len, (size_t)nd, lambda, coeff, (size_t)out->v.cap,
+                          out->v.k, out->v.v, &d);
+        out->v.len = (int64_t)n;
+        if (diag) *diag = d;
+    });
+}
+
+int sfci_energy(sfci_ctx* X, const sfci_vec* v, double* e) {
+    if (!X || !v || !e || v->v.ctx != &X->c) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        double d = 0.0;
+        dev_hv(c, v->v.k, v->v.v, (size_t)v->v.len, (size_t)v->v.len, 0.0, 1.0, 0, nullptr, nullptr, &d);
+        *e = d;
+    });
+}
+
+int sfci_diag_element(sfci_ctx* X, uint64_t det, double* e) {
+    if (!X || !e) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    if (popc64(det & 0x5555555555555555ull) != c->ham.k || popc64(det & 0xaaaaaaaaaaaaaaaaull) != c->ham.k ||
+        (c->n_so < 64 && (det >> c->n_so))) {
+        c->last_error = "determinant does not match n_so / n_elec";
+        return SFCI_E_ARG;
+    }
+    return guarded(c, [&] {
+        uint64_t* k = c->src_k.get<uint64_t>(1);
+        double* v = c->src_v.get<double>(1);
+        double one = 1.0;
+        HIPCHK(hipMemcpyAsync(k, &det, 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(v, &one, 8, hipMemcpyHostToDevice, c->stream));
+        // the input buffers are re-used by dev_hv only when n_in > nd, which is not the case here
+        double d = 0.0;
+        dev_hv(c, k, v, 1, 1, 0.0, 1.0, 0, nullptr, nullptr, &d);
+        *e = d;
+    });
+}
+
+int sfci_dot(sfci_ctx* X, const sfci_vec* a, const sfci_vec* b, double* out) {
+    if (!X || !a || !b || !out) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] { *out = dev_dot(c, a->v.k, a->v.v, (size_t)a->v.len, b->v.k, b->v.v, (size_t)b->v.len); });
+}
+
+int sfci_norm(sfci_ctx* X, const sfci_vec* a, double* out) {
+    if (!X || !a || !out) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] { *out = dev_norm(c, a->v.v, (size_t)a->v.len); });
+}
+
+int sfci_scale(sfci_ctx* X, sfci_vec* a, double s) {
+    if (!X || !a) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] { dev_scale(c, a->v.v, (size_t)a->v.len, s); });
+}
+
+int sfci_add_two(sfci_ctx* X, sfci_vec* v, const sfci_vec* w, double a) {
+    if (!X || !v || !w || v == w) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        size_t n = dev_add_two(c, v->v.k, v->v.v, (size_t)v->v.len, w->v.k, w->v.v, (size_t)w->v.len, a,
+                               (size_t)v->v.cap, v->v.k, v->v.v);
+        v->v.len = (int64_t)n;
+    });
+}
+
+int sfci_shrink_from(sfci_ctx* X, sfci_vec* dst, const sfci_vec* src) {
+    if (!X || !dst || !src || dst == src) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        size_t n = select_top(c, src->v.k, src->v.v, (size_t)src->v.len, (size_t)dst->v.cap, dst->v.k, dst->v.v);
+        dst->v.len = (int64_t)n;
+    });
+}
+
+int sfci_lincomb(sfci_ctx* X, const sfci_vec* const* vecs, const double* coef, int n, sfci_vec* out) {
+    if (!X || !out || n < 0 || (n && (!vecs || !coef))) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    std::vector<const Vec*> vs(n);
+    for (int i = 0; i < n; ++i) {
+        if (!vecs[i]) return SFCI_E_ARG;
+        vs[i] = &vecs[i]->v;
+    }
+    return guarded(c, [&] {
+        size_t m = dev_lincomb(c, vs.data(), coef, n, (size_t)out->v.cap, out->v.k, out->v.v);
+        out->v.len = (int64_t)m;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,611 @@
+// H.v on the GPU: SpMVMtply_bucket (src/functions.cpp:511-804) with 64-bit keys, FP64.
+//
+// Pipeline for one call:
+//   1. input  : top-Nd of the input by |c| when it holds more than Nd entries.
+//   2. plan   : target space split into passes by ranges of the target's spin-0
+//               string rank (the reference's bucket variable, src/bitstr.cpp:219), so
+//               that one pass's candidates fit the workspace.
+//   3. per pass
+//      count    per-source number of connections landing in the pass range
+//      generate one workgroup per source: every connection (diagonal, singles,
+//               same-spin and opposite-spin doubles) evaluated once with its full
+//               Slater-Condon element; record = (dense target index << 1 | is_diag,
+//               H_ij * coeff * c_j), written at deterministic offsets
+//      sort     stable LSD radix sort on the pass-local dense index
+//      reduce   exact fixed-point sum per target; diag += y_t * c_t for targets in
+//               the input; targets with |y| > cut appended to the staging list
+//   4. output : top out_cap by (|y| desc, key asc), sorted by key.
+#include "internal.h"
+
+namespace sfci {
+
+// spin-orbital integral lookup (m n | p q); zero for spin-forbidden index sets
+__device__ __forceinline__ double g_so(const HamDev& H, int m, int n, int p, int q) {
+    if (((m ^ n) & 1) | ((p ^ q) & 1)) return 0.0;
+    size_t N = (size_t)H.n_orb;
+    size_t blk = (size_t)(((m & 1) << 1) | (p & 1));
+    return H.g[blk * N * N * N * N + (((size_t)(m >> 1) * N + (n >> 1)) * N + (p >> 1)) * N + (q >> 1)];
+}
+
+__device__ __forceinline__ double h_so(const HamDev& H, int m, int n) {
+    if ((m ^ n) & 1) return 0.0;
+    size_t N = (size_t)H.n_orb;
+    return H.h[(size_t)(m & 1) * N * N + (size_t)(m >> 1) * N + (n >> 1)];
+}
+
+struct SrcLds {
+    uint64_t key;
+    double temp;        // coeff * c_j
+    double diag_val;    // (E_jj - lambda) * temp
+    uint64_t rank[2];
+    uint32_t str[2];
+    uint8_t occ[2][SFCI_MAX_K];   // spatial orbitals, ascending
+    uint8_t vir[2][SFCI_MAX_ORB];
+    uint8_t occ_so[2 * SFCI_MAX_K];   // all occupied spin orbitals, ascending
+    int n_occ_so;
+};
+
+__device__ void load_source(const HamDev& H, SrcLds& s, uint64_t key, double temp, double lambda, uint32_t* flags) {
+    // called by thread 0
+    s.key = key;
+    s.temp = temp;
+    for (int sp = 0; sp < 2; ++sp) {
+        uint32_t st = spin_string(key, sp);
+        s.str[sp] = st;
+        s.rank[sp] = colex_rank(st, H.binom);
+        int a = 0, b = 0;
+        for (int i = 0; i < H.n_orb; ++i) {
+            if (st >> i & 1u) s.occ[sp][a++] = (uint8_t)i;
+            else s.vir[sp][b++] = (uint8_t)i;
+        }
+    }
+    int n = 0;
+    for (int p = 0; p < 2 * H.n_orb; ++p)
+        if (key >> p & 1ull) s.occ_so[n++] = (uint8_t)p;
+    s.n_occ_so = n;
+    // diagonal element, exact: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)]
+    fx128 acc = fx_from(H.constant);
+    for (int a = 0; a < n; ++a) acc = fx_add(acc, fx_from(h_so(H, s.occ_so[a], s.occ_so[a])));
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            if (a == b) continue;
+            int m = s.occ_so[a], p = s.occ_so[b];
+            acc = fx_add(acc, fx_from(0.5 * g_so(H, m, m, p, p)));
+            acc = fx_add(acc, fx_neg(fx_from(0.5 * g_so(H, m, p, p, m))));
+        }
+    double e = fx_to(acc);
+    double dv = (e - lambda) * temp;
+    if (!fx_in_range(dv)) atomicOr(flags, FLAG_RANGE);
+    s.diag_val = dv;
+}
+
+// <D'|H|D>, D' = D with spin orbital n -> m
+__device__ __forceinline__ double single_elem(const HamDev& H, const SrcLds& s, int n, int m) {
+    fx128 acc = fx_from(h_so(H, m, n));
+    for (int a = 0; a < s.n_occ_so; ++a) {
+        int p = s.occ_so[a];
+        if (p == n) continue;
+        acc = fx_add(acc, fx_from(g_so(H, m, n, p, p)));
+        acc = fx_add(acc, fx_neg(fx_from(g_so(H, m, p, p, n))));
+    }
+    return sign_between(s.key, m, n) * fx_to(acc);
+}
+
+// <D''|H|D>, D'' = D - {n,q} + {x,y}, n < q, x < y (spin orbitals)
+__device__ __forceinline__ double double_elem(const HamDev& H, uint64_t d, int n, int q, int x, int y) {
+    const uint64_t one = 1;
+    uint64_t d1 = d - (one << q) + (one << y);
+    double s1 = sign_between(d, q, y) * sign_between(d1, n, x);
+    uint64_t d2 = d - (one << q) + (one << x);
+    double s2 = sign_between(d, q, x) * sign_between(d2, n, y);
+    return s1 * g_so(H, x, n, y, q) + s2 * g_so(H, y, n, x, q);
+}
+
+struct Cand {
+    uint64_t target;       // determinant key
+    uint64_t dense;        // rank0 * dim + rank1
+    uint64_t row;          // rank0
+    double val;
+    uint32_t tag;
+};
+
+// Connection e (0 <= e < n_conn) of the source in LDS.  Only computes the value when
+// `want_val`.
+__device__ __forceinline__ void connection(const HamDev& H, const SrcLds& s, uint64_t e, bool want_val, Cand& c) {
+    const uint64_t one = 1;
+    const uint64_t d = s.key;
+    c.tag = 0;
+    if (e == 0) {
+        c.target = d;
+        c.row = s.rank[0];
+        c.dense = s.rank[0] * H.dim + s.rank[1];
+        c.val = s.diag_val;
+        c.tag = 1;
+        return;
+    }
+    e -= 1;
+    const uint64_t v = (uint64_t)H.v;
+    if (e < 2 * H.n_sa) {
+        int sp = e >= H.n_sa ? 1 : 0;
+        uint64_t idx = e - (uint64_t)sp * H.n_sa;
+        int ii = (int)(idx / v), aa = (int)(idx - (uint64_t)ii * v);
+        int i = s.occ[sp][ii], a = s.vir[sp][aa];
+        int n = 2 * i + sp, m = 2 * a + sp;
+        c.target = d - (one << n) + (one << m);
+        uint32_t st = s.str[sp] ^ (1u << i) ^ (1u << a);
+        uint64_t r = colex_rank(st, H.binom);
+        uint64_t r0 = sp == 0 ? r : s.rank[0], r1 = sp == 0 ? s.rank[1] : r;
+        c.row = r0;
+        c.dense = r0 * H.dim + r1;
+        c.val = want_val ? single_elem(H, s, n, m) * s.temp : 0.0;
+        return;
+    }
+    e -= 2 * H.n_sa;
+    if (e < 2 * H.n_daa) {
+        int sp = e >= H.n_daa ? 1 : 0;
+        uint64_t idx = e - (uint64_t)sp * H.n_daa;
+        uint64_t ij = idx / H.n_vpair, ab = idx - ij * H.n_vpair;
+        uint16_t op = H.occ_pairs[ij], vp = H.vir_pairs[ab];
+        int i = s.occ[sp][op & 0xff], j = s.occ[sp][op >> 8];
+        int a = s.vir[sp][vp & 0xff], b = s.vir[sp][vp >> 8];
+        int n = 2 * i + sp, q = 2 * j + sp, x = 2 * a + sp, y = 2 * b + sp;
+        c.target = d - (one << n) - (one << q) + (one << x) + (one << y);
+        uint32_t st = s.str[sp] ^ (1u << i) ^ (1u << j) ^ (1u << a) ^ (1u << b);
+        uint64_t r = colex_rank(st, H.binom);
+        uint64_t r0 = sp == 0 ? r : s.rank[0], r1 = sp == 0 ? s.rank[1] : r;
+        c.row = r0;
+        c.dense = r0 * H.dim + r1;
+        c.val = want_val ? double_elem(H, d, n, q, x, y) * s.temp : 0.0;
+        return;
+    }
+    e -= 2 * H.n_daa;
+    {
+        uint64_t ia = e / H.n_sa, jb = e - ia * H.n_sa;
+        int ii = (int)(ia / v), aa = (int)(ia - (uint64_t)ii * v);
+        int jj = (int)(jb / v), bb = (int)(jb - (uint64_t)jj * v);
+        int i = s.occ[0][ii], a = s.vir[0][aa], j = s.occ[1][jj], b = s.vir[1][bb];
+        int ni = 2 * i, xa = 2 * a, nj = 2 * j + 1, yb = 2 * b + 1;
+        c.target = d - (one << ni) - (one << nj) + (one << xa) + (one << yb);
+        uint64_t r0 = colex_rank(s.str[0] ^ (1u << i) ^ (1u << a), H.binom);
+        uint64_t r1 = colex_rank(s.str[1] ^ (1u << j) ^ (1u << b), H.binom);
+        c.row = r0;
+        c.dense = r0 * H.dim + r1;
+        if (want_val) {
+            int n = ni < nj ? ni : nj, q = ni < nj ? nj : ni;
+            int x = xa < yb ? xa : yb, y = xa < yb ? yb : xa;
+            c.val = double_elem(H, d, n, q, x, y) * s.temp;
+        } else {
+            c.val = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Per-source count of connections whose target row lies in [row_lo, row_hi).
+__global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev H, const uint64_t* __restrict__ sk, size_t n_src,
+                                                      uint64_t row_lo, uint64_t row_hi, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t s_str;
+    __shared__ uint8_t s_occ[SFCI_MAX_K], s_vir[SFCI_MAX_ORB];
+    __shared__ uint32_t s_tot;
+    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+        if (threadIdx.x == 0) {
+            uint32_t st = spin_string(sk[j], 0);
+            s_str = st;
+            int a = 0, b = 0;
+            for (int i = 0; i < H.n_orb; ++i) {
+                if (st >> i & 1u) s_occ[a++] = (uint8_t)i;
+                else s_vir[b++] = (uint8_t)i;
+            }
+            uint64_t r = colex_rank(st, H.binom);
+            // diagonal + spin-1 singles + spin-1 doubles keep the row
+            s_tot = (r >= row_lo && r < row_hi) ? (uint32_t)(1 + H.n_sa + H.n_daa) : 0u;
+        }
+        __syncthreads();
+        uint32_t my = 0;
+        const uint64_t n_alpha = H.n_sa + H.n_daa;
+        for (uint64_t e = threadIdx.x; e < n_alpha; e += SFCI_BLOCK) {
+            uint32_t st;
+            uint32_t w;
+            if (e < H.n_sa) {
+                int ii = (int)(e / H.v), aa = (int)(e - (uint64_t)ii * H.v);
+                st = s_str ^ (1u << s_occ[ii]) ^ (1u << s_vir[aa]);
+                w = (uint32_t)(1 + H.n_sa);   // the single + its opposite-spin doubles
+            } else {
+                uint64_t idx = e - H.n_sa;
+                uint64_t ij = idx / H.n_vpair, ab = idx - ij * H.n_vpair;
+                uint16_t op = H.occ_pairs[ij], vp = H.vir_pairs[ab];
+                st = s_str ^ (1u << s_occ[op & 0xff]) ^ (1u << s_occ[op >> 8]) ^ (1u << s_vir[vp & 0xff]) ^
+                     (1u << s_vir[vp >> 8]);
+                w = 1;
+            }
+            uint64_t r = colex_rank(st, H.binom);
+            if (r >= row_lo && r < row_hi) my += w;
+        }
+        for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) my += __shfl_down(my, off, SFCI_WAVE);
+        if ((threadIdx.x & (SFCI_WAVE - 1)) == 0) atomicAdd(&s_tot, my);
+        __syncthreads();
+        if (threadIdx.x == 0) cnt[j] = s_tot;
+        __syncthreads();
+    }
+}
+
+// Histogram of connection counts over target-row groups (row >> grp_shift), for pass
+// planning.  Each thread handles one source at a time.
+__global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev H, const uint64_t* __restrict__ sk,
+                                                              size_t n_src, int grp_shift, int n_grp,
+                                                              unsigned long long* __restrict__ hist) {
+    extern __shared__ unsigned long long s_h[];
+    for (int i = threadIdx.x; i < n_grp; i += SFCI_BLOCK) s_h[i] = 0;
+    __syncthreads();
+    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+        uint32_t st = spin_string(sk[j], 0);
+        uint8_t occ[SFCI_MAX_K], vir[SFCI_MAX_ORB];
+        int a = 0, b = 0;
+        for (int i = 0; i < H.n_orb; ++i) {
+            if (st >> i & 1u) occ[a++] = (uint8_t)i;
+            else vir[b++] = (uint8_t)i;
+        }
+        if (threadIdx.x == 0)
+            atomicAdd(&s_h[colex_rank(st, H.binom) >> grp_shift], (unsigned long long)(1 + H.n_sa + H.n_daa));
+        const uint64_t n_alpha = H.n_sa + H.n_daa;
+        for (uint64_t e = threadIdx.x; e < n_alpha; e += SFCI_BLOCK) {
+            uint32_t s2;
+            unsigned long long w;
+            if (e < H.n_sa) {
+                int ii = (int)(e / H.v), aa = (int)(e - (uint64_t)ii * H.v);
+                s2 = st ^ (1u << occ[ii]) ^ (1u << vir[aa]);
+                w = 1 + H.n_sa;
+            } else {
+                uint64_t idx = e - H.n_sa;
+                uint64_t ij = idx / H.n_vpair, ab = idx - ij * H.n_vpair;
+                uint16_t op = H.occ_pairs[ij], vp = H.vir_pairs[ab];
+                s2 = st ^ (1u << occ[op & 0xff]) ^ (1u << occ[op >> 8]) ^ (1u << vir[vp & 0xff]) ^ (1u << vir[vp >> 8]);
+                w = 1;
+            }
+            atomicAdd(&s_h[colex_rank(s2, H.binom) >> grp_shift], w);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_grp; i += SFCI_BLOCK)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Candidate generation for the rows [row_lo, row_hi): one workgroup per source.
+// off == nullptr means every connection is in range and source j starts at j*n_conn.
+__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_t* __restrict__ sk,
+                                                         const double* __restrict__ sv, size_t n_src, double lambda,
+                                                         double coeff, uint64_t row_lo, uint64_t row_hi,
+                                                         const uint64_t* __restrict__ off, uint64_t* __restrict__ ok,
+                                                         double* __restrict__ ov, uint32_t* flags) {
+    __shared__ SrcLds s;
+    __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
+    const uint64_t base_dense = row_lo * H.dim;
+    const bool all_in = (off == nullptr);
+    bool range_bad = false;
+    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+        if (tid == 0) load_source(H, s, sk[j], coeff * sv[j], lambda, flags);
+        __syncthreads();
+        uint64_t pos = all_in ? (uint64_t)j * H.n_conn : off[j];
+        for (uint64_t e0 = 0; e0 < H.n_conn; e0 += SFCI_BLOCK) {
+            uint64_t e = e0 + tid;
+            Cand c;
+            bool have = false;
+            if (e < H.n_conn) {
+                connection(H, s, e, true, c);
+                have = all_in || (c.row >= row_lo && c.row < row_hi);
+                if (have && !fx_in_range(c.val)) range_bad = true;
+            }
+            if (all_in) {
+                if (have) {
+                    ok[pos + e - e0] = ((c.dense - base_dense) << 1) | c.tag;
+                    ov[pos + e - e0] = c.val;
+                }
+                continue;
+            }
+            uint64_t bal = __ballot(have);
+            const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            uint32_t my = (uint32_t)popc64(bal & lt);
+            if (lane == 0) s_wcnt[w] = (uint32_t)popc64(bal);
+            __syncthreads();
+            uint32_t pre = 0, tot = 0;
+            for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) {
+                if (i < w) pre += s_wcnt[i];
+                tot += s_wcnt[i];
+            }
+            if (have) {
+                ok[pos + pre + my] = ((c.dense - base_dense) << 1) | c.tag;
+                ov[pos + pre + my] = c.val;
+            }
+            pos += tot;
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+    if (range_bad) atomicOr(flags, FLAG_RANGE);
+}
+
+// Exact per-target merge of sorted candidates.  Head threads walk their segment.
+__global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t* __restrict__ ck,
+                                                       const double* __restrict__ cv, size_t n, uint64_t base_dense,
+                                                       const uint64_t* __restrict__ ik, const double* __restrict__ iv,
+                                                       size_t n_in, double cut, uint64_t* __restrict__ sk_out,
+                                                       double* __restrict__ sv_out, unsigned long long* out_count,
+                                                       size_t out_cap, unsigned long long* unique_count,
+                                                       unsigned long long* diag_acc, uint32_t* flags) {
+    fx128 dacc = fx_zero();
+    const int lane = threadIdx.x & (SFCI_WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long uniq = 0;
+    size_t stride = (size_t)gridDim.x * SFCI_BLOCK;
+    size_t n_round = (n + stride - 1) / stride * stride;
+    for (size_t i = (size_t)blockIdx.x * SFCI_BLOCK + threadIdx.x; i < n_round; i += stride) {
+        bool emit = false;
+        uint64_t det = 0;
+        double y = 0.0;
+        if (i < n) {
+            uint64_t key = ck[i] >> 1;
+            bool head = (i == 0) || ((ck[i - 1] >> 1) != key);
+            if (head) {
+                ++uniq;
+                fx128 acc = fx_zero();
+                uint32_t tag = 0;
+                size_t j = i;
+                while (j < n && (ck[j] >> 1) == key) {
+                    acc = fx_add(acc, fx_from(cv[j]));
+                    tag |= (uint32_t)(ck[j] & 1ull);
+                    ++j;
+                }
+                y = fx_to(acc);
+                uint64_t dense = key + base_dense;
+                uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+                uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
+                uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
+                det = spread_string(s0) | (spread_string(s1) << 1);
+                if (tag) {
+                    size_t lo = 0, hi = n_in;
+                    while (lo < hi) {
+                        size_t mid = (lo + hi) >> 1;
+                        if (ik[mid] < det) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < n_in && ik[lo] == det) {
+                        double p = y * iv[lo];
+                        dacc = fx_add(dacc, fx_from(p));
+                    } else {
+                        atomicOr(flags, FLAG_BAD_INPUT);
+                    }
+                }
+                emit = fabs(y) > cut;
+            }
+        }
+        uint64_t bal = __ballot(emit);
+        if (bal) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(out_count, (unsigned long long)popc64(bal));
+            base = __shfl(base, 0, SFCI_WAVE);
+            if (emit) {
+                uint64_t pos = base + (uint64_t)popc64(bal & lt);
+                if (pos < out_cap) {
+                    sk_out[pos] = det;
+                    sv_out[pos] = y;
+                } else {
+                    atomicOr(flags, FLAG_OUT_OVERFLOW);
+                }
+            }
+        }
+    }
+    fx_block_reduce_atomic<SFCI_BLOCK>(dacc, diag_acc);
+    for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) uniq += __shfl_down(uniq, off, SFCI_WAVE);
+    if (lane == 0 && uniq) atomicAdd(unique_count, uniq);
+}
+
+// ---------------------------------------------------------------------------------
+
+static int grid_for(Ctx* c, size_t work, int per_cu = 8) {
+    int cus = 256;
+    (void)c;
+    size_t g = std::min<size_t>(work, (size_t)cus * per_cu);
+    return (int)std::max<size_t>(g, 1);
+}
+
+struct Timer {
+    hipEvent_t a, b;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, s));
+    }
+    float stop() {
+        HIPCHK(hipEventRecord(b, s));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
+              size_t out_cap, uint64_t* ok, double* ov, double* diag) {
+    const HamDev& H = c->ham;
+    auto& st = c->st;
+    st = {};
+    Timer total(c->stream);
+    *diag = 0.0;
+    if (n_in == 0) return 0;
+    // 1. input: top-Nd
+    const uint64_t* sk = ik;
+    const double* sv = iv;
+    size_t ns = n_in;
+    if (n_in > nd) {
+        uint64_t* tk = c->src_k.get<uint64_t>(nd);
+        double* tv = c->src_v.get<double>(nd);
+        ns = select_top(c, ik, iv, n_in, nd, tk, tv);
+        sk = tk;
+        sv = tv;
+    }
+    st.n_src = ns;
+    // 2. plan passes over target rows
+    const uint64_t n_rows = H.dim;
+    const size_t rec = sizeof(uint64_t) + sizeof(double);
+    size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec), (size_t)1 << 20);
+    uint64_t total_cand = (uint64_t)ns * H.n_conn;
+    std::vector<std::pair<uint64_t, uint64_t>> passes;
+    if (total_cand <= pass_cap) {
+        passes.push_back({0, n_rows});
+    } else {
+        int grp_shift = 0;
+        while ((n_rows >> grp_shift) >= 4096) ++grp_shift;
+        int n_grp = (int)((n_rows - 1) >> grp_shift) + 1;
+        unsigned long long* hist = c->rowhist.get<unsigned long long>(n_grp);
+        HIPCHK(hipMemsetAsync(hist, 0, n_grp * sizeof(unsigned long long), c->stream));
+        k_rowgroup_hist<<<grid_for(c, ns), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
+            H, sk, ns, grp_shift, n_grp, hist);
+        std::vector<unsigned long long> hh(n_grp);
+        HIPCHK(hipMemcpyAsync(hh.data(), hist, n_grp * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        uint64_t lo = 0, acc = 0;
+        for (int g = 0; g < n_grp; ++g) {
+            if (hh[g] > pass_cap)
+                throw HipError("one row group exceeds the pass capacity; raise the workspace limit", -4);
+            if (acc + hh[g] > pass_cap) {
+                uint64_t hi = (uint64_t)g << grp_shift;
+                passes.push_back({lo, hi});
+                lo = hi;
+                acc = 0;
+            }
+            acc += hh[g];
+        }
+        passes.push_back({lo, n_rows});
+    }
+    st.n_passes = passes.size();
+    // staging for merged output
+    size_t stage_cap = std::max<size_t>(out_cap + (out_cap >> 2), (size_t)1 << 16);
+    if (c->stage_k.bytes / sizeof(uint64_t) > stage_cap) stage_cap = c->stage_k.bytes / sizeof(uint64_t);
+    uint64_t* stk = c->stage_k.get<uint64_t>(stage_cap);
+    double* stv = c->stage_v.get<double>(stage_cap);
+    // no output wanted (energy): nothing passes the cut
+    const double cut_eff = out_cap ? c->cut : INFINITY;
+    fx128 diag_total = fx_zero();
+    uint64_t uniq_host = 0;
+    unsigned long long* d_out_count = (unsigned long long*)c->d_counter;
+    unsigned long long* d_uniq = d_out_count + 1;
+    HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
+    uint64_t out_count_host = 0;
+    for (auto& ps : passes) {
+        const uint64_t row_lo = ps.first, row_hi = ps.second;
+        if (row_lo >= row_hi) continue;
+        // 3a. count + offsets
+        const uint64_t* off = nullptr;
+        uint64_t n_pass;
+        if (passes.size() == 1) {
+            n_pass = total_cand;
+        } else {
+            uint32_t* cnt = c->src_cnt.get<uint32_t>(ns);
+            uint64_t* offs = c->src_off.get<uint64_t>(ns + 1);
+            k_count<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ns, row_lo, row_hi, cnt);
+            uint64_t* tot_dev = offs + ns;
+            exclusive_scan_u32(c, cnt, ns, offs, tot_dev);
+            HIPCHK(hipMemcpyAsync(&n_pass, tot_dev, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            off = offs;
+        }
+        if (n_pass == 0) continue;
+        uint64_t* k0 = c->cand_k[0].get<uint64_t>(n_pass);
+        double* v0 = c->cand_v[0].get<double>(n_pass);
+        uint64_t* k1 = c->cand_k[1].get<uint64_t>(n_pass);
+        double* v1 = c->cand_v[1].get<double>(n_pass);
+        {
+            Timer t(c->stream);
+            k_generate<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo, row_hi,
+                                                                     off, k0, v0, c->d_flags);
+            HIPCHK(hipGetLastError());
+            st.ms_generate += t.stop();
+        }
+        st.n_candidates += n_pass;
+        // 3b. sort
+        int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
+        {
+            Timer t(c->stream);
+            radix_sort_pairs(c, &k0, &v0, k1, v1, n_pass, bits);
+            st.ms_sort += t.stop();
+        }
+        // 3c. reduce (the pass's diag goes to accumulator slot 1, folded on the host)
+        for (int attempt = 0;; ++attempt) {
+            Timer t(c->stream);
+            uint64_t before = out_count_host;
+            acc_clear(c, 1);
+            int grid = (int)std::min<size_t>((n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 256 * 16);
+            k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, k0, v0, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff, stk,
+                                                         stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
+                                                         c->d_flags);
+            HIPCHK(hipGetLastError());
+            uint64_t cnts[2];
+            HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
+            st.ms_reduce += t.stop();
+            if (cnts[0] <= stage_cap) {
+                out_count_host = cnts[0];
+                uniq_host = cnts[1];
+                diag_total = fx_add(diag_total, acc_read_raw(c, 1));
+                break;
+            }
+            if (attempt > 0) throw HipError("merged-output staging overflow after regrow", -3);
+            // grow the staging list, keep earlier passes' output, redo this pass
+            size_t ncap = cnts[0] + (cnts[0] >> 2);
+            DevBuf nk, nv;
+            uint64_t* nkp = nk.get<uint64_t>(ncap);
+            double* nvp = nv.get<double>(ncap);
+            if (before) {
+                HIPCHK(hipMemcpyAsync(nkp, stk, before * 8, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipMemcpyAsync(nvp, stv, before * 8, hipMemcpyDeviceToDevice, c->stream));
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->stage_k.release();
+            c->stage_v.release();
+            c->stage_k = nk;
+            c->stage_v = nv;
+            stk = nkp;
+            stv = nvp;
+            stage_cap = ncap;
+            cnts[0] = before;
+            cnts[1] = uniq_host;
+            HIPCHK(hipMemcpyAsync(d_out_count, cnts, 16, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            uint32_t fl = flags_read_clear(c);
+            if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+        }
+    }
+    uint64_t uniq = uniq_host;
+    *diag = fx_to(diag_total);
+    uint32_t fl = flags_read_clear(c);
+    if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    if (fl & FLAG_OUT_OVERFLOW) throw HipError("merged-output staging overflow", -3);
+    if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
+    st.n_unique = uniq;
+    // 4. top out_cap by |y|, then key order
+    size_t m = out_count_host;
+    size_t kept = 0;
+    if (out_cap > 0 && m > 0) {
+        Timer t(c->stream);
+        uint64_t* sel_k = c->tmp_k[0].get<uint64_t>(m);
+        double* sel_v = c->tmp_v[0].get<double>(m);
+        kept = select_top(c, stk, stv, m, out_cap, sel_k, sel_v);
+        uint64_t* tk = c->tmp_k[1].get<uint64_t>(kept);
+        double* tv = c->tmp_v[1].get<double>(kept);
+        uint64_t* kk = sel_k;
+        double* vv = sel_v;
+        radix_sort_pairs(c, &kk, &vv, tk, tv, kept, c->n_so);
+        dev_copy(c, ok, ov, kk, vv, kept);
+        st.ms_select += t.stop();
+    }
+    st.n_out = kept;
+    st.ms_total = total.stop();
+    return kept;
+}
+
+}  // namespace sfci

@@ -1,0 +1,154 @@
+// Internal declarations of the SFCI HIP library (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace sfci {
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::sfci::HipError(std::string(#x) + ": " + hipGetErrorString(e_), -2);            \
+    } while (0)
+
+// Growable device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <typename T>
+    T* get(size_t count) {
+        size_t need = count * sizeof(T);
+        if (need == 0) need = sizeof(T);
+        if (need > bytes) {
+            if (p) HIPCHK(hipFree(p));
+            p = nullptr;
+            bytes = 0;
+            size_t alloc = need + need / 8;
+            HIPCHK(hipMalloc(&p, alloc));
+            bytes = alloc;
+        }
+        return (T*)p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// Device-side error flags (bit set by kernels).
+enum : uint32_t {
+    FLAG_RANGE = 1u,        // a term left the fixed-point range (|x| >= 2^42)
+    FLAG_OUT_OVERFLOW = 2u, // merged-output staging buffer too small (host grows and retries)
+    FLAG_BAD_INPUT = 4u,    // an input determinant was not found in its own (unsorted) vector
+};
+
+// Fixed per-Hamiltonian quantities passed by value to the kernels.
+struct HamDev {
+    int n_orb;
+    int k;                 // electrons per spin (the two spins are equal)
+    int v;                 // virtual spatial orbitals per spin
+    uint64_t dim;          // C(n_orb, k): strings per spin
+    const double* h;       // [2][n_orb][n_orb]  h[s][a][i] = h(2a+s, 2i+s)
+    const double* g;       // [4][n_orb^4]       block 2*s1+s2: (a i | b j) with (a,i) spin s1, (b,j) spin s2
+    const uint64_t* binom; // [65][33]
+    const uint16_t* occ_pairs;  // C(k,2) entries (lo byte, hi byte) = positions in the occupied list
+    const uint16_t* vir_pairs;  // C(v,2) entries = positions in the virtual list
+    double constant;
+    // class sizes
+    uint64_t n_sa, n_daa, n_dab, n_conn;   // singles per spin, same-spin doubles per spin, opposite-spin, U+1
+    uint32_t n_opair, n_vpair;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_so = 0, n_orb = 0, n_elec = 0;
+    HamDev ham{};
+    double* d_h = nullptr;
+    double* d_g = nullptr;
+    uint64_t* d_binom = nullptr;
+    uint16_t* d_occ_pairs = nullptr;
+    uint16_t* d_vir_pairs = nullptr;
+    std::vector<uint64_t> h_binom;   // host copy
+    double cut = 1e-5;
+    size_t ws_limit = 0;             // bytes for the candidate buffers of one pass
+    // scratch
+    DevBuf cand_k[2], cand_v[2];
+    DevBuf src_cnt, src_off, src_k, src_v;
+    DevBuf stage_k, stage_v;         // merged H.v output before the top-cap select
+    DevBuf tmp_k[2], tmp_v[2];
+    DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
+    DevBuf scratch_hist, scratch_off, scratch_small;
+    DevBuf scratch_scan[8];
+    DevBuf rowhist;
+    unsigned long long* d_acc = nullptr;   // 8 x 2 words of fx128 accumulators
+    uint32_t* d_flags = nullptr;
+    uint64_t* d_counter = nullptr;         // 8 general counters
+    std::string last_error;
+    // statistics of the last H.v
+    struct {
+        uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0;
+        double ms_generate = 0, ms_sort = 0, ms_reduce = 0, ms_select = 0, ms_total = 0;
+    } st;
+};
+
+struct Vec {
+    Ctx* ctx = nullptr;
+    uint64_t* k = nullptr;
+    double* v = nullptr;
+    int64_t len = 0;
+    int64_t cap = 0;
+};
+
+// sort.hip
+void exclusive_scan_u32(Ctx* c, const uint32_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
+void exclusive_scan_u64(Ctx* c, const uint64_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
+void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits);
+
+// vecops.hip
+// Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.
+// Output may not alias input.  Returns the kept count.
+size_t select_top(Ctx* c, const uint64_t* k, const double* v, size_t n, size_t cap, uint64_t* ok, double* ov);
+double dev_dot(Ctx* c, const uint64_t* ak, const double* av, size_t na, const uint64_t* bk, const double* bv,
+               size_t nb);
+double dev_norm(Ctx* c, const double* v, size_t n);
+void dev_scale(Ctx* c, double* v, size_t n, double s);
+size_t dev_add_two(Ctx* c, const uint64_t* vk, const double* vv, size_t nv, const uint64_t* wk, const double* wv,
+                   size_t nw, double a, size_t cap, uint64_t* ok, double* ov);
+size_t dev_lincomb(Ctx* c, const Vec* const* vecs, const double* coef, int n, size_t cap, uint64_t* ok, double* ov);
+void dev_copy(Ctx* c, uint64_t* dk, double* dv, const uint64_t* sk, const double* sv, size_t n);
+
+// hv.hip
+// H.v with the reference's semantics; returns the output length written to (ok, ov)
+// (capacity out_cap, key-sorted) and the exact diag.
+size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
+              size_t out_cap, uint64_t* ok, double* ov, double* diag);
+
+// accumulator helpers
+void acc_clear(Ctx* c, int slot);
+fx128 acc_read_raw(Ctx* c, int slot);
+double acc_read(Ctx* c, int slot);
+uint32_t flags_read_clear(Ctx* c);
+
+static inline int bits_for(uint64_t max_value_exclusive) {
+    // smallest b with 2^b >= max_value_exclusive
+    int b = 0;
+    while (b < 64 && ((uint64_t)1 << b) < max_value_exclusive) ++b;
+    return b;
+}
+
+}  // namespace sfci

@@ -1,0 +1,148 @@
+// Ground-state driver: sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548)
+// restated in host C++ on top of the library's C-ABI.  Every vector stays on the GPU;
+// the host only holds the Nl x Nl projected matrices.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/sfci_gpu.h"
+#include "gsaes.h"
+
+namespace {
+
+struct VecGuard {
+    std::vector<sfci_vec*> v;
+    ~VecGuard() {
+        for (auto* p : v) sfci_vec_destroy(p);
+    }
+};
+
+#define CK(x)                      \
+    do {                           \
+        int rc_ = (x);             \
+        if (rc_ != SFCI_OK) return rc_; \
+    } while (0)
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); }
+
+}  // namespace
+
+extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_lanczos_params_t* p, double* ritz,
+                                 double* e_trunc, sfci_lanczos_result_t* res, sfci_vec* v_out) {
+    if (!ctx || !p || !ritz || !e_trunc || !res || p->nd < 1 || p->nl < 1 || p->num_restart < 1 || p->hpsi_cap < 1)
+        return SFCI_E_ARG;
+    const int nl = p->nl;
+    const int64_t nd = p->nd, cap = p->hpsi_cap;
+    auto t_start = clk::now();
+    *res = sfci_lanczos_result_t{};
+    VecGuard g;
+    auto mk = [&](int64_t c, sfci_vec** out) {
+        int rc = sfci_vec_create(ctx, c, out);
+        if (rc == SFCI_OK) g.v.push_back(*out);
+        return rc;
+    };
+    std::vector<sfci_vec*> K(nl);
+    for (int i = 0; i < nl; ++i) CK(mk(nd, &K[i]));
+    sfci_vec *v, *u, *temp, *temp_u, *v_next;
+    CK(mk(nd, &v));
+    CK(mk(cap, &u));
+    CK(mk(cap, &temp));
+    CK(mk(cap, &temp_u));
+    CK(mk(nd, &v_next));
+    double one = 1.0;
+    CK(sfci_vec_upload(v, &init_det, &one, 1));
+
+    const double deltaE = p->delta_e;
+    double previousE = 0.0, nearestE = -1000000.0;
+    std::vector<double> KM(nl * nl, 0.0), S(nl * nl, 0.0);
+    auto hv = [&](const sfci_vec* in, sfci_vec* out, double* alpha) {
+        auto t = clk::now();
+        int rc = sfci_hv(ctx, in, out, nd, 0.0, 1.0, alpha);
+        res->ms_hv += ms_since(t);
+        res->n_hv += 1;
+        return rc;
+    };
+    int r = 0;
+    for (r = 0; r < p->num_restart; ++r) {
+        if (p->verbose) printf("\n*****************block %d starts*****************\n", r);
+        CK(sfci_vec_copy(K[0], v));
+        double alpha = 0.0, beta = 0.0;
+        CK(hv(v, u, &alpha));
+        if (p->verbose) printf("H * phi_0 len:%lld\n", (long long)sfci_vec_len(u));
+        KM[0] = alpha;
+        for (int i = 1; i < nl; ++i) {
+            auto t_it = clk::now();
+            CK(sfci_vec_copy(temp, u));
+            CK(sfci_add_two(ctx, temp, v, -alpha));
+            CK(sfci_shrink_from(ctx, v_next, temp));
+            double gamma = 0.0;
+            CK(sfci_norm(ctx, v_next, &gamma));
+            beta = gamma;
+            CK(sfci_scale(ctx, v_next, 1.0 / gamma));
+            CK(hv(v_next, temp_u, &alpha));
+            KM[i * nl + i] = alpha;
+            for (int j = 0; j < i; ++j) {
+                double d = 0.0;
+                CK(sfci_dot(ctx, K[j], temp_u, &d));
+                KM[j * nl + i] = KM[i * nl + j] = d;
+            }
+            CK(sfci_add_two(ctx, temp_u, v, -beta));
+            CK(sfci_vec_copy(u, temp_u));
+            CK(sfci_vec_copy(v, v_next));
+            CK(sfci_vec_copy(K[i], v));
+            if (p->verbose) printf("iter %d: len %lld alpha %.12f beta %.12f time %.3f ms\n", i,
+                                   (long long)sfci_vec_len(v), alpha, beta, ms_since(t_it));
+        }
+        for (int i = 0; i < nl; ++i)
+            for (int j = i; j < nl; ++j) {
+                double d = 0.0;
+                CK(sfci_dot(ctx, K[i], K[j], &d));
+                S[i * nl + j] = S[j * nl + i] = d;
+            }
+        std::vector<double> ev, evec;
+        if (sfci_dense::gsaes(nl, KM, S, ev, evec)) return SFCI_E_ARG;
+        int nearest = 0;
+        double maxGap = 0.0;
+        for (int i = 0; i < nl; ++i) {
+            double gap = std::fabs(nearestE - ev[i]);
+            if (i == 0 || gap < maxGap) {
+                maxGap = gap;
+                nearest = i;
+            }
+        }
+        nearestE = ev[nearest];
+        std::vector<double> c(nl);
+        for (int i = 0; i < nl; ++i) c[i] = evec[i * nl + nearest];
+        CK(sfci_lincomb(ctx, K.data(), c.data(), nl, v));
+        double nrm = 0.0;
+        CK(sfci_norm(ctx, v, &nrm));
+        CK(sfci_scale(ctx, v, 1.0 / nrm));
+        double et = 0.0;
+        CK(sfci_energy(ctx, v, &et));
+        ritz[r] = ev[nearest];
+        e_trunc[r] = et;
+        if (p->verbose) {
+            printf("energy Psi_res after truncation:%.12f\n", et);
+            printf("gsaes energy:%.12f\n", ev[nearest]);
+            printf("*****************block %d ends*****************\n", r);
+        }
+        std::fill(KM.begin(), KM.end(), 0.0);
+        for (int i = 0; i < nl; ++i) CK(sfci_vec_clear(K[i]));
+        if (previousE - ev[nearest] < deltaE) {
+            ++r;
+            break;
+        }
+        previousE = ev[nearest];
+    }
+    double ef = 0.0;
+    CK(sfci_energy(ctx, v, &ef));
+    if (p->verbose) printf("final e:%.12f\n", ef);
+    if (v_out) CK(sfci_vec_copy(v_out, v));
+    CK(sfci_sync(ctx));
+    res->n_restarts = r;
+    res->e_final = ef;
+    res->ms_total = ms_since(t_start);
+    return SFCI_OK;
+}

@@ -1,0 +1,196 @@
+// Device-wide exclusive scan and stable LSD radix sort of (u64 key, f64 value) pairs.
+//
+// Radix sort: 8-bit digits, tiles of 4096 items (256 threads x 16 rounds).  Per pass:
+//   hist    - per-tile digit histogram in LDS, written digit-major [256][n_tiles]
+//   scan    - exclusive scan of the histogram -> global start of (digit, tile)
+//   scatter - stable in-tile ranking: each round of 256 consecutive items is ranked
+//             with a 64-lane digit match built from 8 ballots, wave counts are
+//             combined in wave order through LDS, so items keep their input order.
+// A pass whose items all share one digit is skipped (identity permutation).
+#include "internal.h"
+
+namespace sfci {
+
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SFCI_BLOCK * SCAN_ITEMS;   // 1024
+
+// block-wide exclusive scan of one value per thread; returns the block total
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T& total) {
+    __shared__ T s_w[SFCI_BLOCK / SFCI_WAVE];
+    int lane = threadIdx.x & (SFCI_WAVE - 1), w = threadIdx.x / SFCI_WAVE;
+    T x = v;
+    for (int off = 1; off < SFCI_WAVE; off <<= 1) {
+        T y = __shfl_up(x, off, SFCI_WAVE);
+        if (lane >= off) x += y;
+    }
+    if (lane == SFCI_WAVE - 1) s_w[w] = x;
+    __syncthreads();
+    T pre = 0, tot = 0;
+    for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) {
+        if (i < w) pre += s_w[i];
+        tot += s_w[i];
+    }
+    __syncthreads();
+    total = tot;
+    return pre + x - v;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_scan_reduce(const TIn* in, size_t n, uint64_t* partial) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    uint64_t s = 0;
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        size_t i = base + (size_t)threadIdx.x * SCAN_ITEMS + r;
+        if (i < n) s += (uint64_t)in[i];
+    }
+    uint64_t tot;
+    block_exclusive_scan<uint64_t>(s, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_scan_down(const TIn* in, size_t n, const uint64_t* partial_scanned,
+                                                          uint64_t* out, uint64_t* total_out) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    uint64_t v[SCAN_ITEMS];
+    uint64_t s = 0;
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        size_t i = base + (size_t)threadIdx.x * SCAN_ITEMS + r;
+        v[r] = i < n ? (uint64_t)in[i] : 0;
+        s += v[r];
+    }
+    uint64_t tot;
+    uint64_t pre = block_exclusive_scan<uint64_t>(s, tot) + partial_scanned[blockIdx.x];
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        size_t i = base + (size_t)threadIdx.x * SCAN_ITEMS + r;
+        if (i < n) out[i] = pre;
+        pre += v[r];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == SFCI_BLOCK - 1) *total_out = pre;
+}
+
+template <typename TIn>
+static void scan_impl(Ctx* c, const TIn* in, size_t n, uint64_t* out, uint64_t* total_dev, int depth) {
+    if (n == 0) {
+        if (total_dev) HIPCHK(hipMemsetAsync(total_dev, 0, 8, c->stream));
+        return;
+    }
+    size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    uint64_t* partial = c->scratch_scan[depth].get<uint64_t>(2 * nb + 2);
+    uint64_t* pscan = partial + nb + 1;
+    k_scan_reduce<TIn><<<nb, SFCI_BLOCK, 0, c->stream>>>(in, n, partial);
+    if (nb == 1) {
+        HIPCHK(hipMemsetAsync(pscan, 0, 8, c->stream));
+    } else {
+        scan_impl<uint64_t>(c, partial, nb, pscan, nullptr, depth + 1);
+    }
+    k_scan_down<TIn><<<nb, SFCI_BLOCK, 0, c->stream>>>(in, n, pscan, out, total_dev);
+}
+
+void exclusive_scan_u32(Ctx* c, const uint32_t* in, size_t n, uint64_t* out, uint64_t* total_dev) {
+    scan_impl<uint32_t>(c, in, n, out, total_dev, 0);
+}
+void exclusive_scan_u64(Ctx* c, const uint64_t* in, size_t n, uint64_t* out, uint64_t* total_dev) {
+    scan_impl<uint64_t>(c, in, n, out, total_dev, 0);
+}
+
+// ---------------------------------------------------------------------------------
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int SORT_ROUNDS = 16;
+constexpr int SORT_TILE = SFCI_BLOCK * SORT_ROUNDS;   // 4096
+
+__global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __restrict__ kin, size_t n, int shift,
+                                                           uint32_t* __restrict__ hist, size_t n_tiles) {
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        size_t i = base + (size_t)r * SFCI_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(kin[i] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __restrict__ kin,
+                                                              const double* __restrict__ vin,
+                                                              uint64_t* __restrict__ kout, double* __restrict__ vout,
+                                                              size_t n, int shift, const uint64_t* __restrict__ off,
+                                                              size_t n_tiles) {
+    constexpr int NW = SFCI_BLOCK / SFCI_WAVE;
+    __shared__ uint64_t s_base[RADIX];
+    __shared__ uint32_t s_cnt[NW][RADIX];
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
+    s_base[tid] = off[(size_t)tid * n_tiles + blockIdx.x];
+    for (int i = 0; i < NW; ++i) s_cnt[i][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    size_t base = (size_t)blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        size_t i = base + (size_t)r * SFCI_BLOCK + tid;
+        bool valid = i < n;
+        uint64_t k = valid ? kin[i] : 0ull;
+        double v = valid ? vin[i] : 0.0;
+        uint32_t d = (uint32_t)(k >> shift) & (RADIX - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; ++b) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t rank = (uint32_t)popc64(peers & lt);
+        if (valid && rank == 0) s_cnt[w][d] = (uint32_t)popc64(peers);
+        __syncthreads();
+        if (valid) {
+            uint64_t pos = s_base[d] + rank;
+            for (int j = 0; j < w; ++j) pos += s_cnt[j][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int j = 0; j < NW; ++j) { tot += s_cnt[j][tid]; s_cnt[j][tid] = 0; }
+        s_base[tid] += tot;
+        __syncthreads();
+    }
+}
+
+// Sort n pairs by the low `bits` bits of the key (stable).  (*keys, *vals) hold the
+// input; on return they point at the sorted data, which is either the input buffers
+// or (ktmp, vtmp).
+void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits) {
+    if (n <= 1 || bits <= 0) return;
+    size_t n_tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    uint32_t* hist = c->scratch_hist.get<uint32_t>((size_t)RADIX * n_tiles);
+    uint64_t* off = c->scratch_off.get<uint64_t>((size_t)RADIX * n_tiles + 1);
+    uint64_t* bounds = c->scratch_small.get<uint64_t>(RADIX + 1);
+    std::vector<uint64_t> hb(RADIX + 1);
+    uint64_t* kin = *keys;
+    double* vin = *vals;
+    uint64_t* kout = ktmp;
+    double* vout = vtmp;
+    for (int shift = 0; shift < bits; shift += RADIX_BITS) {
+        k_radix_hist<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, n, shift, hist, n_tiles);
+        exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
+        // digit boundaries: off[d * n_tiles] for each d
+        HIPCHK(hipMemcpy2DAsync(bounds, sizeof(uint64_t), off, n_tiles * sizeof(uint64_t), sizeof(uint64_t), RADIX,
+                                hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(hb.data(), bounds, RADIX * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hb[RADIX] = n;
+        bool single = false;
+        for (int d = 0; d < RADIX; ++d)
+            if (hb[d + 1] - hb[d] == n) single = true;
+        if (single) continue;
+        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, shift, off, n_tiles);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    *keys = kin;
+    *vals = vin;
+}
+
+}  // namespace sfci

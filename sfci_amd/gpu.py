@@ -1,0 +1,263 @@
+"""ctypes binding of libsfci_hip.so (the C-ABI declared in include/sfci_gpu.h).
+
+The shared library is built in-tree by ``sfci_amd/Makefile``.  Loading fails loudly when
+it is missing: there is no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsfci_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "sfci_gpu.h")
+
+SFCI_OK = 0
+ERRORS = {-1: "SFCI_E_ARG", -2: "SFCI_E_HIP", -3: "SFCI_E_OVERFLOW", -4: "SFCI_E_CAPACITY",
+          -5: "SFCI_E_RANGE", -6: "SFCI_E_IO"}
+
+
+class SfciError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class InfoT(C.Structure):
+    _fields_ = [("n_so", C.c_int), ("n_elec", C.c_int), ("string_dim", C.c_uint64),
+                ("n_conn", C.c_uint64), ("e_core", C.c_double)]
+
+
+class StatsT(C.Structure):
+    _fields_ = [("n_src", C.c_uint64), ("n_candidates", C.c_uint64), ("n_unique", C.c_uint64),
+                ("n_out", C.c_uint64), ("n_passes", C.c_uint64), ("ms_generate", C.c_double),
+                ("ms_sort", C.c_double), ("ms_reduce", C.c_double), ("ms_select", C.c_double),
+                ("ms_total", C.c_double)]
+
+
+class LanczosParamsT(C.Structure):
+    _fields_ = [("nd", C.c_int64), ("nl", C.c_int), ("num_restart", C.c_int), ("hpsi_cap", C.c_int64),
+                ("delta_e", C.c_double), ("verbose", C.c_int)]
+
+
+class LanczosResultT(C.Structure):
+    _fields_ = [("n_restarts", C.c_int), ("e_final", C.c_double), ("ms_total", C.c_double),
+                ("ms_hv", C.c_double), ("n_hv", C.c_int64)]
+
+
+def build(force: bool = False) -> str:
+    args = ["make", "-s", "-C", HERE, "-j4"]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True)
+    return LIB_PATH
+
+
+def declared_symbols() -> list:
+    """Function names declared in include/sfci_gpu.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sfci_[a-z_0-9]+)\s*\(", txt)))
+
+
+_lib = None
+
+
+def lib():
+    """Load libsfci_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsfci_hip.so is not built (run `make -C sfci_amd` or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, u64, dbl, ip = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int
+    pvp = C.POINTER(C.c_void_p)
+    sig = {
+        "sfci_create": (ip, [C.c_char_p, ip, ip, ip, pvp]),
+        "sfci_create_from_entries": (ip, [ip, ip, i64, vp, vp, ip, pvp]),
+        "sfci_destroy": (None, [vp]),
+        "sfci_last_error": (C.c_char_p, [vp]),
+        "sfci_info": (ip, [vp, C.POINTER(InfoT)]),
+        "sfci_set_cut": (ip, [vp, dbl]),
+        "sfci_set_workspace": (ip, [vp, u64]),
+        "sfci_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
+        "sfci_sync": (ip, [vp]),
+        "sfci_vec_create": (ip, [vp, i64, pvp]),
+        "sfci_vec_destroy": (None, [vp]),
+        "sfci_vec_len": (i64, [vp]),
+        "sfci_vec_cap": (i64, [vp]),
+        "sfci_vec_upload": (ip, [vp, vp, vp, i64]),
+        "sfci_vec_download": (ip, [vp, vp, vp, i64, C.POINTER(C.c_int64)]),
+        "sfci_vec_device_ptrs": (ip, [vp, pvp, pvp]),
+        "sfci_vec_set_len": (ip, [vp, i64]),
+        "sfci_vec_copy": (ip, [vp, vp]),
+        "sfci_vec_clear": (ip, [vp]),
+        "sfci_hv": (ip, [vp, vp, vp, i64, dbl, dbl, C.POINTER(C.c_double)]),
+        "sfci_energy": (ip, [vp, vp, C.POINTER(C.c_double)]),
+        "sfci_diag_element": (ip, [vp, u64, C.POINTER(C.c_double)]),
+        "sfci_dot": (ip, [vp, vp, vp, C.POINTER(C.c_double)]),
+        "sfci_norm": (ip, [vp, vp, C.POINTER(C.c_double)]),
+        "sfci_scale": (ip, [vp, vp, dbl]),
+        "sfci_add_two": (ip, [vp, vp, vp, dbl]),
+        "sfci_shrink_from": (ip, [vp, vp, vp]),
+        "sfci_lincomb": (ip, [vp, vp, vp, ip, vp]),
+        "sfci_ground_state": (ip, [vp, u64, C.POINTER(LanczosParamsT), vp, vp, C.POINTER(LanczosResultT), vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Vector:
+    """Device sparse vector (dynVec) of fixed capacity."""
+
+    def __init__(self, ctx: "Context", cap: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().sfci_vec_create(ctx._h, int(cap), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            lib().sfci_vec_destroy(self._h)
+            self._h = None
+
+    @property
+    def len(self) -> int:
+        return lib().sfci_vec_len(self._h)
+
+    @property
+    def cap(self) -> int:
+        return lib().sfci_vec_cap(self._h)
+
+    def upload(self, keys, vals) -> "Vector":
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        self.ctx._check(lib().sfci_vec_upload(self._h, _ptr(k), _ptr(v), len(k)))
+        return self
+
+    def download(self):
+        n = self.len
+        k, v = np.zeros(max(n, 1), np.uint64), np.zeros(max(n, 1))
+        got = C.c_int64()
+        self.ctx._check(lib().sfci_vec_download(self._h, _ptr(k), _ptr(v), n, C.byref(got)))
+        return k[:got.value].copy(), v[:got.value].copy()
+
+    def copy_from(self, other: "Vector") -> "Vector":
+        self.ctx._check(lib().sfci_vec_copy(self._h, other._h))
+        return self
+
+    def clear(self):
+        lib().sfci_vec_clear(self._h)
+
+
+class Context:
+    """One loaded Hamiltonian on one GPU."""
+
+    def __init__(self, in_path: str, n_so: int, n_elec: int, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().sfci_create(in_path.encode(), n_so, n_elec, device, C.byref(h))
+        if rc:
+            raise SfciError(rc, "sfci_create(%s)" % in_path)
+        self._h = h
+        self.n_so, self.n_elec = n_so, n_elec
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sfci_destroy(self._h)
+            self._h = None
+
+    def _check(self, rc: int):
+        if rc != SFCI_OK:
+            msg = lib().sfci_last_error(self._h) if self._h else b""
+            raise SfciError(rc, (msg or b"").decode())
+
+    def info(self) -> dict:
+        i = InfoT()
+        self._check(lib().sfci_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in InfoT._fields_}
+
+    def set_cut(self, cut: float):
+        self._check(lib().sfci_set_cut(self._h, cut))
+
+    def set_workspace(self, nbytes: int):
+        self._check(lib().sfci_set_workspace(self._h, int(nbytes)))
+
+    def stats(self) -> dict:
+        s = StatsT()
+        self._check(lib().sfci_hv_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in StatsT._fields_}
+
+    def sync(self):
+        self._check(lib().sfci_sync(self._h))
+
+    def vector(self, cap: int, keys=None, vals=None) -> Vector:
+        v = Vector(self, cap)
+        if keys is not None:
+            v.upload(keys, vals)
+        return v
+
+    def hv(self, vin: Vector, vout: Vector, nd: int, lam: float = 0.0, coeff: float = 1.0) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_hv(self._h, vin._h, vout._h, int(nd), lam, coeff, C.byref(d)))
+        return d.value
+
+    def energy(self, v: Vector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_energy(self._h, v._h, C.byref(d)))
+        return d.value
+
+    def diag_element(self, det: int) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_diag_element(self._h, det, C.byref(d)))
+        return d.value
+
+    def dot(self, a: Vector, b: Vector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_dot(self._h, a._h, b._h, C.byref(d)))
+        return d.value
+
+    def norm(self, a: Vector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_norm(self._h, a._h, C.byref(d)))
+        return d.value
+
+    def scale(self, a: Vector, s: float):
+        self._check(lib().sfci_scale(self._h, a._h, s))
+
+    def add_two(self, v: Vector, w: Vector, a: float):
+        self._check(lib().sfci_add_two(self._h, v._h, w._h, a))
+
+    def shrink_from(self, dst: Vector, src: Vector):
+        self._check(lib().sfci_shrink_from(self._h, dst._h, src._h))
+
+    def lincomb(self, vecs: Sequence[Vector], coeffs, out: Vector):
+        arr = (C.c_void_p * len(vecs))(*[v._h.value for v in vecs])
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self._check(lib().sfci_lincomb(self._h, C.cast(arr, C.c_void_p), _ptr(c), len(vecs), out._h))
+
+    def ground_state(self, init_det: int, nd: int, nl: int, num_restart: int, hpsi_cap: int = 60_000_000,
+                     delta_e: float = 1e-5, verbose: bool = False, v_out: Optional[Vector] = None) -> dict:
+        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0)
+        ritz, et = np.zeros(num_restart), np.zeros(num_restart)
+        r = LanczosResultT()
+        self._check(lib().sfci_ground_state(self._h, init_det, C.byref(p), _ptr(ritz), _ptr(et), C.byref(r),
+                                            v_out._h if v_out is not None else None))
+        n = r.n_restarts
+        return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), e_final=r.e_final,
+                    ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv)

@@ -1,0 +1,108 @@
+// sfci: command-line driver with the reference's interface (src/main.cpp:58-280).
+//
+// Reads ./sfci.input (key=value, split at the first '=', no trimming, unknown keys
+// ignored; src/main.cpp:128-151), loads the _IN integral file named by `filename`, and
+// for task=ground runs the truncated Lanczos solve from init_hf on the GPU.
+// Keys: task, Nd, Nl, num_restart, num_orb, num_e, filename, init_hf, coeff_cut,
+// target_E, num_thread, bucket_unit (the last two are accepted and unused: the GPU
+// path needs neither a thread count nor bucket over-allocation).  Additional keys:
+// hpsi_cap (default 60000000), device (default 0), workspace_gb.
+// Differences from the reference: coeff_cut is honoured (the reference overwrites it
+// with 1e-5, src/main.cpp:196), and task=excited reports that it is not available
+// (the reference's branch is empty, src/main.cpp:235-242).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/sfci_gpu.h"
+
+int main(int argc, char** argv) {
+    std::string input = argc > 1 ? argv[1] : "sfci.input";
+    std::string task, filename;
+    long long Nd = 200000, Nl = 10, num_restart = 2, hpsi_cap = 60000000;
+    int num_orb = 0, num_e = 0, device = 0;
+    unsigned long long init_hf = 0;
+    double coeff_cut = 1e-5, target_E = 0.0, workspace_gb = 0.0;
+    std::ifstream f(input);
+    if (!f.is_open()) {
+        fprintf(stderr, "cannot open %s\n", input.c_str());
+        return 1;
+    }
+    std::string raw;
+    while (std::getline(f, raw)) {
+        auto pos = raw.find('=');
+        if (pos == std::string::npos) continue;
+        std::string key = raw.substr(0, pos), val = raw.substr(pos + 1);
+        try {
+            if (key == "num_thread" || key == "bucket_unit") {
+                std::cout << "using input " << key << "=" << val << " (not used on the GPU)" << std::endl;
+                continue;
+            } else if (key == "Nd") Nd = std::stoll(val);
+            else if (key == "Nl") Nl = std::stoll(val);
+            else if (key == "num_restart") num_restart = std::stoll(val);
+            else if (key == "num_orb") num_orb = std::stoi(val);
+            else if (key == "num_e") num_e = std::stoi(val);
+            else if (key == "filename") filename = val;
+            else if (key == "coeff_cut") coeff_cut = std::stod(val);
+            else if (key == "init_hf") init_hf = std::stoull(val);
+            else if (key == "task") task = val;
+            else if (key == "target_E") target_E = std::stod(val);
+            else if (key == "hpsi_cap") hpsi_cap = std::stoll(val);
+            else if (key == "device") device = std::stoi(val);
+            else if (key == "workspace_gb") workspace_gb = std::stod(val);
+            else continue;
+        } catch (const std::exception&) {
+            fprintf(stderr, "bad value for %s: %s\n", key.c_str(), val.c_str());
+            return 1;
+        }
+        std::cout << "using input " << key << "=" << val << std::endl;
+    }
+    if (num_orb > 64) {
+        fprintf(stderr, "orb num larger than 64 is not supported at this point\n");
+        return 1;
+    }
+    sfci_ctx* ctx = nullptr;
+    int rc = sfci_create(filename.c_str(), num_orb, num_e, device, &ctx);
+    if (rc) {
+        fprintf(stderr, "failed to load %s (%d)\n", filename.c_str(), rc);
+        return 1;
+    }
+    std::cout << "integrals loaded\n";
+    sfci_set_cut(ctx, coeff_cut);
+    if (workspace_gb > 0) sfci_set_workspace(ctx, (uint64_t)(workspace_gb * 1e9));
+    sfci_info_t info;
+    sfci_info(ctx, &info);
+    std::cout.precision(12);
+    std::cout << "\n*****************parameters summary*****************\n";
+    std::cout << "input file:" << filename << "\norbital num:" << num_orb << "\nelectron num:" << num_e
+              << "\nNd:" << Nd << "\nstring dim:" << info.string_dim << "\nconnections per det:" << info.n_conn
+              << "\ntarget energy:" << target_E << "\nSpMSpV final result cut:" << coeff_cut << "\n";
+    std::cout << "\n****************************************************\n";
+    if (task == "excited") {
+        std::cout << "task=excited is not available (the reference's branch is empty)\n";
+    } else if (task == "ground") {
+        double ehf = 0.0;
+        if ((rc = sfci_diag_element(ctx, init_hf, &ehf))) {
+            fprintf(stderr, "init_hf: %s\n", sfci_last_error(ctx));
+            sfci_destroy(ctx);
+            return 1;
+        }
+        std::cout << "E_hf:" << ehf << std::endl;
+        sfci_lanczos_params_t p{Nd, (int)Nl, (int)num_restart, hpsi_cap, 1e-5, 1};
+        std::vector<double> ritz(num_restart), et(num_restart);
+        sfci_lanczos_result_t r;
+        rc = sfci_ground_state(ctx, init_hf, &p, ritz.data(), et.data(), &r, nullptr);
+        if (rc) {
+            fprintf(stderr, "solve failed (%d): %s\n", rc, sfci_last_error(ctx));
+            sfci_destroy(ctx);
+            return 1;
+        }
+        std::cout << "SFCI time: " << r.ms_total / 1000.0 << std::endl;
+    }
+    sfci_destroy(ctx);
+    return 0;
+}

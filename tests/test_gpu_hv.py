@@ -1,0 +1,197 @@
+"""GPU parity: the HIP H.v and sparse-vector operations against the oracle, bit for bit.
+
+All values are compared through their IEEE bit patterns: the oracle and the GPU
+library both form every sum exactly (128-bit fixed point) and round once, and neither
+fuses multiply-adds, so any difference is a bug.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_same_vec(k1, v1, k2, v2):
+    assert len(k1) == len(k2), (len(k1), len(k2))
+    assert np.array_equal(k1, k2)
+    diff = np.nonzero(bits(v1) != bits(v2))[0]
+    assert len(diff) == 0, "first mismatch at %d: %r vs %r" % (diff[0], v1[diff[0]], v2[diff[0]])
+
+
+def all_dets(n_orb, k):
+    strings = []
+    for occ in itertools.combinations(range(n_orb), k):
+        s = 0
+        for i in occ:
+            s |= 1 << (2 * i)
+        strings.append(s)
+    a = np.array(strings, dtype=np.uint64)
+    d = (a[:, None] | (a[None, :] << np.uint64(1))).ravel()
+    return np.sort(d)
+
+
+@pytest.fixture(scope="module")
+def sys1(make_system):
+    return make_system(12, 8, 0)          # the 12-orbital / 8-electron workload, seed 0
+
+
+@pytest.fixture(scope="module")
+def orc1(sys1):
+    return O.Oracle(sys1["in_path"], sys1["n_so"], sys1["n_elec"])
+
+
+@pytest.fixture(scope="module")
+def ctx1(sys1):
+    from sfci_amd import gpu
+    log("ctx1: loading %s" % sys1["in_path"])
+    c = gpu.Context(sys1["in_path"], sys1["n_so"], sys1["n_elec"])
+    yield c
+    c.close()
+
+
+def random_input(sys, n, seed, ties=False):
+    rng = np.random.default_rng(seed)
+    dets = all_dets(sys["n_orb"], sys["n_elec"] // 2)
+    pick = np.sort(rng.choice(len(dets), size=n, replace=False))
+    k = dets[pick]
+    if ties:
+        v = rng.choice([-0.25, 0.125, 0.5, -0.5, 0.03125], size=n)
+    else:
+        v = rng.normal(size=n) * np.exp(-rng.uniform(0, 8, size=n))
+    return k, v
+
+
+def gpu_hv(ctx, k, v, nd, lam, coeff, cap):
+    vin = ctx.vector(max(len(k), 1), k, v)
+    vout = ctx.vector(cap)
+    d = ctx.hv(vin, vout, nd, lam, coeff)
+    ok, ov = vout.download()
+    return ok, ov, d
+
+
+@pytest.mark.parametrize("case", ["hf", "rand", "rand_shift", "cap", "nd_ties", "cut0"])
+def test_hv_bit_exact(case, sys1, orc1, ctx1):
+    log("hv case %s" % case)
+    hf = sys1["init_hf"]
+    cut, cap, lam, coeff = 1e-5, 60_000_000, 0.0, 1.0
+    if case == "hf":
+        k, v = np.array([hf], np.uint64), np.array([1.0])
+    elif case == "rand":
+        k, v = random_input(sys1, 3000, 1)
+    elif case == "rand_shift":
+        k, v = random_input(sys1, 3000, 2)
+        lam, coeff = 0.37, -0.6
+    elif case == "cap":
+        k, v = random_input(sys1, 2000, 3, ties=True)
+        cap = 5000
+    elif case == "nd_ties":
+        k, v = random_input(sys1, 2000, 4, ties=True)
+    else:
+        k, v = random_input(sys1, 1000, 5)
+        cut = 0.0
+    nd = 700 if case == "nd_ties" else len(k)
+    ctx1.set_cut(cut)
+    try:
+        gk, gv, gd = gpu_hv(ctx1, k, v, nd, lam, coeff, cap)
+    finally:
+        ctx1.set_cut(1e-5)
+    ok, ov, od, _ = orc1.hv(k, v, nd=nd, lam=lam, coeff=coeff, cut=cut, out_cap=cap)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits([gd])[0] == bits([od])[0], (gd, od)
+    st = ctx1.stats()
+    assert st["n_out"] == len(ok)
+
+
+def test_hv_multipass_matches(sys1, orc1, ctx1):
+    k, v = random_input(sys1, 4000, 7)
+    ctx1.set_workspace(1 << 26)            # 64 MB: forces several target-row passes
+    try:
+        gk, gv, gd = gpu_hv(ctx1, k, v, len(k), 0.1, 1.0, 60_000_000)
+        st = ctx1.stats()
+    finally:
+        ctx1.set_workspace(8 << 30)
+    log("multipass: %d passes" % st["n_passes"])
+    assert st["n_passes"] > 1
+    ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=0.1, coeff=1.0, cut=1e-5)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits([gd])[0] == bits([od])[0]
+
+
+def test_hv_empty(ctx1):
+    vin, vout = ctx1.vector(10), ctx1.vector(10)
+    assert ctx1.hv(vin, vout, 10) == 0.0
+    assert vout.len == 0
+
+
+def test_energy_and_diag(sys1, orc1, ctx1):
+    k, v = random_input(sys1, 1500, 11)
+    vec = ctx1.vector(len(k), k, v)
+    assert bits([ctx1.energy(vec)])[0] == bits([orc1.energy(k, v)])[0]
+    for d in [sys1["init_hf"], int(k[0]), int(k[-1])]:
+        assert bits([ctx1.diag_element(d)])[0] == bits([orc1.diag(d)])[0]
+
+
+def test_vector_ops_bit_exact(sys1, ctx1):
+    rng = np.random.default_rng(3)
+    ak, av = random_input(sys1, 5000, 21, ties=True)
+    bk, bv = random_input(sys1, 7000, 22)
+    av[::17] = 0.0                                   # zeros are kept by add_two
+    A, B = ctx1.vector(20000, ak, av), ctx1.vector(20000, bk, bv)
+    assert bits([ctx1.dot(A, B)])[0] == bits([O.dot(ak, av, bk, bv)])[0]
+    assert bits([ctx1.norm(B)])[0] == bits([O.norm(bv)])[0]
+    for cap in [20000, 6000, 1234]:
+        V = ctx1.vector(cap, ak, av) if cap >= len(ak) else ctx1.vector(20000, ak, av)
+        W = ctx1.vector(cap)
+        if cap >= len(ak):
+            ctx1.add_two(V, B, -0.7)
+            gk, gv = V.download()
+            ok, ov = O.add_two(ak, av, bk, bv, -0.7, cap)
+        else:
+            M = ctx1.vector(20000, bk, bv)
+            ctx1.add_two(M, A, 0.3)
+            ctx1.shrink_from(W, M)
+            gk, gv = W.download()
+            mk, mv = O.add_two(bk, bv, ak, av, 0.3, 20000)
+            ok, ov = O.shrink(mk, mv, cap)
+        assert_same_vec(gk, gv, ok, ov)
+    ck, cv = random_input(sys1, 3000, 23)
+    Cv = ctx1.vector(3000, ck, cv)
+    coef = rng.normal(size=3)
+    out = ctx1.vector(4000)
+    ctx1.lincomb([A, B, Cv], coef, out)
+    gk, gv = out.download()
+    ok, ov = O.lincomb([(ak, av), (bk, bv), (ck, cv)], coef, 4000)
+    # the oracle's lincomb also normalizes; undo on the GPU side by the same steps
+    n = ctx1.norm(out)
+    ctx1.scale(out, 1.0 / n)
+    gk, gv = out.download()
+    assert_same_vec(gk, gv, ok, ov)
+    S = ctx1.vector(5000, ak, av)
+    ctx1.scale(S, -1.75)
+    sk, sv = S.download()
+    assert np.array_equal(bits(sv), bits(av * -1.75))
+
+
+def test_lanczos_matches_oracle(sys1, orc1, ctx1):
+    nd, nl, nr = 3000, 8, 3
+    log("lanczos gpu")
+    g = ctx1.ground_state(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000)
+    log("lanczos oracle")
+    o = orc1.lanczos(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000)
+    assert g["restarts"] == o["restarts"]
+    assert np.allclose(g["ritz"], o["ritz"], rtol=0, atol=1e-9)
+    assert abs(g["e_final"] - o["e_final"]) < 1e-9
+    log("ritz gpu %r oracle %r" % (g["ritz"], o["ritz"]))
