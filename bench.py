@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""H.v throughput of the SFCI hot path on one MI355X.
+
+One step = one call of the GPU H.v (SpMVMtply_bucket, src/functions.cpp:511-804) in FP64
+on an input vector already resident in HBM.  Workload (default): 26 spatial = 52 spin
+orbitals, 14 electrons, synthetic integrals seed 1, Nd = 1,000,000, output capacity
+60,000,000 (the reference's HPsiCap).  The input vector is built untimed on the GPU
+from the Hartree-Fock determinant: two rounds of H.v, top-Nd truncation and
+normalization, i.e. the second Krylov direction of the reference's driver at full Nd.
+
+The printed JSON line carries:
+  value          connections/s = n_src * (U + 1) / t_step   (U = connections per det)
+  roofline       the dominant kernel by time: its algorithmic bytes per launch over its
+                 average launch duration (HIP events on the library's stream)
+  cpu_baseline   the oracle (CPU restatement, OpenMP) on a bounded sample of the same
+                 input, timed on this host
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
+    os.path.join(ROOT, "BASELINE.json")) else "H.v connections/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_input(ctx, w, hf, nd, cap):
+    """HF -> H.v -> top-Nd -> normalize, twice; returns the device vector."""
+    v = ctx.vector(nd, np.array([hf], np.uint64), np.array([1.0]))
+    out = ctx.vector(cap)
+    for it in range(2):
+        ctx.hv(v, out, nd, 0.0, 1.0)
+        nxt = ctx.vector(nd)
+        ctx.shrink_from(nxt, out)
+        n = ctx.norm(nxt)
+        ctx.scale(nxt, 1.0 / n)
+        v = nxt
+        log("input build %d: len %d" % (it, v.len))
+    return v
+
+
+def cpu_baseline(w, keys, vals, n_conn, target_s=15.0, max_threads=16):
+    from oracle import oracle as O
+    orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    # calibrate on a small sample, then size the timed sample for ~target_s of CPU work
+    s0 = max(1, min(len(keys), int(2e6 // n_conn) + 1))
+    t = time.perf_counter()
+    orc.hv(keys[:s0], vals[:s0], cut=1e-5, nthreads=threads)
+    dt = time.perf_counter() - t
+    rate = s0 / max(dt, 1e-6)
+    s = int(min(len(keys), max(s0, rate * target_s), 1.2e8 // n_conn + 1))
+    t = time.perf_counter()
+    orc.hv(keys[:s], vals[:s], cut=1e-5, nthreads=threads)
+    dt = time.perf_counter() - t
+    return dict(value=s * n_conn / dt, unit="connections/s", cores=threads, kind="port",
+                sample="first %d of the %d input determinants (key order), all %d connections each, "
+                       "oracle H.v with %d OpenMP threads: %.2f s" % (s, len(keys), n_conn, threads, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", help="c1..c5 (spatial orbitals / electrons / seed / Nd)")
+    ap.add_argument("--nd", type=int, default=None, help="override Nd")
+    ap.add_argument("--cap", type=int, default=60_000_000, help="H.v output capacity (HPsiCap)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--data", default=None, help="directory for the generated integral files")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: multi-GPU sharding of H.v is described in DESIGN.md and not built",
+              file=sys.stderr)
+        sys.exit(2)
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    t0 = time.perf_counter()
+    w = I.make_workload(args.workload, data)
+    nd = args.nd or w["nd"]
+    log("integrals ready (%.1f s): %s" % (time.perf_counter() - t0, w["in_path"]))
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    info = ctx.info()
+    n_conn = info["n_conn"]
+    v = build_input(ctx, w, w["init_hf"], nd, args.cap)
+    out = ctx.vector(args.cap)
+    for i in range(args.warmup):
+        ctx.hv(v, out, nd, 0.0, 1.0)
+        log("warmup %d: %s" % (i, ctx.stats()))
+    ctx.sync()
+    ctx.kernel_stats(reset=True)
+    t = time.perf_counter()
+    tot_cand = tot_out = tot_uniq = 0
+    for i in range(args.steps):
+        ctx.hv(v, out, nd, 0.0, 1.0)
+        st = ctx.stats()
+        tot_cand += st["n_candidates"]
+        tot_out += st["n_out"]
+        tot_uniq += st["n_unique"]
+        log("step %d: %.1f ms (%d passes, %d candidates, %d unique, %d out)" % (
+            i, st["ms_total"], st["n_passes"], st["n_candidates"], st["n_unique"], st["n_out"]))
+    ctx.sync()
+    dt = time.perf_counter() - t
+    ks = ctx.kernel_stats()
+    n_in = v.len
+    conn = n_in * n_conn
+    ms_step = dt * 1e3 / args.steps
+    dom = max(ks.items(), key=lambda kv: kv[1]["ms"])
+    dom_name, d = dom
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    roofline = dict(bound="hbm", kernel=dom_name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    avg_launch_ms=d["ms"] / max(1, d["launches"]),
+                    bytes_per_launch=d["bytes"] / max(1, d["launches"]))
+    # whole-H.v bytes of an ideal single-pass design: e * (N_in + 2 M_c + M_out), e = 16
+    m_c = tot_cand / args.steps
+    m_out = tot_out / args.steps
+    b_hv = 16.0 * (n_in + 2 * m_c + m_out)
+    res = dict(metric=METRIC, value=conn / (ms_step * 1e-3), unit="connections/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=ms_step, higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="H.v, %d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, cap=%d"
+                           % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, args.cap),
+                           n_in=n_in, n_conn=n_conn, candidates=int(m_c), unique=int(tot_uniq / args.steps),
+                           n_out=int(m_out)),
+               roofline=roofline,
+               determinants_merged_per_s=m_c / (ks["k_reduce"]["ms"] * 1e-3 / args.steps + 1e-30),
+               hv_ideal_bytes_frac=b_hv / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               kernels={k: dict(ms_per_step=x["ms"] / args.steps, launches=x["launches"],
+                                gbs=(x["bytes"] / (x["ms"] * 1e-3) / 1e9) if x["ms"] > 0 else 0.0)
+                        for k, x in ks.items() if x["launches"]})
+    if not args.no_cpu:
+        keys, vals = v.download()
+        res["cpu_baseline"] = cpu_baseline(w, keys, vals, n_conn)
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,19 @@ int sfci_set_workspace(sfci_ctx* ctx, uint64_t bytes);
 int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
 int sfci_sync(sfci_ctx* ctx);
 
+/* Per-kernel HIP-event timings accumulated on the context's stream since the last reset
+ * (no reference counterpart: the reference only prints wall-clock timers,
+ * src/functions.cpp:800-801).  Kernel ids 0..SFCI_KERNEL_COUNT-1 are named by
+ * sfci_kernel_name(); bytes are the algorithmic bytes each launch reads and writes. */
+#define SFCI_KERNEL_COUNT 8
+typedef struct {
+    double ms;
+    uint64_t launches;
+    double bytes;
+} sfci_kernel_stat_t;
+int sfci_kernel_stats(sfci_ctx* ctx, sfci_kernel_stat_t* out, int n, int reset);
+const char* sfci_kernel_name(int id);
+
 /* Device sparse vector with fixed capacity: dynVec(cap) (src/all.h:170-179). */
 int sfci_vec_create(sfci_ctx* ctx, int64_t cap, sfci_vec** out);
 void sfci_vec_destroy(sfci_vec* v);

@@ -269,6 +269,27 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     return SFCI_OK;
 }
 
+int sfci_kernel_stats(sfci_ctx* X, sfci_kernel_stat_t* out, int n, int reset) {
+    if (!X || n < 0 || (n && !out)) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        kt_resolve(c);
+        for (int i = 0; i < n && i < K_NUM; ++i) {
+            out[i].ms = c->kclock.stat[i].ms;
+            out[i].launches = c->kclock.stat[i].launches;
+            out[i].bytes = c->kclock.stat[i].bytes;
+        }
+        if (reset)
+            for (auto& s : c->kclock.stat) s = KernelStat{};
+    });
+}
+
+const char* sfci_kernel_name(int id) {
+    static const char* names[K_NUM] = {"k_generate", "k_count", "k_radix_hist", "k_radix_scatter",
+                                       "scan",       "k_reduce", "select",      "vecops"};
+    return (id >= 0 && id < K_NUM) ? names[id] : "";
+}
+
 int sfci_sync(sfci_ctx* X) {
     if (!X) return SFCI_E_ARG;
     return guarded(&X->c, [&] { HIPCHK(hipStreamSynchronize(X->c.stream)); });

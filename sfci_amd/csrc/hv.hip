@@ -508,7 +508,9 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         } else {
             uint32_t* cnt = c->src_cnt.get<uint32_t>(ns);
             uint64_t* offs = c->src_off.get<uint64_t>(ns + 1);
+            int tc = kt_begin(c, K_COUNT, 12.0 * ns);
             k_count<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ns, row_lo, row_hi, cnt);
+            kt_end(c, tc);
             uint64_t* tot_dev = offs + ns;
             exclusive_scan_u32(c, cnt, ns, offs, tot_dev);
             HIPCHK(hipMemcpyAsync(&n_pass, tot_dev, 8, hipMemcpyDeviceToHost, c->stream));
@@ -522,8 +524,10 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         double* v1 = c->cand_v[1].get<double>(n_pass);
         {
             Timer t(c->stream);
+            int tg = kt_begin(c, K_GENERATE, 16.0 * n_pass + 16.0 * ns);
             k_generate<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo, row_hi,
                                                                      off, k0, v0, c->d_flags);
+            kt_end(c, tg);
             HIPCHK(hipGetLastError());
             st.ms_generate += t.stop();
         }
@@ -541,9 +545,11 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             uint64_t before = out_count_host;
             acc_clear(c, 1);
             int grid = (int)std::min<size_t>((n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 256 * 16);
+            int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
             k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, k0, v0, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff, stk,
                                                          stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
                                                          c->d_flags);
+            kt_end(c, tr);
             HIPCHK(hipGetLastError());
             uint64_t cnts[2];
             HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
@@ -604,6 +610,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         st.ms_select += t.stop();
     }
     st.n_out = kept;
+    kt_resolve(c);
     st.ms_total = total.stop();
     return kept;
 }

@@ -73,6 +73,37 @@ struct HamDev {
     uint32_t n_opair, n_vpair;
 };
 
+// Per-kernel timing with HIP events on the library stream (resolved after each call).
+enum KernelId : int {
+    K_GENERATE = 0,
+    K_COUNT,
+    K_RADIX_HIST,
+    K_RADIX_SCATTER,
+    K_SCAN,
+    K_REDUCE,
+    K_SELECT,
+    K_VECOPS,
+    K_NUM
+};
+
+struct KernelStat {
+    double ms = 0.0;
+    uint64_t launches = 0;
+    double bytes = 0.0;   // algorithmic bytes moved (reads + writes of the kernel's operands)
+};
+
+struct KernelClock {
+    struct Rec {
+        int id;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<Rec> recs;
+    KernelStat stat[K_NUM];
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -99,6 +130,7 @@ struct Ctx {
     uint32_t* d_flags = nullptr;
     uint64_t* d_counter = nullptr;         // 8 general counters
     std::string last_error;
+    KernelClock kclock;
     // statistics of the last H.v
     struct {
         uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0;
@@ -137,6 +169,12 @@ void dev_copy(Ctx* c, uint64_t* dk, double* dv, const uint64_t* sk, const double
 // (capacity out_cap, key-sorted) and the exact diag.
 size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
               size_t out_cap, uint64_t* ok, double* ov, double* diag);
+
+// kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
+// stream and folds the elapsed times into c->kclock.stat.
+int kt_begin(Ctx* c, int id, double bytes);
+void kt_end(Ctx* c, int rec);
+void kt_resolve(Ctx* c);
 
 // accumulator helpers
 void acc_clear(Ctx* c, int slot);

@@ -173,8 +173,12 @@ void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, do
     uint64_t* kout = ktmp;
     double* vout = vtmp;
     for (int shift = 0; shift < bits; shift += RADIX_BITS) {
+        int t0 = kt_begin(c, K_RADIX_HIST, 8.0 * n + 4.0 * RADIX * n_tiles);
         k_radix_hist<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, n, shift, hist, n_tiles);
+        kt_end(c, t0);
+        int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * n_tiles);
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
+        kt_end(c, t1);
         // digit boundaries: off[d * n_tiles] for each d
         HIPCHK(hipMemcpy2DAsync(bounds, sizeof(uint64_t), off, n_tiles * sizeof(uint64_t), sizeof(uint64_t), RADIX,
                                 hipMemcpyDeviceToDevice, c->stream));
@@ -185,7 +189,9 @@ void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, do
         for (int d = 0; d < RADIX; ++d)
             if (hb[d + 1] - hb[d] == n) single = true;
         if (single) continue;
+        int t2 = kt_begin(c, K_RADIX_SCATTER, 32.0 * n);
         k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, shift, off, n_tiles);
+        kt_end(c, t2);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

@@ -24,6 +24,48 @@ __device__ __forceinline__ uint64_t abs_bits(double x) {
     return b & 0x7fffffffffffffffull;
 }
 
+// ---- kernel timing -----------------------------------------------------------------
+static hipEvent_t kt_event(Ctx* c) {
+    auto& k = c->kclock;
+    if (k.used == k.pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        k.pool.push_back(e);
+    }
+    return k.pool[k.used++];
+}
+
+int kt_begin(Ctx* c, int id, double bytes) {
+    auto& k = c->kclock;
+    if (k.recs.size() >= 4096) kt_resolve(c);
+    KernelClock::Rec r{id, bytes, kt_event(c), nullptr};
+    HIPCHK(hipEventRecord(r.a, c->stream));
+    k.recs.push_back(r);
+    return (int)k.recs.size() - 1;
+}
+
+void kt_end(Ctx* c, int rec) {
+    auto& r = c->kclock.recs[rec];
+    r.b = kt_event(c);
+    HIPCHK(hipEventRecord(r.b, c->stream));
+}
+
+void kt_resolve(Ctx* c) {
+    auto& k = c->kclock;
+    if (k.recs.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& r : k.recs) {
+        if (!r.b) continue;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        k.stat[r.id].ms += ms;
+        k.stat[r.id].launches += 1;
+        k.stat[r.id].bytes += r.bytes;
+    }
+    k.recs.clear();
+    k.used = 0;
+}
+
 // ---- accumulators ------------------------------------------------------------------
 void acc_clear(Ctx* c, int slot) {
     HIPCHK(hipMemsetAsync(c->d_acc + 2 * slot, 0, 16, c->stream));

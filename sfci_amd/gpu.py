@@ -40,6 +40,10 @@ class StatsT(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class KernelStatT(C.Structure):
+    _fields_ = [("ms", C.c_double), ("launches", C.c_uint64), ("bytes", C.c_double)]
+
+
 class LanczosParamsT(C.Structure):
     _fields_ = [("nd", C.c_int64), ("nl", C.c_int), ("num_restart", C.c_int), ("hpsi_cap", C.c_int64),
                 ("delta_e", C.c_double), ("verbose", C.c_int)]
@@ -87,6 +91,8 @@ def lib():
         "sfci_set_workspace": (ip, [vp, u64]),
         "sfci_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
         "sfci_sync": (ip, [vp]),
+        "sfci_kernel_stats": (ip, [vp, C.POINTER(KernelStatT), ip, ip]),
+        "sfci_kernel_name": (C.c_char_p, [ip]),
         "sfci_vec_create": (ip, [vp, i64, pvp]),
         "sfci_vec_destroy": (None, [vp]),
         "sfci_vec_len": (i64, [vp]),
@@ -202,6 +208,13 @@ class Context:
         s = StatsT()
         self._check(lib().sfci_hv_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in StatsT._fields_}
+
+    def kernel_stats(self, reset: bool = False) -> dict:
+        n = 8
+        arr = (KernelStatT * n)()
+        self._check(lib().sfci_kernel_stats(self._h, arr, n, 1 if reset else 0))
+        return {lib().sfci_kernel_name(i).decode(): dict(ms=arr[i].ms, launches=arr[i].launches,
+                                                           bytes=arr[i].bytes) for i in range(n)}
 
     def sync(self):
         self._check(lib().sfci_sync(self._h))
