@@ -1,0 +1,82 @@
+"""Integral files: synthetic generator spec, FCIDUMP -> _IN converter against the
+reference converter's own output (golden fixtures), and the loader."""
+import filecmp
+import os
+
+import numpy as np
+import pytest
+
+from sfci_amd import integrals as I
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("case", ["g3o2e", "g4o4e"])
+def test_converter_matches_reference_bytes(tmp_path, case):
+    out = tmp_path / (case + "_IN")
+    I.convert_rhf(os.path.join(GOLDEN, case + "_fcidump"), str(out))
+    assert filecmp.cmp(str(out), os.path.join(GOLDEN, case + "_IN"), shallow=False)
+
+
+def test_synthetic_generator_spec():
+    h, canon, ec = I.synth_integrals(12, 0)
+    n = 12
+    assert ec == 0.0
+    assert np.allclose(np.diag(h), np.linspace(-2.0, 1.5, n))
+    assert np.array_equal(h, h.T)
+    assert len(canon) == (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+    for v, i, j, k, l in canon:
+        assert i >= j and k >= l and i * (i + 1) // 2 + j >= k * (k + 1) // 2 + l
+        if i == j and k == l:
+            assert 0.3 <= v <= 0.8
+    # deterministic
+    h2, canon2, _ = I.synth_integrals(12, 0)
+    assert np.array_equal(h, h2) and canon == canon2
+    h3, _, _ = I.synth_integrals(12, 1)
+    assert not np.array_equal(h, h3)
+    h4, _, _ = I.synth_integrals(50, 3, eps_lo=-1.0, eps_hi=0.5, sigma_g=0.04)
+    assert np.isclose(h4[0, 0], -1.0) and np.isclose(h4[-1, -1], 0.5)
+
+
+def test_fcidump_roundtrip(tmp_path):
+    h, canon, ec = I.synth_integrals(5, 3)
+    p = str(tmp_path / "x_fcidump")
+    I.write_fcidump(p, h, canon, 4, ec)
+    h2, eri, ec2, n = I.read_fcidump(p)
+    assert n == 5 and ec2 == ec and np.array_equal(h2, h)
+    for v, i, j, k, l in canon:
+        for a, b, c, d in ((i, j, k, l), (j, i, l, k), (k, l, i, j), (l, k, j, i)):
+            assert eri[a, b, c, d] == v
+
+
+def test_in_file_entry_counts(make_system):
+    # every spin-allowed (mn|pq) with m != p and n != q (same spin), all permutations once
+    s = make_system(3, 2, 0)
+    lines = open(s["in_path"]).read().split("\n")
+    i, blocks = 0, []
+    while i < len(lines) and lines[i].strip():
+        c = int(lines[i])
+        blocks.append(lines[i + 1:i + 1 + c])
+        i += 1 + c
+    assert len(blocks) == 7
+    n = 3
+    # one-body: n*(n+1)/2 lines, off-diagonal doubled, per spin
+    assert len(blocks[4]) == len(blocks[5]) == n * n
+    assert len(blocks[6]) == 1
+    entries = set()
+    for b in blocks[:4]:
+        for line in b:
+            t = line.split()
+            entries.add(tuple(int(x) for x in t[1:]))
+    nso = 2 * n
+    expect = 0
+    for m in range(nso):
+        for nn in range(nso):
+            for p in range(nso):
+                for q in range(nso):
+                    if (m & 1) != (nn & 1) or (p & 1) != (q & 1):
+                        continue
+                    if (m & 1) == (p & 1) and (m == p or nn == q):
+                        continue
+                    expect += 1
+    assert len(entries) == expect
