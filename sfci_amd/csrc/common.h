@@ -217,7 +217,7 @@ __host__ __device__ __forceinline__ uint32_t colex_unrank(uint64_t r, int k, int
     uint32_t s = 0;
     int pos = n_orb - 1;
     for (int t = k; t >= 1; --t) {
-        while (binom[pos * 33 + t] > r) --pos;
+        while (pos > 0 && binom[pos * 33 + t] > r) --pos;
         s |= 1u << pos;
         r -= binom[pos * 33 + t];
         --pos;

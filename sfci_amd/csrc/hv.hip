@@ -298,8 +298,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_
             }
             if (all_in) {
                 if (have) {
-                    ok[pos + e - e0] = ((c.dense - base_dense) << 1) | c.tag;
-                    ov[pos + e - e0] = c.val;
+                    ok[pos + e] = ((c.dense - base_dense) << 1) | c.tag;
+                    ov[pos + e] = c.val;
                 }
                 continue;
             }
