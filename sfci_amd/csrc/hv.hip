@@ -325,7 +325,46 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_
     if (range_bad) atomicOr(flags, FLAG_RANGE);
 }
 
-// Exact per-target merge of sorted candidates.  Head threads walk their segment.
+// Exact per-target merge of sorted candidates (segmented reduction).
+//
+// Block b owns the targets whose first candidate lies in tile b (RED_TILE elements); it
+// reads on past the tile end to finish its last target.  The owned range is consumed in
+// chunks of RED_CHUNK: every thread sums RED_IPT consecutive candidates sequentially,
+// then a block-wide segmented scan over threads carries partial sums of targets that
+// straddle threads (and chunks).  Sums are exact 128-bit fixed point, so the grouping
+// does not affect the result.
+constexpr int RED_IPT = 4;
+constexpr int RED_CHUNK = SFCI_BLOCK * RED_IPT;
+constexpr int RED_TILE = 4 * RED_CHUNK;
+
+struct SegVal {
+    fx128 v;
+    uint32_t f;   // a target starts inside the range
+    uint32_t t;   // a diagonal (input) contribution seen
+};
+
+__device__ __forceinline__ SegVal seg_combine(const SegVal& a, const SegVal& b) {
+    SegVal r;
+    r.f = a.f | b.f;
+    if (b.f) {
+        r.v = b.v;
+        r.t = b.t;
+    } else {
+        r.v = fx_add(a.v, b.v);
+        r.t = a.t | b.t;
+    }
+    return r;
+}
+
+__device__ __forceinline__ SegVal seg_shfl_up(const SegVal& x, int off) {
+    SegVal r;
+    r.v.lo = __shfl_up(x.v.lo, off, SFCI_WAVE);
+    r.v.hi = __shfl_up(x.v.hi, off, SFCI_WAVE);
+    r.f = __shfl_up(x.f, off, SFCI_WAVE);
+    r.t = __shfl_up(x.t, off, SFCI_WAVE);
+    return r;
+}
+
 __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t* __restrict__ ck,
                                                        const double* __restrict__ cv, size_t n, uint64_t base_dense,
                                                        const uint64_t* __restrict__ ik, const double* __restrict__ iv,
@@ -333,68 +372,156 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                                                        double* __restrict__ sv_out, unsigned long long* out_count,
                                                        size_t out_cap, unsigned long long* unique_count,
                                                        unsigned long long* diag_acc, uint32_t* flags) {
-    fx128 dacc = fx_zero();
-    const int lane = threadIdx.x & (SFCI_WAVE - 1);
+    __shared__ unsigned long long s_start, s_end;
+    __shared__ SegVal s_wave[SFCI_BLOCK / SFCI_WAVE];
+    __shared__ SegVal s_incl[SFCI_BLOCK];
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    fx128 dacc = fx_zero();
     unsigned long long uniq = 0;
-    size_t stride = (size_t)gridDim.x * SFCI_BLOCK;
-    size_t n_round = (n + stride - 1) / stride * stride;
-    for (size_t i = (size_t)blockIdx.x * SFCI_BLOCK + threadIdx.x; i < n_round; i += stride) {
-        bool emit = false;
-        uint64_t det = 0;
-        double y = 0.0;
-        if (i < n) {
-            uint64_t key = ck[i] >> 1;
-            bool head = (i == 0) || ((ck[i - 1] >> 1) != key);
-            if (head) {
-                ++uniq;
-                fx128 acc = fx_zero();
-                uint32_t tag = 0;
-                size_t j = i;
-                while (j < n && (ck[j] >> 1) == key) {
-                    acc = fx_add(acc, fx_from(cv[j]));
-                    tag |= (uint32_t)(ck[j] & 1ull);
-                    ++j;
-                }
-                y = fx_to(acc);
-                uint64_t dense = key + base_dense;
-                uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-                uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
-                uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
-                det = spread_string(s0) | (spread_string(s1) << 1);
-                if (tag) {
-                    size_t lo = 0, hi = n_in;
-                    while (lo < hi) {
-                        size_t mid = (lo + hi) >> 1;
-                        if (ik[mid] < det) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    if (lo < n_in && ik[lo] == det) {
-                        double p = y * iv[lo];
-                        dacc = fx_add(dacc, fx_from(p));
-                    } else {
-                        atomicOr(flags, FLAG_BAD_INPUT);
-                    }
-                }
-                emit = fabs(y) > cut;
+    bool bad_input = false, overflow = false;
+    for (size_t tile = blockIdx.x; tile * RED_TILE < n; tile += gridDim.x) {
+        const size_t tb = tile * RED_TILE, te = (tb + RED_TILE < n) ? tb + RED_TILE : n;
+        if (tid == 0) {
+            s_start = te;
+            s_end = n;
+        }
+        __syncthreads();
+        for (size_t i = tb + tid; i < te; i += SFCI_BLOCK) {
+            if (i == 0 || (ck[i] >> 1) != (ck[i - 1] >> 1)) {
+                atomicMin(&s_start, (unsigned long long)i);
+                break;
             }
         }
-        uint64_t bal = __ballot(emit);
-        if (bal) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(out_count, (unsigned long long)popc64(bal));
-            base = __shfl(base, 0, SFCI_WAVE);
-            if (emit) {
-                uint64_t pos = base + (uint64_t)popc64(bal & lt);
-                if (pos < out_cap) {
-                    sk_out[pos] = det;
-                    sv_out[pos] = y;
+        for (size_t i0 = te; i0 < n; i0 += SFCI_BLOCK) {
+            size_t i = i0 + tid;
+            if (i < n && (ck[i] >> 1) != (ck[i - 1] >> 1)) atomicMin(&s_end, (unsigned long long)i);
+            __syncthreads();
+            bool done = s_end < n;
+            __syncthreads();
+            if (done) break;
+        }
+        __syncthreads();
+        const size_t start = s_start, end = s_end;
+        __syncthreads();
+        if (start >= te) continue;   // the whole tile continues a target owned by an earlier block
+        SegVal carry{fx_zero(), 0u, 0u};
+        for (size_t c0 = start; c0 < end; c0 += RED_CHUNK) {
+            const size_t b = c0 + (size_t)tid * RED_IPT;
+            fx128 first = fx_zero(), tailv = fx_zero();
+            uint32_t ftag = 0, ttag = 0;
+            bool has_head = false, first_ends = false;
+            uint64_t first_key = 0;
+            uint64_t ekey[RED_IPT];
+            fx128 eval[RED_IPT];
+            uint32_t etag[RED_IPT];
+            int ne = 0;
+#pragma unroll
+            for (int r = 0; r < RED_IPT; ++r) {
+                size_t i = b + r;
+                if (i >= end) break;
+                uint64_t raw = ck[i];
+                uint64_t key = raw >> 1;
+                bool head = (i == start) || (key != (ck[i - 1] >> 1));
+                bool ends = (i + 1 == end) || ((ck[i + 1] >> 1) != key);
+                fx128 x = fx_from(cv[i]);
+                uint32_t tg = (uint32_t)(raw & 1ull);
+                if (head) {
+                    has_head = true;
+                    tailv = x;
+                    ttag = tg;
+                } else if (!has_head) {
+                    first = fx_add(first, x);
+                    ftag |= tg;
                 } else {
-                    atomicOr(flags, FLAG_OUT_OVERFLOW);
+                    tailv = fx_add(tailv, x);
+                    ttag |= tg;
+                }
+                if (ends) {
+                    if (has_head) {
+                        ekey[ne] = key;
+                        eval[ne] = tailv;
+                        etag[ne] = ttag;
+                        ++ne;
+                    } else {
+                        first_ends = true;
+                        first_key = key;
+                    }
+                }
+            }
+            // segmented inclusive scan over threads, seeded with the chunk carry
+            SegVal x{has_head ? tailv : first, has_head ? 1u : 0u, has_head ? ttag : ftag};
+            for (int off = 1; off < SFCI_WAVE; off <<= 1) {
+                SegVal o = seg_shfl_up(x, off);
+                if (lane >= off) x = seg_combine(o, x);
+            }
+            if (lane == SFCI_WAVE - 1) s_wave[w] = x;
+            __syncthreads();
+            SegVal pre = carry;
+            for (int j = 0; j < w; ++j) pre = seg_combine(pre, s_wave[j]);
+            s_incl[tid] = seg_combine(pre, x);
+            __syncthreads();
+            const SegVal cin = tid > 0 ? s_incl[tid - 1] : carry;
+            carry = s_incl[SFCI_BLOCK - 1];
+            __syncthreads();
+            if (first_ends) {
+                // the target that entered this thread ends here: prepend the carried part
+                for (int q = ne; q > 0; --q) {
+                    ekey[q] = ekey[q - 1];
+                    eval[q] = eval[q - 1];
+                    etag[q] = etag[q - 1];
+                }
+                ekey[0] = first_key;
+                eval[0] = fx_add(cin.v, first);
+                etag[0] = cin.t | ftag;
+                ++ne;
+            }
+            // finish the targets: round, diagonal contribution, cut, append
+#pragma unroll
+            for (int r = 0; r < RED_IPT; ++r) {
+                bool emit = false;
+                uint64_t det = 0;
+                double y = 0.0;
+                if (r < ne) {
+                    ++uniq;
+                    y = fx_to(eval[r]);
+                    uint64_t dense = ekey[r] + base_dense;
+                    uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+                    uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
+                    uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
+                    det = spread_string(s0) | (spread_string(s1) << 1);
+                    if (etag[r]) {
+                        size_t lo = 0, hi = n_in;
+                        while (lo < hi) {
+                            size_t mid = (lo + hi) >> 1;
+                            if (ik[mid] < det) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        if (lo < n_in && ik[lo] == det) dacc = fx_add(dacc, fx_from(y * iv[lo]));
+                        else bad_input = true;
+                    }
+                    emit = fabs(y) > cut;
+                }
+                uint64_t bal = __ballot(emit);
+                if (bal) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(out_count, (unsigned long long)popc64(bal));
+                    base = __shfl(base, 0, SFCI_WAVE);
+                    if (emit) {
+                        uint64_t pos = base + (uint64_t)popc64(bal & lt);
+                        if (pos < out_cap) {
+                            sk_out[pos] = det;
+                            sv_out[pos] = y;
+                        } else {
+                            overflow = true;
+                        }
+                    }
                 }
             }
         }
     }
+    if (bad_input) atomicOr(flags, FLAG_BAD_INPUT);
+    if (overflow) atomicOr(flags, FLAG_OUT_OVERFLOW);
     fx_block_reduce_atomic<SFCI_BLOCK>(dacc, diag_acc);
     for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) uniq += __shfl_down(uniq, off, SFCI_WAVE);
     if (lane == 0 && uniq) atomicAdd(unique_count, uniq);
@@ -544,7 +671,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             Timer t(c->stream);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
-            int grid = (int)std::min<size_t>((n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 256 * 16);
+            int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
             int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
             k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, k0, v0, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff, stk,
                                                          stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
