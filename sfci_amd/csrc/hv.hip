@@ -485,11 +485,14 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                 if (r < ne) {
                     ++uniq;
                     y = fx_to(eval[r]);
-                    uint64_t dense = ekey[r] + base_dense;
-                    uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-                    uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
-                    uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
-                    det = spread_string(s0) | (spread_string(s1) << 1);
+                    emit = fabs(y) > cut;
+                    if (emit || etag[r]) {
+                        uint64_t dense = ekey[r] + base_dense;
+                        uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+                        uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
+                        uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
+                        det = spread_string(s0) | (spread_string(s1) << 1);
+                    }
                     if (etag[r]) {
                         size_t lo = 0, hi = n_in;
                         while (lo < hi) {
@@ -500,7 +503,6 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                         if (lo < n_in && ik[lo] == det) dacc = fx_add(dacc, fx_from(y * iv[lo]));
                         else bad_input = true;
                     }
-                    emit = fabs(y) > cut;
                 }
                 uint64_t bal = __ballot(emit);
                 if (bal) {

@@ -6,7 +6,7 @@
 //   scatter - stable in-tile ranking: each round of 256 consecutive items is ranked
 //             with a 64-lane digit match built from 8 ballots, wave counts are
 //             combined in wave order through LDS, so items keep their input order.
-// A pass whose items all share one digit is skipped (identity permutation).
+// Passes run back to back on the stream with no host synchronization.
 #include "internal.h"
 
 namespace sfci {
@@ -115,25 +115,51 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __res
     hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one tile.  Items are ranked round by round (256 consecutive items
+// per round, 8 ballots build each lane's digit-peer mask), placed in LDS in tile-local
+// digit order, then written out so that consecutive threads write consecutive slots of
+// each digit's output run.
 __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                               const double* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, double* __restrict__ vout,
                                                               size_t n, int shift, const uint64_t* __restrict__ off,
-                                                              size_t n_tiles) {
+                                                              const uint32_t* __restrict__ hist, size_t n_tiles) {
     constexpr int NW = SFCI_BLOCK / SFCI_WAVE;
-    __shared__ uint64_t s_base[RADIX];
+    __shared__ uint64_t s_key[SORT_TILE];
+    __shared__ double s_val[SORT_TILE];
+    __shared__ uint64_t s_gbase[RADIX];     // global start of each digit's run for this tile
+    __shared__ uint32_t s_lstart[RADIX];    // tile-local start of each digit
+    __shared__ uint32_t s_run[RADIX];       // items of each digit placed so far
     __shared__ uint32_t s_cnt[NW][RADIX];
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
-    s_base[tid] = off[(size_t)tid * n_tiles + blockIdx.x];
-    for (int i = 0; i < NW; ++i) s_cnt[i][tid] = 0;
-    __syncthreads();
+    const size_t tile = blockIdx.x;
+    {
+        uint32_t h = hist[(size_t)tid * n_tiles + tile];
+        // exclusive scan of the tile histogram over digits (one digit per thread)
+        uint32_t x = h;
+        for (int o = 1; o < SFCI_WAVE; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, SFCI_WAVE);
+            if (lane >= o) x += y;
+        }
+        if (lane == SFCI_WAVE - 1) s_cnt[0][w] = x;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int j = 0; j < w; ++j) pre += s_cnt[0][j];
+        s_lstart[tid] = pre + x - h;
+        s_gbase[tid] = off[(size_t)tid * n_tiles + tile];
+        s_run[tid] = 0;
+        __syncthreads();
+        for (int j = 0; j < NW; ++j) s_cnt[j][tid] = 0;
+        __syncthreads();
+    }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    size_t base = (size_t)blockIdx.x * SORT_TILE;
+    const size_t base = tile * SORT_TILE;
+    const int n_here = (int)((n - base) < (size_t)SORT_TILE ? (n - base) : (size_t)SORT_TILE);
     for (int r = 0; r < SORT_ROUNDS; ++r) {
-        size_t i = base + (size_t)r * SFCI_BLOCK + tid;
-        bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0ull;
-        double v = valid ? vin[i] : 0.0;
+        int li = r * SFCI_BLOCK + tid;
+        bool valid = li < n_here;
+        uint64_t k = valid ? kin[base + li] : 0ull;
+        double v = valid ? vin[base + li] : 0.0;
         uint32_t d = (uint32_t)(k >> shift) & (RADIX - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -145,16 +171,26 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __
         if (valid && rank == 0) s_cnt[w][d] = (uint32_t)popc64(peers);
         __syncthreads();
         if (valid) {
-            uint64_t pos = s_base[d] + rank;
+            uint32_t pos = s_lstart[d] + s_run[d] + rank;
             for (int j = 0; j < w; ++j) pos += s_cnt[j][d];
-            kout[pos] = k;
-            vout[pos] = v;
+            s_key[pos] = k;
+            s_val[pos] = v;
         }
         __syncthreads();
         uint32_t tot = 0;
-        for (int j = 0; j < NW; ++j) { tot += s_cnt[j][tid]; s_cnt[j][tid] = 0; }
-        s_base[tid] += tot;
+        for (int j = 0; j < NW; ++j) {
+            tot += s_cnt[j][tid];
+            s_cnt[j][tid] = 0;
+        }
+        s_run[tid] += tot;
         __syncthreads();
+    }
+    for (int p = tid; p < n_here; p += SFCI_BLOCK) {
+        uint64_t k = s_key[p];
+        uint32_t d = (uint32_t)(k >> shift) & (RADIX - 1);
+        uint64_t g = s_gbase[d] + (uint64_t)(p - s_lstart[d]);
+        kout[g] = k;
+        vout[g] = s_val[p];
     }
 }
 
@@ -166,8 +202,6 @@ void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, do
     size_t n_tiles = (n + SORT_TILE - 1) / SORT_TILE;
     uint32_t* hist = c->scratch_hist.get<uint32_t>((size_t)RADIX * n_tiles);
     uint64_t* off = c->scratch_off.get<uint64_t>((size_t)RADIX * n_tiles + 1);
-    uint64_t* bounds = c->scratch_small.get<uint64_t>(RADIX + 1);
-    std::vector<uint64_t> hb(RADIX + 1);
     uint64_t* kin = *keys;
     double* vin = *vals;
     uint64_t* kout = ktmp;
@@ -179,18 +213,8 @@ void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, do
         int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * n_tiles);
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
         kt_end(c, t1);
-        // digit boundaries: off[d * n_tiles] for each d
-        HIPCHK(hipMemcpy2DAsync(bounds, sizeof(uint64_t), off, n_tiles * sizeof(uint64_t), sizeof(uint64_t), RADIX,
-                                hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(hb.data(), bounds, RADIX * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        hb[RADIX] = n;
-        bool single = false;
-        for (int d = 0; d < RADIX; ++d)
-            if (hb[d + 1] - hb[d] == n) single = true;
-        if (single) continue;
         int t2 = kt_begin(c, K_RADIX_SCATTER, 32.0 * n);
-        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, shift, off, n_tiles);
+        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, shift, off, hist, n_tiles);
         kt_end(c, t2);
         std::swap(kin, kout);
         std::swap(vin, vout);
