@@ -36,7 +36,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_input(ctx, w, hf, nd, cap, max_rounds=8):
+def build_input(ctx, w, hf, nd, cap, max_rounds=12):
     """HF -> (H.v -> top-Nd -> normalize) repeated until the vector holds Nd determinants
     (or stops growing); returns the device vector."""
     v = ctx.vector(nd, np.array([hf], np.uint64), np.array([1.0]))

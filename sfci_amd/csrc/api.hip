@@ -115,13 +115,18 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
             for (int j = i + 1; j < k; ++j) op.push_back((uint16_t)(i | (j << 8)));
         for (int i = 0; i < v; ++i)
             for (int j = i + 1; j < v; ++j) vp.push_back((uint16_t)(i | (j << 8)));
+        std::vector<uint16_t> sd;
+        for (int x = 0; x < k * v; ++x) sd.push_back((uint16_t)((x / v) | ((x % v) << 8)));
         if (op.empty()) op.push_back(0);
         if (vp.empty()) vp.push_back(0);
+        if (sd.empty()) sd.push_back(0);
         HIPCHK(hipMalloc(&c.d_h, h.size() * 8));
         HIPCHK(hipMalloc(&c.d_g, g.size() * 8));
         HIPCHK(hipMalloc(&c.d_binom, c.h_binom.size() * 8));
         HIPCHK(hipMalloc(&c.d_occ_pairs, op.size() * 2));
         HIPCHK(hipMalloc(&c.d_vir_pairs, vp.size() * 2));
+        HIPCHK(hipMalloc(&c.d_sdec, sd.size() * 2));
+        HIPCHK(hipMemcpy(c.d_sdec, sd.data(), sd.size() * 2, hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&c.d_acc, 16 * 8));
         HIPCHK(hipMalloc(&c.d_flags, 16));
         HIPCHK(hipMalloc(&c.d_counter, 8 * 8));
@@ -221,7 +226,7 @@ void sfci_destroy(sfci_ctx* X) {
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();
     for (auto& b : c.scratch_scan) b.release();
-    void* ptrs[] = {c.d_h, c.d_g, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_acc, c.d_flags, c.d_counter};
+    void* ptrs[] = {c.d_h, c.d_g, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_sdec, c.d_acc, c.d_flags, c.d_counter};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.stream) (void)hipStreamDestroy(c.stream);

@@ -33,6 +33,8 @@ __device__ __forceinline__ double h_so(const HamDev& H, int m, int n) {
     return H.h[(size_t)(m & 1) * N * N + (size_t)(m >> 1) * N + (n >> 1)];
 }
 
+constexpr int MAX_SINGLES = (SFCI_MAX_ORB / 2) * (SFCI_MAX_ORB / 2);   // k*v <= 16*16 for 32 orbitals
+
 struct SrcLds {
     uint64_t key;
     double temp;        // coeff * c_j
@@ -43,40 +45,85 @@ struct SrcLds {
     uint8_t vir[2][SFCI_MAX_ORB];
     uint8_t occ_so[2 * SFCI_MAX_K];   // all occupied spin orbitals, ascending
     int n_occ_so;
+    uint64_t srank[2][MAX_SINGLES];   // rank of the spin-s string after single excitation e
+    uint16_t alist[MAX_SINGLES];      // spin-0 singles whose target row is in the pass range
+    int n_alist;
+    int row_in;                       // the source's own row is in the pass range
 };
 
-__device__ void load_source(const HamDev& H, SrcLds& s, uint64_t key, double temp, double lambda, uint32_t* flags) {
-    // called by thread 0
-    s.key = key;
-    s.temp = temp;
-    for (int sp = 0; sp < 2; ++sp) {
-        uint32_t st = spin_string(key, sp);
-        s.str[sp] = st;
-        s.rank[sp] = colex_rank(st, H.binom);
-        int a = 0, b = 0;
-        for (int i = 0; i < H.n_orb; ++i) {
-            if (st >> i & 1u) s.occ[sp][a++] = (uint8_t)i;
-            else s.vir[sp][b++] = (uint8_t)i;
+// One-time per-source staging.  Thread 0 builds the orbital lists; all threads then
+// compute single-excitation ranks and the exact diagonal cooperatively.
+__device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, uint64_t key, double temp,
+                            double lambda, uint64_t row_lo, uint64_t row_hi, uint32_t* flags) {
+    __shared__ uint64_t s_dlo[SFCI_BLOCK / SFCI_WAVE], s_dhi[SFCI_BLOCK / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s.key = key;
+        s.temp = temp;
+        for (int sp = 0; sp < 2; ++sp) {
+            uint32_t st = spin_string(key, sp);
+            s.str[sp] = st;
+            s.rank[sp] = colex_rank(st, H.binom);
+            int a = 0, b = 0;
+            for (int i = 0; i < H.n_orb; ++i) {
+                if (st >> i & 1u) s.occ[sp][a++] = (uint8_t)i;
+                else s.vir[sp][b++] = (uint8_t)i;
+            }
         }
+        int n = 0;
+        for (int p = 0; p < 2 * H.n_orb; ++p)
+            if (key >> p & 1ull) s.occ_so[n++] = (uint8_t)p;
+        s.n_occ_so = n;
+        s.row_in = (s.rank[0] >= row_lo && s.rank[0] < row_hi) ? 1 : 0;
+        s.n_alist = 0;
     }
-    int n = 0;
-    for (int p = 0; p < 2 * H.n_orb; ++p)
-        if (key >> p & 1ull) s.occ_so[n++] = (uint8_t)p;
-    s.n_occ_so = n;
-    // diagonal element, exact: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)]
-    fx128 acc = fx_from(H.constant);
-    for (int a = 0; a < n; ++a) acc = fx_add(acc, fx_from(h_so(H, s.occ_so[a], s.occ_so[a])));
-    for (int a = 0; a < n; ++a)
-        for (int b = 0; b < n; ++b) {
-            if (a == b) continue;
-            int m = s.occ_so[a], p = s.occ_so[b];
+    __syncthreads();
+    // single-excitation target ranks, both spins
+    const int nsa = (int)H.n_sa;
+    for (int e = tid; e < 2 * nsa; e += SFCI_BLOCK) {
+        int sp = e >= nsa ? 1 : 0, x = e - sp * nsa;
+        uint16_t dd = sdec[x];
+        uint32_t st = s.str[sp] ^ (1u << s.occ[sp][dd & 0xff]) ^ (1u << s.vir[sp][dd >> 8]);
+        s.srank[sp][x] = colex_rank(st, H.binom);
+    }
+    // exact diagonal: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)]
+    fx128 acc = fx_zero();
+    const int n = s.n_occ_so;
+    for (int t = tid; t < n * n; t += SFCI_BLOCK) {
+        int a = t / n, b = t - a * n;
+        int m = s.occ_so[a];
+        if (a == b) {
+            acc = fx_add(acc, fx_from(h_so(H, m, m)));
+        } else {
+            int p = s.occ_so[b];
             acc = fx_add(acc, fx_from(0.5 * g_so(H, m, m, p, p)));
             acc = fx_add(acc, fx_neg(fx_from(0.5 * g_so(H, m, p, p, m))));
         }
-    double e = fx_to(acc);
-    double dv = (e - lambda) * temp;
-    if (!fx_in_range(dv)) atomicOr(flags, FLAG_RANGE);
-    s.diag_val = dv;
+    }
+    for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) {
+        fx128 o;
+        o.lo = __shfl_down(acc.lo, off, SFCI_WAVE);
+        o.hi = __shfl_down(acc.hi, off, SFCI_WAVE);
+        acc = fx_add(acc, o);
+    }
+    if ((tid & (SFCI_WAVE - 1)) == 0) {
+        s_dlo[tid / SFCI_WAVE] = acc.lo;
+        s_dhi[tid / SFCI_WAVE] = acc.hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        fx128 t = fx_from(H.constant);
+        for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) t = fx_add(t, fx128{s_dlo[i], s_dhi[i]});
+        double dv = (fx_to(t) - lambda) * temp;
+        if (!fx_in_range(dv)) atomicOr(flags, FLAG_RANGE);
+        s.diag_val = dv;
+        // spin-0 singles whose target row is in range (ascending order)
+        int m = 0;
+        for (int x = 0; x < nsa; ++x)
+            if (s.srank[0][x] >= row_lo && s.srank[0][x] < row_hi) s.alist[m++] = (uint16_t)x;
+        s.n_alist = m;
+    }
+    __syncthreads();
 }
 
 // <D'|H|D>, D' = D with spin orbital n -> m
@@ -101,84 +148,20 @@ __device__ __forceinline__ double double_elem(const HamDev& H, uint64_t d, int n
     return s1 * g_so(H, x, n, y, q) + s2 * g_so(H, y, n, x, q);
 }
 
-struct Cand {
-    uint64_t target;       // determinant key
-    uint64_t dense;        // rank0 * dim + rank1
-    uint64_t row;          // rank0
-    double val;
-    uint32_t tag;
-};
-
-// Connection e (0 <= e < n_conn) of the source in LDS.  Only computes the value when
-// `want_val`.
-__device__ __forceinline__ void connection(const HamDev& H, const SrcLds& s, uint64_t e, bool want_val, Cand& c) {
-    const uint64_t one = 1;
-    const uint64_t d = s.key;
-    c.tag = 0;
-    if (e == 0) {
-        c.target = d;
-        c.row = s.rank[0];
-        c.dense = s.rank[0] * H.dim + s.rank[1];
-        c.val = s.diag_val;
-        c.tag = 1;
-        return;
-    }
-    e -= 1;
-    const uint64_t v = (uint64_t)H.v;
-    if (e < 2 * H.n_sa) {
-        int sp = e >= H.n_sa ? 1 : 0;
-        uint64_t idx = e - (uint64_t)sp * H.n_sa;
-        int ii = (int)(idx / v), aa = (int)(idx - (uint64_t)ii * v);
-        int i = s.occ[sp][ii], a = s.vir[sp][aa];
-        int n = 2 * i + sp, m = 2 * a + sp;
-        c.target = d - (one << n) + (one << m);
-        uint32_t st = s.str[sp] ^ (1u << i) ^ (1u << a);
-        uint64_t r = colex_rank(st, H.binom);
-        uint64_t r0 = sp == 0 ? r : s.rank[0], r1 = sp == 0 ? s.rank[1] : r;
-        c.row = r0;
-        c.dense = r0 * H.dim + r1;
-        c.val = want_val ? single_elem(H, s, n, m) * s.temp : 0.0;
-        return;
-    }
-    e -= 2 * H.n_sa;
-    if (e < 2 * H.n_daa) {
-        int sp = e >= H.n_daa ? 1 : 0;
-        uint64_t idx = e - (uint64_t)sp * H.n_daa;
-        uint64_t ij = idx / H.n_vpair, ab = idx - ij * H.n_vpair;
-        uint16_t op = H.occ_pairs[ij], vp = H.vir_pairs[ab];
-        int i = s.occ[sp][op & 0xff], j = s.occ[sp][op >> 8];
-        int a = s.vir[sp][vp & 0xff], b = s.vir[sp][vp >> 8];
-        int n = 2 * i + sp, q = 2 * j + sp, x = 2 * a + sp, y = 2 * b + sp;
-        c.target = d - (one << n) - (one << q) + (one << x) + (one << y);
-        uint32_t st = s.str[sp] ^ (1u << i) ^ (1u << j) ^ (1u << a) ^ (1u << b);
-        uint64_t r = colex_rank(st, H.binom);
-        uint64_t r0 = sp == 0 ? r : s.rank[0], r1 = sp == 0 ? s.rank[1] : r;
-        c.row = r0;
-        c.dense = r0 * H.dim + r1;
-        c.val = want_val ? double_elem(H, d, n, q, x, y) * s.temp : 0.0;
-        return;
-    }
-    e -= 2 * H.n_daa;
-    {
-        uint64_t ia = e / H.n_sa, jb = e - ia * H.n_sa;
-        int ii = (int)(ia / v), aa = (int)(ia - (uint64_t)ii * v);
-        int jj = (int)(jb / v), bb = (int)(jb - (uint64_t)jj * v);
-        int i = s.occ[0][ii], a = s.vir[0][aa], j = s.occ[1][jj], b = s.vir[1][bb];
-        int ni = 2 * i, xa = 2 * a, nj = 2 * j + 1, yb = 2 * b + 1;
-        c.target = d - (one << ni) - (one << nj) + (one << xa) + (one << yb);
-        uint64_t r0 = colex_rank(s.str[0] ^ (1u << i) ^ (1u << a), H.binom);
-        uint64_t r1 = colex_rank(s.str[1] ^ (1u << j) ^ (1u << b), H.binom);
-        c.row = r0;
-        c.dense = r0 * H.dim + r1;
-        if (want_val) {
-            int n = ni < nj ? ni : nj, q = ni < nj ? nj : ni;
-            int x = xa < yb ? xa : yb, y = xa < yb ? yb : xa;
-            c.val = double_elem(H, d, n, q, x, y) * s.temp;
-        } else {
-            c.val = 0.0;
+// Incremental quotient/remainder of i / d for i = start, start + step, ...
+struct DivIter {
+    uint32_t q, r, dq, dr, d;
+    __device__ __forceinline__ DivIter(uint32_t start, uint32_t step, uint32_t div)
+        : q(start / div), r(start % div), dq(step / div), dr(step % div), d(div) {}
+    __device__ __forceinline__ void next() {
+        q += dq;
+        r += dr;
+        if (r >= d) {
+            r -= d;
+            ++q;
         }
     }
-}
+};
 
 // ---------------------------------------------------------------------------------
 // Per-source count of connections whose target row lies in [row_lo, row_hi).
@@ -270,55 +253,145 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev H, const ui
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
-// Candidate generation for the rows [row_lo, row_hi): one workgroup per source.
-// off == nullptr means every connection is in range and source j starts at j*n_conn.
+// Candidate generation for the target rows [row_lo, row_hi): one workgroup per source.
+//   phase A  the source's own row: diagonal, spin-1 singles, spin-1 doubles
+//   phase B  spin-0 singles whose row is in range, each followed by its opposite-spin
+//            doubles (same row; ranks of both strings come from the LDS tables)
+//   phase C  spin-0 doubles, one row each (rank computed, then range-checked)
+// Records are written at `pos`, which starts at j * n_conn (single pass, off == nullptr)
+// or off[j] (per-source exclusive scan of k_count), so the layout is deterministic.
 __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_t* __restrict__ sk,
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
                                                          double coeff, uint64_t row_lo, uint64_t row_hi,
-                                                         const uint64_t* __restrict__ off, uint64_t* __restrict__ ok,
+                                                         const uint64_t* __restrict__ off,
+                                                         const uint16_t* __restrict__ sdec_g, uint64_t* __restrict__ ok,
                                                          double* __restrict__ ov, uint32_t* flags) {
     __shared__ SrcLds s;
+    __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t one = 1;
     const uint64_t base_dense = row_lo * H.dim;
-    const bool all_in = (off == nullptr);
+    const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair, v = (uint32_t)H.v;
+    for (uint32_t x = tid; x < nsa; x += SFCI_BLOCK) sdec[x] = sdec_g[x];
+    __syncthreads();
     bool range_bad = false;
     for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
-        if (tid == 0) load_source(H, s, sk[j], coeff * sv[j], lambda, flags);
-        __syncthreads();
-        uint64_t pos = all_in ? (uint64_t)j * H.n_conn : off[j];
-        for (uint64_t e0 = 0; e0 < H.n_conn; e0 += SFCI_BLOCK) {
-            uint64_t e = e0 + tid;
-            Cand c;
-            bool have = false;
-            if (e < H.n_conn) {
-                connection(H, s, e, true, c);
-                have = all_in || (c.row >= row_lo && c.row < row_hi);
-                if (have && !fx_in_range(c.val)) range_bad = true;
-            }
-            if (all_in) {
-                if (have) {
-                    ok[pos + e] = ((c.dense - base_dense) << 1) | c.tag;
-                    ov[pos + e] = c.val;
+        load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
+        const uint64_t d = s.key;
+        const double temp = s.temp;
+        uint64_t pos = off ? off[j] : (uint64_t)j * H.n_conn;
+        // ---- phase A
+        if (s.row_in) {
+            const uint32_t nA = 1 + nsa + ndaa;
+            const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
+            DivIter it(tid >= 1 + nsa ? tid - 1 - nsa : 0, SFCI_BLOCK, nvp);
+            for (uint32_t f = tid; f < nA; f += SFCI_BLOCK) {
+                uint64_t key;
+                double val;
+                if (f == 0) {
+                    key = (rowbase + s.rank[1]) << 1 | 1ull;
+                    val = s.diag_val;
+                } else if (f <= nsa) {
+                    uint16_t dd = sdec[f - 1];
+                    int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
+                    key = (rowbase + s.srank[1][f - 1]) << 1;
+                    val = single_elem(H, s, 2 * jo + 1, 2 * b + 1) * temp;
+                } else {
+                    uint32_t idx = f - 1 - nsa;
+                    if (it.q * nvp + it.r != idx) it = DivIter(idx, SFCI_BLOCK, nvp);
+                    uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+                    int i = s.occ[1][op & 0xff], jo = s.occ[1][op >> 8];
+                    int a = s.vir[1][vp & 0xff], b = s.vir[1][vp >> 8];
+                    uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                    key = (rowbase + colex_rank(st, H.binom)) << 1;
+                    val = double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1) * temp;
+                    it.next();
                 }
-                continue;
+                range_bad |= !fx_in_range(val);
+                ok[pos + f] = key;
+                ov[pos + f] = val;
             }
-            uint64_t bal = __ballot(have);
-            const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            uint32_t my = (uint32_t)popc64(bal & lt);
-            if (lane == 0) s_wcnt[w] = (uint32_t)popc64(bal);
-            __syncthreads();
-            uint32_t pre = 0, tot = 0;
-            for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) {
-                if (i < w) pre += s_wcnt[i];
-                tot += s_wcnt[i];
+            pos += nA;
+        }
+        // ---- phase B
+        {
+            const uint32_t per = 1 + nsa;
+            const uint32_t nB = (uint32_t)s.n_alist * per;
+            DivIter it(tid, SFCI_BLOCK, per);
+            for (uint32_t f = tid; f < nB; f += SFCI_BLOCK, it.next()) {
+                uint32_t sa = s.alist[it.q];
+                uint16_t da = sdec[sa];
+                int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
+                const uint64_t rowbase = s.srank[0][sa] * H.dim - base_dense;
+                uint64_t key;
+                double val;
+                if (it.r == 0) {
+                    key = (rowbase + s.rank[1]) << 1;
+                    val = single_elem(H, s, 2 * i, 2 * a) * temp;
+                } else {
+                    uint32_t sb = it.r - 1;
+                    uint16_t db = sdec[sb];
+                    int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
+                    key = (rowbase + s.srank[1][sb]) << 1;
+                    int ni = 2 * i, xa = 2 * a, nj = 2 * jo + 1, yb = 2 * b + 1;
+                    int n = ni < nj ? ni : nj, q = ni < nj ? nj : ni;
+                    int x = xa < yb ? xa : yb, y = xa < yb ? yb : xa;
+                    val = double_elem(H, d, n, q, x, y) * temp;
+                }
+                range_bad |= !fx_in_range(val);
+                ok[pos + f] = key;
+                ov[pos + f] = val;
             }
-            if (have) {
-                ok[pos + pre + my] = ((c.dense - base_dense) << 1) | c.tag;
-                ov[pos + pre + my] = c.val;
+            pos += nB;
+        }
+        // ---- phase C
+        {
+            const bool all_in = (off == nullptr);
+            DivIter it(tid, SFCI_BLOCK, nvp);
+            for (uint32_t f0 = 0; f0 < ndaa; f0 += SFCI_BLOCK) {
+                uint32_t f = f0 + tid;
+                bool have = false;
+                uint64_t key = 0;
+                double val = 0.0;
+                if (f < ndaa) {
+                    uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+                    int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
+                    int a = s.vir[0][vp & 0xff], b = s.vir[0][vp >> 8];
+                    uint32_t st = s.str[0] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                    uint64_t r0 = colex_rank(st, H.binom);
+                    have = all_in || (r0 >= row_lo && r0 < row_hi);
+                    if (have) {
+                        key = (r0 * H.dim - base_dense + s.rank[1]) << 1;
+                        val = double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b) * temp;
+                        range_bad |= !fx_in_range(val);
+                    }
+                }
+                it.next();
+                if (all_in) {
+                    if (have) {
+                        ok[pos + f] = key;
+                        ov[pos + f] = val;
+                    }
+                    continue;
+                }
+                uint64_t bal = __ballot(have);
+                uint32_t my = (uint32_t)popc64(bal & lt);
+                if (lane == 0) s_wcnt[w] = (uint32_t)popc64(bal);
+                __syncthreads();
+                uint32_t pre = 0, tot = 0;
+                for (int i2 = 0; i2 < SFCI_BLOCK / SFCI_WAVE; ++i2) {
+                    if (i2 < w) pre += s_wcnt[i2];
+                    tot += s_wcnt[i2];
+                }
+                if (have) {
+                    ok[pos + pre + my] = key;
+                    ov[pos + pre + my] = val;
+                }
+                pos += tot;
+                __syncthreads();
             }
-            pos += tot;
-            __syncthreads();
         }
         __syncthreads();
     }
@@ -655,7 +728,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             Timer t(c->stream);
             int tg = kt_begin(c, K_GENERATE, 16.0 * n_pass + 16.0 * ns);
             k_generate<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo, row_hi,
-                                                                     off, k0, v0, c->d_flags);
+                                                                     off, c->d_sdec, k0, v0, c->d_flags);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
             st.ms_generate += t.stop();

@@ -114,6 +114,7 @@ struct Ctx {
     uint64_t* d_binom = nullptr;
     uint16_t* d_occ_pairs = nullptr;
     uint16_t* d_vir_pairs = nullptr;
+    uint16_t* d_sdec = nullptr;      // single excitation x -> (occupied position | virtual position << 8)
     std::vector<uint64_t> h_binom;   // host copy
     double cut = 1e-5;
     size_t ws_limit = 0;             // bytes for the candidate buffers of one pass
