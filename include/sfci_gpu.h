@@ -58,6 +58,8 @@ typedef struct {
     uint64_t n_out;         /* output length */
     uint64_t n_passes;      /* target-range passes */
     double ms_generate, ms_sort, ms_reduce, ms_select, ms_total;
+    uint64_t n_fallback;    /* passes whose hash-bucket merge overflowed and were re-merged by full sort */
+    int hash_bits;          /* hash bits of the last pass's bucket merge (0: full sort) */
 } sfci_hv_stats_t;
 
 /* Context from a reference _IN integral file: replaces load_integrals/ReadElements
@@ -79,6 +81,10 @@ int sfci_set_cut(sfci_ctx* ctx, double cut);
 /* Device bytes the H.v candidate buffers of one target pass may use. */
 int sfci_set_workspace(sfci_ctx* ctx, uint64_t bytes);
 int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
+/* How H.v groups candidates by target (no reference counterpart; a tuning and test knob):
+ * mode 0 = automatic (hash buckets for large passes), 1 = full radix sort only,
+ * 2 = hash buckets with exactly hash_bits bits (1..24).  Results are identical in every mode. */
+int sfci_set_merge_mode(sfci_ctx* ctx, int mode, int hash_bits);
 int sfci_sync(sfci_ctx* ctx);
 
 /* Per-kernel HIP-event timings accumulated on the context's stream since the last reset

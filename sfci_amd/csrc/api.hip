@@ -271,6 +271,15 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     s->ms_reduce = t.ms_reduce;
     s->ms_select = t.ms_select;
     s->ms_total = t.ms_total;
+    s->n_fallback = t.n_fallback;
+    s->hash_bits = t.hash_bits;
+    return SFCI_OK;
+}
+
+int sfci_set_merge_mode(sfci_ctx* X, int mode, int hash_bits) {
+    if (!X || mode < 0 || mode > 2 || (mode == 2 && (hash_bits < 1 || hash_bits > 24))) return SFCI_E_ARG;
+    X->c.merge_mode = mode;
+    X->c.merge_hash_bits = hash_bits;
     return SFCI_OK;
 }
 

@@ -198,6 +198,13 @@ __host__ __device__ __forceinline__ double sign_between(uint64_t d, int p, int q
     return (popc64(d & mask) & 1) ? -1.0 : 1.0;
 }
 
+// Multiplicative hash of a target index; the top hb bits select a bucket, the bits
+// below them a slot inside the bucket's table.
+__host__ __device__ __forceinline__ uint64_t target_hash_full(uint64_t t) { return t * 0x9E3779B97F4A7C15ull; }
+__host__ __device__ __forceinline__ uint64_t target_hash(uint64_t t, int hb) {
+    return hb ? (target_hash_full(t) >> (64 - hb)) : 0ull;
+}
+
 // colex rank of a k-subset given as a bit string over spatial orbitals:
 // sum_t C(pos_t, t+1), pos_t ascending.  binom is a row-major [65][33] table.
 __host__ __device__ __forceinline__ uint64_t colex_rank(uint32_t s, const uint64_t* binom) {

@@ -603,6 +603,135 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 }
 
 // ---------------------------------------------------------------------------------
+// Hash-bucket merge.  After radix_partition_hash the candidates are grouped by the top
+// hb bits of target_hash(target); a bucket holds about a few thousand distinct targets,
+// which one workgroup merges in an LDS open-addressing table with exact 128-bit
+// accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
+// than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
+constexpr int BR_THREADS = 512;
+constexpr int BR_SLOT_BITS = 12;
+constexpr int BR_SLOTS = 1 << BR_SLOT_BITS;
+constexpr unsigned long long BR_EMPTY = ~0ull;
+
+__global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t* __restrict__ bstart,
+                                uint64_t* __restrict__ bend) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t h = target_hash(ck[i] >> 1, hb);
+        if (i == 0 || target_hash(ck[i - 1] >> 1, hb) != h) bstart[h] = i;
+        if (i + 1 == n || target_hash(ck[i + 1] >> 1, hb) != h) bend[h] = i + 1;
+    }
+}
+
+__global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
+    HamDev H, const uint64_t* __restrict__ ck, const double* __restrict__ cv, const uint64_t* __restrict__ bstart,
+    const uint64_t* __restrict__ bend, uint64_t n_buckets, int hb, uint64_t base_dense,
+    const uint64_t* __restrict__ ik, const double* __restrict__ iv, size_t n_in, double cut,
+    uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
+    unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags) {
+    __shared__ unsigned long long t_key[BR_SLOTS];
+    __shared__ unsigned long long t_lo[BR_SLOTS];
+    __shared__ unsigned long long t_hi[BR_SLOTS];
+    __shared__ uint32_t t_tag[BR_SLOTS / 32];
+    __shared__ int s_full;
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int slot_shift = 64 - hb - BR_SLOT_BITS;
+    fx128 dacc = fx_zero();
+    unsigned long long uniq = 0;
+    bool bad_input = false, overflow = false, any_full = false;
+    for (uint64_t b = blockIdx.x; b < n_buckets; b += gridDim.x) {
+        const uint64_t st = bstart[b], en = bend[b];
+        if (st >= en) continue;
+        for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
+            t_key[i] = BR_EMPTY;
+            t_lo[i] = 0;
+            t_hi[i] = 0;
+        }
+        for (int i = tid; i < BR_SLOTS / 32; i += BR_THREADS) t_tag[i] = 0;
+        if (tid == 0) s_full = 0;
+        __syncthreads();
+        for (uint64_t i = st + tid; i < en; i += BR_THREADS) {
+            uint64_t raw = ck[i];
+            unsigned long long key = raw >> 1;
+            uint32_t slot = (uint32_t)(target_hash_full(key) >> slot_shift) & (BR_SLOTS - 1);
+            fx128 x = fx_from(cv[i]);
+            int probe = 0;
+            for (; probe < BR_SLOTS; ++probe) {
+                unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
+                if (prev == BR_EMPTY || prev == key) break;
+                slot = (slot + 1) & (BR_SLOTS - 1);
+            }
+            if (probe == BR_SLOTS) {
+                s_full = 1;
+                continue;
+            }
+            unsigned long long old = atomicAdd(&t_lo[slot], (unsigned long long)x.lo);
+            unsigned long long carry = (old + x.lo) < old ? 1ull : 0ull;
+            atomicAdd(&t_hi[slot], (unsigned long long)(x.hi + carry));
+            if (raw & 1ull) atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
+        }
+        __syncthreads();
+        const bool full = s_full != 0;
+        __syncthreads();
+        if (full) {
+            any_full = true;
+            continue;
+        }
+        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {
+            const int slot = s0 + tid;
+            bool emit = false;
+            uint64_t det = 0;
+            double y = 0.0;
+            const unsigned long long key = t_key[slot];
+            if (key != BR_EMPTY) {
+                ++uniq;
+                y = fx_to(fx128{t_lo[slot], t_hi[slot]});
+                const bool tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
+                emit = fabs(y) > cut;
+                if (emit || tag) {
+                    uint64_t dense = key + base_dense;
+                    uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+                    det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
+                          (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
+                }
+                if (tag) {
+                    size_t lo = 0, hi = n_in;
+                    while (lo < hi) {
+                        size_t mid = (lo + hi) >> 1;
+                        if (ik[mid] < det) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < n_in && ik[lo] == det) dacc = fx_add(dacc, fx_from(y * iv[lo]));
+                    else bad_input = true;
+                }
+            }
+            uint64_t bal = __ballot(emit);
+            if (bal) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(out_count, (unsigned long long)popc64(bal));
+                base = __shfl(base, 0, SFCI_WAVE);
+                if (emit) {
+                    uint64_t pos = base + (uint64_t)popc64(bal & lt);
+                    if (pos < out_cap) {
+                        sk_out[pos] = det;
+                        sv_out[pos] = y;
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (bad_input) atomicOr(flags, FLAG_BAD_INPUT);
+    if (overflow) atomicOr(flags, FLAG_OUT_OVERFLOW);
+    if (any_full) atomicOr(flags, FLAG_BUCKET_FULL);
+    fx_block_reduce_atomic<BR_THREADS>(dacc, diag_acc);
+    for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) uniq += __shfl_down(uniq, off, SFCI_WAVE);
+    if (lane == 0 && uniq) atomicAdd(unique_count, uniq);
+}
+
+// ---------------------------------------------------------------------------------
 
 static int grid_for(Ctx* c, size_t work, int per_cu = 8) {
     int cus = 256;
@@ -734,35 +863,102 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             st.ms_generate += t.stop();
         }
         st.n_candidates += n_pass;
-        // 3b. sort
-        int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
-        {
+        // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (1-3
+        // radix passes) merged in LDS tables; otherwise, or when a bucket overflows its
+        // table, a full sort on the dense index followed by the segmented reduction.
+        uint64_t *kd = k0, *kt = k1;
+        double *vd = v0, *vt = v1;
+        int hb = 0;
+        if (c->merge_mode == 2) {
+            hb = c->merge_hash_bits;
+        } else if (c->merge_mode == 0 && n_pass >= ((size_t)1 << 20)) {
+            double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.5);
+            while (hb < 24 && est > 0.45 * BR_SLOTS * (double)(1ull << hb)) ++hb;
+        }
+        bool hashed = hb > 0;
+        uint64_t* bstart = nullptr;
+        uint64_t* bend = nullptr;
+        if (hashed) {
             Timer t(c->stream);
-            radix_sort_pairs(c, &k0, &v0, k1, v1, n_pass, bits);
+            radix_partition_hash(c, &kd, &vd, kt, vt, n_pass, hb);
+            if (kd != k0) {
+                kt = k0;
+                vt = v0;
+            }
+            const size_t nb = (size_t)1 << hb;
+            bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
+            bend = bstart + nb;
+            HIPCHK(hipMemsetAsync(bstart, 0, 2 * nb * sizeof(uint64_t), c->stream));
+            int tb = kt_begin(c, K_REDUCE, 8.0 * n_pass);
+            k_bucket_bounds<<<grid_for(c, (n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
+                kd, n_pass, hb, bstart, bend);
+            kt_end(c, tb);
+            st.ms_sort += t.stop();
+        } else {
+            Timer t(c->stream);
+            int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
+            radix_sort_pairs(c, &kd, &vd, kt, vt, n_pass, bits);
+            if (kd != k0) {
+                kt = k0;
+                vt = v0;
+            }
             st.ms_sort += t.stop();
         }
-        // 3c. reduce (the pass's diag goes to accumulator slot 1, folded on the host)
         for (int attempt = 0;; ++attempt) {
             Timer t(c->stream);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
-            int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
-            int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
-            k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, k0, v0, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff, stk,
-                                                         stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
-                                                         c->d_flags);
-            kt_end(c, tr);
+            if (hashed) {
+                const uint64_t nb = (uint64_t)1 << hb;
+                int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
+                k_bucket_reduce<<<256, BR_THREADS, 0, c->stream>>>(H, kd, vd, bstart, bend, nb, hb, row_lo * H.dim, sk,
+                                                                   sv, ns, cut_eff, stk, stv, d_out_count, stage_cap,
+                                                                   d_uniq, c->d_acc + 2, c->d_flags);
+                kt_end(c, tr);
+            } else {
+                int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
+                int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
+                k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, kd, vd, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff,
+                                                             stk, stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
+                                                             c->d_flags);
+                kt_end(c, tr);
+            }
             HIPCHK(hipGetLastError());
             uint64_t cnts[2];
             HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
             st.ms_reduce += t.stop();
+            uint32_t fl_now = 0;
+            if (hashed) {
+                fl_now = flags_read_clear(c);
+                if (fl_now & FLAG_BUCKET_FULL) {
+                    // roll back this pass's output and merge it through the full sort instead
+                    st.n_fallback += 1;
+                    c->unique_ratio = std::min(1.0, c->unique_ratio * 2.0);
+                    uint64_t restore[2] = {before, uniq_host};
+                    HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
+                    Timer ts(c->stream);
+                    int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
+                    uint64_t* k_in = kd;
+                    double* v_in = vd;
+                    radix_sort_pairs(c, &kd, &vd, kt, vt, n_pass, bits);
+                    if (kd != k_in) {
+                        kt = k_in;
+                        vt = v_in;
+                    }
+                    st.ms_sort += ts.stop();
+                    hashed = false;
+                    if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
+                    continue;
+                }
+                if (fl_now & ~(uint32_t)FLAG_OUT_OVERFLOW) atomic_flags_restore(c, fl_now);
+            }
             if (cnts[0] <= stage_cap) {
                 out_count_host = cnts[0];
                 uniq_host = cnts[1];
                 diag_total = fx_add(diag_total, acc_read_raw(c, 1));
                 break;
             }
-            if (attempt > 0) throw HipError("merged-output staging overflow after regrow", -3);
+            if (attempt > 2) throw HipError("merged-output staging overflow after regrow", -3);
             // grow the staging list, keep earlier passes' output, redo this pass
             size_t ncap = cnts[0] + (cnts[0] >> 2);
             DevBuf nk, nv;
@@ -787,6 +983,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             uint32_t fl = flags_read_clear(c);
             if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
         }
+        st.hash_bits = hashed ? hb : 0;
     }
     uint64_t uniq = uniq_host;
     *diag = fx_to(diag_total);
@@ -795,6 +992,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
     if (fl & FLAG_OUT_OVERFLOW) throw HipError("merged-output staging overflow", -3);
     if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
     st.n_unique = uniq;
+    if (st.n_candidates) c->unique_ratio = std::max(1e-6, (double)uniq / (double)st.n_candidates);
     // 4. top out_cap by |y|, then key order
     size_t m = out_count_host;
     size_t kept = 0;

@@ -54,6 +54,7 @@ enum : uint32_t {
     FLAG_RANGE = 1u,        // a term left the fixed-point range (|x| >= 2^42)
     FLAG_OUT_OVERFLOW = 2u, // merged-output staging buffer too small (host grows and retries)
     FLAG_BAD_INPUT = 4u,    // an input determinant was not found in its own (unsorted) vector
+    FLAG_BUCKET_FULL = 8u,  // a hash bucket had more distinct targets than its LDS table holds
 };
 
 // Fixed per-Hamiltonian quantities passed by value to the kernels.
@@ -127,6 +128,10 @@ struct Ctx {
     DevBuf scratch_hist, scratch_off, scratch_small;
     DevBuf scratch_scan[8];
     DevBuf rowhist;
+    DevBuf bucket_bounds;
+    double unique_ratio = 0.25;
+    int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
+    int merge_hash_bits = 0;      // distinct targets per candidate, from the last H.v (sizes the hash buckets)
     unsigned long long* d_acc = nullptr;   // 8 x 2 words of fx128 accumulators
     uint32_t* d_flags = nullptr;
     uint64_t* d_counter = nullptr;         // 8 general counters
@@ -134,7 +139,8 @@ struct Ctx {
     KernelClock kclock;
     // statistics of the last H.v
     struct {
-        uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0;
+        uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0, n_fallback = 0;
+        int hash_bits = 0;
         double ms_generate = 0, ms_sort = 0, ms_reduce = 0, ms_select = 0, ms_total = 0;
     } st;
 };
@@ -151,6 +157,7 @@ struct Vec {
 void exclusive_scan_u32(Ctx* c, const uint32_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
 void exclusive_scan_u64(Ctx* c, const uint64_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
 void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits);
+void radix_partition_hash(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int hb);
 
 // vecops.hip
 // Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.
@@ -182,6 +189,7 @@ void acc_clear(Ctx* c, int slot);
 fx128 acc_read_raw(Ctx* c, int slot);
 double acc_read(Ctx* c, int slot);
 uint32_t flags_read_clear(Ctx* c);
+void atomic_flags_restore(Ctx* c, uint32_t f);   // put back flags read by flags_read_clear
 
 static inline int bits_for(uint64_t max_value_exclusive) {
     // smallest b with 2^b >= max_value_exclusive

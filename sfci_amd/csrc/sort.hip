@@ -101,7 +101,20 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_ROUNDS = 16;
 constexpr int SORT_TILE = SFCI_BLOCK * SORT_ROUNDS;   // 4096
 
-__global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __restrict__ kin, size_t n, int shift,
+// digit of a key: plain bits, or bits of the target hash (key >> 1 hashed, hb bits)
+struct DigitPlain {
+    int shift;
+    __device__ __forceinline__ uint32_t operator()(uint64_t k) const { return (uint32_t)(k >> shift) & (RADIX - 1); }
+};
+struct DigitHash {
+    int shift, hb;
+    __device__ __forceinline__ uint32_t operator()(uint64_t k) const {
+        return (uint32_t)(target_hash(k >> 1, hb) >> shift) & (RADIX - 1);
+    }
+};
+
+template <class Digit>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __restrict__ kin, size_t n, Digit dg,
                                                            uint32_t* __restrict__ hist, size_t n_tiles) {
     __shared__ uint32_t h[RADIX];
     h[threadIdx.x] = 0;
@@ -109,7 +122,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __res
     size_t base = (size_t)blockIdx.x * SORT_TILE;
     for (int r = 0; r < SORT_ROUNDS; ++r) {
         size_t i = base + (size_t)r * SFCI_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[(kin[i] >> shift) & (RADIX - 1)], 1u);
+        if (i < n) atomicAdd(&h[dg(kin[i])], 1u);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
@@ -119,10 +132,11 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __res
 // per round, 8 ballots build each lane's digit-peer mask), placed in LDS in tile-local
 // digit order, then written out so that consecutive threads write consecutive slots of
 // each digit's output run.
+template <class Digit>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                               const double* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, double* __restrict__ vout,
-                                                              size_t n, int shift, const uint64_t* __restrict__ off,
+                                                              size_t n, Digit dg, const uint64_t* __restrict__ off,
                                                               const uint32_t* __restrict__ hist, size_t n_tiles) {
     constexpr int NW = SFCI_BLOCK / SFCI_WAVE;
     __shared__ uint64_t s_key[SORT_TILE];
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __
         bool valid = li < n_here;
         uint64_t k = valid ? kin[base + li] : 0ull;
         double v = valid ? vin[base + li] : 0.0;
-        uint32_t d = (uint32_t)(k >> shift) & (RADIX - 1);
+        uint32_t d = dg(k);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < RADIX_BITS; ++b) {
@@ -187,17 +201,16 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __
     }
     for (int p = tid; p < n_here; p += SFCI_BLOCK) {
         uint64_t k = s_key[p];
-        uint32_t d = (uint32_t)(k >> shift) & (RADIX - 1);
+        uint32_t d = dg(k);
         uint64_t g = s_gbase[d] + (uint64_t)(p - s_lstart[d]);
         kout[g] = k;
         vout[g] = s_val[p];
     }
 }
 
-// Sort n pairs by the low `bits` bits of the key (stable).  (*keys, *vals) hold the
-// input; on return they point at the sorted data, which is either the input buffers
-// or (ktmp, vtmp).
-void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits) {
+template <class MakeDigit>
+static void lsd_passes(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits,
+                       MakeDigit make) {
     if (n <= 1 || bits <= 0) return;
     size_t n_tiles = (n + SORT_TILE - 1) / SORT_TILE;
     uint32_t* hist = c->scratch_hist.get<uint32_t>((size_t)RADIX * n_tiles);
@@ -207,20 +220,35 @@ void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, do
     uint64_t* kout = ktmp;
     double* vout = vtmp;
     for (int shift = 0; shift < bits; shift += RADIX_BITS) {
+        auto dg = make(shift);
         int t0 = kt_begin(c, K_RADIX_HIST, 8.0 * n + 4.0 * RADIX * n_tiles);
-        k_radix_hist<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, n, shift, hist, n_tiles);
+        k_radix_hist<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, n, dg, hist, n_tiles);
         kt_end(c, t0);
         int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * n_tiles);
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
         kt_end(c, t1);
         int t2 = kt_begin(c, K_RADIX_SCATTER, 32.0 * n);
-        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, shift, off, hist, n_tiles);
+        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, dg, off, hist, n_tiles);
         kt_end(c, t2);
+        HIPCHK(hipGetLastError());
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     *keys = kin;
     *vals = vin;
+}
+
+// Sort n pairs by the low `bits` bits of the key (stable).  (*keys, *vals) hold the
+// input; on return they point at the sorted data, which is either the input buffers
+// or (ktmp, vtmp).
+void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits) {
+    lsd_passes(c, keys, vals, ktmp, vtmp, n, bits, [](int shift) { return DigitPlain{shift}; });
+}
+
+// Group n candidate records by the hb-bit hash of their target (key >> 1): stable LSD
+// passes over the hash digits.  Same buffer convention as radix_sort_pairs.
+void radix_partition_hash(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int hb) {
+    lsd_passes(c, keys, vals, ktmp, vtmp, n, hb, [hb](int shift) { return DigitHash{shift, hb}; });
 }
 
 }  // namespace sfci

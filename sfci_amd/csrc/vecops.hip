@@ -88,6 +88,16 @@ uint32_t flags_read_clear(Ctx* c) {
     return f;
 }
 
+void atomic_flags_restore(Ctx* c, uint32_t f) {
+    if (!f) return;
+    uint32_t cur = 0;
+    HIPCHK(hipMemcpyAsync(&cur, c->d_flags, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    cur |= f;
+    HIPCHK(hipMemcpyAsync(c->d_flags, &cur, 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 // ---- elementwise -------------------------------------------------------------------
 __global__ void k_copy(uint64_t* __restrict__ dk, double* __restrict__ dv, const uint64_t* __restrict__ sk,
                        const double* __restrict__ sv, size_t n) {

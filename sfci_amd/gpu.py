@@ -37,7 +37,7 @@ class StatsT(C.Structure):
     _fields_ = [("n_src", C.c_uint64), ("n_candidates", C.c_uint64), ("n_unique", C.c_uint64),
                 ("n_out", C.c_uint64), ("n_passes", C.c_uint64), ("ms_generate", C.c_double),
                 ("ms_sort", C.c_double), ("ms_reduce", C.c_double), ("ms_select", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("n_fallback", C.c_uint64), ("hash_bits", C.c_int)]
 
 
 class KernelStatT(C.Structure):
@@ -90,6 +90,7 @@ def lib():
         "sfci_set_cut": (ip, [vp, dbl]),
         "sfci_set_workspace": (ip, [vp, u64]),
         "sfci_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
+        "sfci_set_merge_mode": (ip, [vp, ip, ip]),
         "sfci_sync": (ip, [vp]),
         "sfci_kernel_stats": (ip, [vp, C.POINTER(KernelStatT), ip, ip]),
         "sfci_kernel_name": (C.c_char_p, [ip]),
@@ -203,6 +204,9 @@ class Context:
 
     def set_workspace(self, nbytes: int):
         self._check(lib().sfci_set_workspace(self._h, int(nbytes)))
+
+    def set_merge_mode(self, mode: int, hash_bits: int = 0):
+        self._check(lib().sfci_set_merge_mode(self._h, mode, hash_bits))
 
     def stats(self) -> dict:
         s = StatsT()

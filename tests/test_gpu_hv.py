@@ -130,6 +130,29 @@ def test_hv_multipass_matches(sys1, orc1, ctx1):
     assert bits([gd])[0] == bits([od])[0]
 
 
+@pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14)])
+def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
+    # full sort, hash buckets, and hash buckets too few to hold the targets (each
+    # overflowing bucket set falls back to the full sort) give the same bits
+    k, v = random_input(sys1, 3000, 31)
+    ctx1.set_merge_mode(mode, hbits)
+    try:
+        gk, gv, gd = gpu_hv(ctx1, k, v, len(k), 0.05, 1.0, 60_000_000)
+        st = ctx1.stats()
+    finally:
+        ctx1.set_merge_mode(0, 0)
+    log("merge mode %d bits %d: fallback %d hash_bits %d" % (mode, hbits, st["n_fallback"], st["hash_bits"]))
+    if mode == 2 and hbits == 1:
+        assert st["n_fallback"] >= 1
+    ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=0.05, coeff=1.0, cut=1e-5)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits_of(gd) == bits_of(od)
+
+
+def bits_of(x):
+    return bits([x])[0]
+
+
 def test_hv_empty(ctx1):
     vin, vout = ctx1.vector(10), ctx1.vector(10)
     assert ctx1.hv(vin, vout, 10) == 0.0
