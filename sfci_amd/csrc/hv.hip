@@ -148,6 +148,38 @@ __device__ __forceinline__ double double_elem(const HamDev& H, uint64_t d, int n
     return s1 * g_so(H, x, n, y, q) + s2 * g_so(H, y, n, x, q);
 }
 
+// Opposite-spin double: spin-0 i -> a and spin-1 j -> b (spatial indices).  Same value
+// as double_elem(d, n, q, x, y) with n < q, x < y the sorted spin orbitals: exactly one
+// of its two terms has a spin-allowed integral, (a i|b j) from the block whose first
+// pair is the lower annihilated orbital's spin.
+__device__ __forceinline__ double opposite_double(const HamDev& H, uint64_t d, int i, int a, int j, int b) {
+    const uint64_t one = 1;
+    const int ni = 2 * i, xa = 2 * a, nj = 2 * j + 1, yb = 2 * b + 1;
+    const bool n_is_0 = ni < nj;                  // n = ni, q = nj; else n = nj, q = ni
+    const int n = n_is_0 ? ni : nj, q = n_is_0 ? nj : ni;
+    const bool x_is_0 = xa < yb;
+    const int x = x_is_0 ? xa : yb, y = x_is_0 ? yb : xa;
+    // the nonzero term is s1*G(x,n,y,q) when x and n share a spin, else s2*G(y,n,x,q)
+    const bool first = (n_is_0 == x_is_0);
+    double sg;
+    if (first) {
+        uint64_t d1 = d - (one << q) + (one << y);
+        sg = sign_between(d, q, y) * sign_between(d1, n, x);
+    } else {
+        uint64_t d2 = d - (one << q) + (one << x);
+        sg = sign_between(d, q, x) * sign_between(d2, n, y);
+    }
+    const uint32_t N = (uint32_t)H.n_orb, N2 = N * N, N4 = N2 * N2;
+    double g;
+    if (n_is_0)   // block (0,1): (a i | b j)
+        g = H.g[1u * N4 + ((uint32_t)a * N + (uint32_t)i) * N2 + (uint32_t)b * N + (uint32_t)j];
+    else          // block (1,0): (b j | a i)
+        g = H.g[2u * N4 + ((uint32_t)b * N + (uint32_t)j) * N2 + (uint32_t)a * N + (uint32_t)i];
+    // the other term is an exact zero; adding it can only change the sign of a zero
+    // value, which contributes nothing to any sum
+    return sg * g;
+}
+
 // Incremental quotient/remainder of i / d for i = start, start + step, ...
 struct DivIter {
     uint32_t q, r, dq, dr, d;
@@ -335,10 +367,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_
                     uint16_t db = sdec[sb];
                     int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
                     key = (rowbase + s.srank[1][sb]) << 1;
-                    int ni = 2 * i, xa = 2 * a, nj = 2 * jo + 1, yb = 2 * b + 1;
-                    int n = ni < nj ? ni : nj, q = ni < nj ? nj : ni;
-                    int x = xa < yb ? xa : yb, y = xa < yb ? yb : xa;
-                    val = double_elem(H, d, n, q, x, y) * temp;
+                    val = opposite_double(H, d, i, a, jo, b) * temp;
                 }
                 range_bad |= !fx_in_range(val);
                 ok[pos + f] = key;
@@ -609,8 +638,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
 // than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
 constexpr int BR_THREADS = 512;
-constexpr int BR_SLOT_BITS = 12;
-constexpr int BR_SLOTS = 1 << BR_SLOT_BITS;
+constexpr int BR_SLOTS = 6144;                 // 6144 x 24 B = 144 KB of LDS per workgroup
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 __global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t* __restrict__ bstart,
@@ -631,11 +659,11 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     __shared__ unsigned long long t_key[BR_SLOTS];
     __shared__ unsigned long long t_lo[BR_SLOTS];
     __shared__ unsigned long long t_hi[BR_SLOTS];
-    __shared__ uint32_t t_tag[BR_SLOTS / 32];
+    __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];
     __shared__ int s_full;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int slot_shift = 64 - hb - BR_SLOT_BITS;
+    const int slot_shift = 48 - hb;             // 16 hash bits below the bucket bits
     fx128 dacc = fx_zero();
     unsigned long long uniq = 0;
     bool bad_input = false, overflow = false, any_full = false;
@@ -647,19 +675,19 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             t_lo[i] = 0;
             t_hi[i] = 0;
         }
-        for (int i = tid; i < BR_SLOTS / 32; i += BR_THREADS) t_tag[i] = 0;
+        for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) t_tag[i] = 0;
         if (tid == 0) s_full = 0;
         __syncthreads();
         for (uint64_t i = st + tid; i < en; i += BR_THREADS) {
             uint64_t raw = ck[i];
             unsigned long long key = raw >> 1;
-            uint32_t slot = (uint32_t)(target_hash_full(key) >> slot_shift) & (BR_SLOTS - 1);
+            uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
             fx128 x = fx_from(cv[i]);
             int probe = 0;
             for (; probe < BR_SLOTS; ++probe) {
                 unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
                 if (prev == BR_EMPTY || prev == key) break;
-                slot = (slot + 1) & (BR_SLOTS - 1);
+                slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
             }
             if (probe == BR_SLOTS) {
                 s_full = 1;
@@ -677,7 +705,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             any_full = true;
             continue;
         }
-        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {
+        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // BR_SLOTS is a multiple of BR_THREADS
             const int slot = s0 + tid;
             bool emit = false;
             uint64_t det = 0;
@@ -872,8 +900,12 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         if (c->merge_mode == 2) {
             hb = c->merge_hash_bits;
         } else if (c->merge_mode == 0 && n_pass >= ((size_t)1 << 20)) {
-            double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.5);
-            while (hb < 24 && est > 0.45 * BR_SLOTS * (double)(1ull << hb)) ++hb;
+            // distinct targets expected in this pass; aim at a table load below 0.6, and drop
+            // to a whole number of 8-bit radix digits when that keeps the load below 0.8
+            double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.25);
+            while (hb < 24 && est > 0.6 * BR_SLOTS * (double)(1ull << hb)) ++hb;
+            int down = hb & ~7;
+            if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS * (double)(1ull << down)) hb = down;
         }
         bool hashed = hb > 0;
         uint64_t* bstart = nullptr;

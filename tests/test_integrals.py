@@ -80,3 +80,23 @@ def test_in_file_entry_counts(make_system):
                         continue
                     expect += 1
     assert len(entries) == expect
+
+
+def test_real_molecule_c2_hf_energy(tmp_path):
+    """The reference's C2/cc-pVDZ example (symmetry-sparse FCIDUMP, 26 orbitals, 8
+    correlated electrons; example/C2, run by example/sfci.input with
+    init_hf=202125315): the converted _IN gives E_HF equal to the closed form from the
+    FCIDUMP's spatial integrals."""
+    from oracle import oracle as O
+    fc = os.path.join(GOLDEN, "c2-cc-pvdz-hf_fcidump")
+    inp = I.convert_rhf(fc, str(tmp_path / "c2_IN"))
+    o = O.Oracle(inp, 52, 8)
+    hf = 202125315
+    occ = [i // 2 for i in range(52) if (hf >> i) & 1 and i % 2 == 0]
+    assert occ == [0, 6, 9, 13]
+    h, eri, ec, n = I.read_fcidump(fc)
+    e_hf = ec + 2 * sum(h[i, i] for i in occ) + sum(2 * eri[i, i, j, j] - eri[i, j, j, i] for i in occ for j in occ)
+    assert abs(o.diag(hf) - e_hf) < 1e-10
+    # symmetry-sparse: many connections have an exactly zero element
+    k, v = o.connections(hf)
+    assert (v == 0.0).sum() > 0 and len(k) == 1 + 2 * 4 * 22 + 2 * 6 * 231 + (4 * 22) ** 2
