@@ -33,6 +33,25 @@ __device__ __forceinline__ double h_so(const HamDev& H, int m, int n) {
     return H.h[(size_t)(m & 1) * N * N + (size_t)(m >> 1) * N + (n >> 1)];
 }
 
+// LDS copies of the small index tables (binomials for colex ranks, pair lists).  Rank
+// computations gather from these with per-lane indices; from LDS that is a bank-parallel
+// read instead of a scattered L1 request per lane.
+constexpr int LDS_BINOM = SFCI_MAX_ORB * 33;
+constexpr int LDS_PAIRS = SFCI_MAX_ORB * (SFCI_MAX_ORB - 1) / 2;
+
+__device__ __forceinline__ HamDev stage_tables(const HamDev& H, uint64_t* s_binom, uint16_t* s_op, uint16_t* s_vp) {
+    for (int i = threadIdx.x; i < H.n_orb * 33; i += blockDim.x) s_binom[i] = H.binom[i];
+    const int nop = H.k * (H.k - 1) / 2, nvp = H.v * (H.v - 1) / 2;
+    for (int i = threadIdx.x; i < nop; i += blockDim.x) s_op[i] = H.occ_pairs[i];
+    for (int i = threadIdx.x; i < nvp; i += blockDim.x) s_vp[i] = H.vir_pairs[i];
+    __syncthreads();
+    HamDev L = H;
+    L.binom = s_binom;
+    L.occ_pairs = s_op;
+    L.vir_pairs = s_vp;
+    return L;
+}
+
 constexpr int MAX_SINGLES = (SFCI_MAX_ORB / 2) * (SFCI_MAX_ORB / 2);   // k*v <= 16*16 for 32 orbitals
 
 struct SrcLds {
@@ -197,8 +216,11 @@ struct DivIter {
 
 // ---------------------------------------------------------------------------------
 // Per-source count of connections whose target row lies in [row_lo, row_hi).
-__global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev H, const uint64_t* __restrict__ sk, size_t n_src,
+__global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev Hg, const uint64_t* __restrict__ sk, size_t n_src,
                                                       uint64_t row_lo, uint64_t row_hi, uint32_t* __restrict__ cnt) {
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     __shared__ uint32_t s_str;
     __shared__ uint8_t s_occ[SFCI_MAX_K], s_vir[SFCI_MAX_ORB];
     __shared__ uint32_t s_tot;
@@ -246,9 +268,12 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev H, const uint64_t* 
 
 // Histogram of connection counts over target-row groups (row >> grp_shift), for pass
 // planning.  Each thread handles one source at a time.
-__global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev H, const uint64_t* __restrict__ sk,
+__global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const uint64_t* __restrict__ sk,
                                                               size_t n_src, int grp_shift, int n_grp,
                                                               unsigned long long* __restrict__ hist) {
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     extern __shared__ unsigned long long s_h[];
     for (int i = threadIdx.x; i < n_grp; i += SFCI_BLOCK) s_h[i] = 0;
     __syncthreads();
@@ -292,7 +317,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev H, const ui
 //   phase C  spin-0 doubles, one row each (rank computed, then range-checked)
 // Records are written at `pos`, which starts at j * n_conn (single pass, off == nullptr)
 // or off[j] (per-source exclusive scan of k_count), so the layout is deterministic.
-__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_t* __restrict__ sk,
+__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
                                                          double coeff, uint64_t row_lo, uint64_t row_hi,
                                                          const uint64_t* __restrict__ off,
@@ -301,6 +326,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev H, const uint64_
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t one = 1;

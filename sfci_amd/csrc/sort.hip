@@ -128,49 +128,57 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_hist(const uint64_t* __res
     hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one tile.  Items are ranked round by round (256 consecutive items
-// per round, 8 ballots build each lane's digit-peer mask), placed in LDS in tile-local
+// Stable scatter of one tile (SORT_TILE items, SCATTER_THREADS threads).  Items are
+// ranked round by round (one item per thread per round, 8 ballots build each lane's
+// digit-peer mask, wave counts are combined in wave order), placed in LDS in tile-local
 // digit order, then written out so that consecutive threads write consecutive slots of
 // each digit's output run.
+constexpr int SCATTER_THREADS = 512;
+constexpr int SCATTER_WAVES = SCATTER_THREADS / SFCI_WAVE;
+constexpr int SCATTER_ROUNDS = SORT_TILE / SCATTER_THREADS;
+
 template <class Digit>
-__global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __restrict__ kin,
-                                                              const double* __restrict__ vin,
-                                                              uint64_t* __restrict__ kout, double* __restrict__ vout,
-                                                              size_t n, Digit dg, const uint64_t* __restrict__ off,
-                                                              const uint32_t* __restrict__ hist, size_t n_tiles) {
-    constexpr int NW = SFCI_BLOCK / SFCI_WAVE;
+__global__ __launch_bounds__(SCATTER_THREADS) void k_radix_scatter(const uint64_t* __restrict__ kin,
+                                                                   const double* __restrict__ vin,
+                                                                   uint64_t* __restrict__ kout,
+                                                                   double* __restrict__ vout, size_t n, Digit dg,
+                                                                   const uint64_t* __restrict__ off,
+                                                                   const uint32_t* __restrict__ hist, size_t n_tiles) {
     __shared__ uint64_t s_key[SORT_TILE];
     __shared__ double s_val[SORT_TILE];
     __shared__ uint64_t s_gbase[RADIX];     // global start of each digit's run for this tile
     __shared__ uint32_t s_lstart[RADIX];    // tile-local start of each digit
     __shared__ uint32_t s_run[RADIX];       // items of each digit placed so far
-    __shared__ uint32_t s_cnt[NW][RADIX];
+    __shared__ uint32_t s_cnt[SCATTER_WAVES][RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
     const size_t tile = blockIdx.x;
-    {
-        uint32_t h = hist[(size_t)tid * n_tiles + tile];
+    uint32_t h = 0, x = 0;
+    if (tid < RADIX) {
         // exclusive scan of the tile histogram over digits (one digit per thread)
-        uint32_t x = h;
+        h = hist[(size_t)tid * n_tiles + tile];
+        x = h;
         for (int o = 1; o < SFCI_WAVE; o <<= 1) {
             uint32_t y = __shfl_up(x, o, SFCI_WAVE);
             if (lane >= o) x += y;
         }
-        if (lane == SFCI_WAVE - 1) s_cnt[0][w] = x;
-        __syncthreads();
+        if (lane == SFCI_WAVE - 1) s_wsum[w] = x;
+    }
+    for (int i = tid; i < SCATTER_WAVES * RADIX; i += SCATTER_THREADS) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    if (tid < RADIX) {
         uint32_t pre = 0;
-        for (int j = 0; j < w; ++j) pre += s_cnt[0][j];
+        for (int j = 0; j < w; ++j) pre += s_wsum[j];
         s_lstart[tid] = pre + x - h;
         s_gbase[tid] = off[(size_t)tid * n_tiles + tile];
         s_run[tid] = 0;
-        __syncthreads();
-        for (int j = 0; j < NW; ++j) s_cnt[j][tid] = 0;
-        __syncthreads();
     }
+    __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const size_t base = tile * SORT_TILE;
     const int n_here = (int)((n - base) < (size_t)SORT_TILE ? (n - base) : (size_t)SORT_TILE);
-    for (int r = 0; r < SORT_ROUNDS; ++r) {
-        int li = r * SFCI_BLOCK + tid;
+    for (int r = 0; r < SCATTER_ROUNDS; ++r) {
+        int li = r * SCATTER_THREADS + tid;
         bool valid = li < n_here;
         uint64_t k = valid ? kin[base + li] : 0ull;
         double v = valid ? vin[base + li] : 0.0;
@@ -191,15 +199,17 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_radix_scatter(const uint64_t* __
             s_val[pos] = v;
         }
         __syncthreads();
-        uint32_t tot = 0;
-        for (int j = 0; j < NW; ++j) {
-            tot += s_cnt[j][tid];
-            s_cnt[j][tid] = 0;
+        if (tid < RADIX) {
+            uint32_t tot = 0;
+            for (int j = 0; j < SCATTER_WAVES; ++j) {
+                tot += s_cnt[j][tid];
+                s_cnt[j][tid] = 0;
+            }
+            s_run[tid] += tot;
         }
-        s_run[tid] += tot;
         __syncthreads();
     }
-    for (int p = tid; p < n_here; p += SFCI_BLOCK) {
+    for (int p = tid; p < n_here; p += SCATTER_THREADS) {
         uint64_t k = s_key[p];
         uint32_t d = dg(k);
         uint64_t g = s_gbase[d] + (uint64_t)(p - s_lstart[d]);
@@ -228,7 +238,7 @@ static void lsd_passes(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, d
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
         kt_end(c, t1);
         int t2 = kt_begin(c, K_RADIX_SCATTER, 32.0 * n);
-        k_radix_scatter<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, vin, kout, vout, n, dg, off, hist, n_tiles);
+        k_radix_scatter<<<n_tiles, SCATTER_THREADS, 0, c->stream>>>(kin, vin, kout, vout, n, dg, off, hist, n_tiles);
         kt_end(c, t2);
         HIPCHK(hipGetLastError());
         std::swap(kin, kout);
