@@ -77,9 +77,12 @@ def time_to_e(ctx, w, nd, args):
     """The reference's ground-state solve (truncated Lanczos, restarts until the Ritz
     value drops by less than 1e-5 Ha) on the GPU; optionally the CPU oracle's solve."""
     t = time.perf_counter()
-    r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap)
+    # the synthetic workloads have positive energies; starting the stopping test from
+    # +inf (instead of the reference's 0.0) makes the solve run until two restarts agree
+    r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, e_prev_init=float("inf"))
     dt = time.perf_counter() - t
-    res = dict(metric="time-to-E (truncated Lanczos to a Ritz change < 1e-5 Ha)", value=dt, unit="s",
+    res = dict(metric="time-to-E (truncated Lanczos until two restarts' Ritz values differ by < 1e-5 Ha)",
+               value=dt, unit="s",
                higher_is_better=False, dtype="f64", data="synthetic",
                config=dict(workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, Nl=%d, restarts<=%d"
                            % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, w["nl"], w["restarts"])),
@@ -90,7 +93,8 @@ def time_to_e(ctx, w, nd, args):
         orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
         threads = max(1, min(16, os.cpu_count() or 1))
         t = time.perf_counter()
-        o = orc.lanczos(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, nthreads=threads)
+        o = orc.lanczos(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, nthreads=threads,
+                        e_prev_init=float("inf"))
         tc = time.perf_counter() - t
         res["cpu_baseline"] = dict(value=tc, unit="s", cores=threads, kind="port", e_final=o["e_final"],
                                    restarts=o["restarts"], sample="the same solve with the oracle")

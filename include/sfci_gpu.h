@@ -158,6 +158,9 @@ typedef struct {
     int64_t hpsi_cap;    /* capacity of H.v results (HPsiCap, src/main.cpp:194): 60,000,000 */
     double delta_e;      /* convergence threshold on the Ritz value (src/sparse_solver.cpp:269): 1e-5 */
     int verbose;         /* 1: print progress lines like the reference's driver */
+    double e_prev_init;  /* "previous" Ritz value before the first restart.  The reference uses 0.0
+                            (src/sparse_solver.cpp:270), which ends the solve after the first restart
+                            whenever the energy is positive; +INFINITY always compares two restarts. */
 } sfci_lanczos_params_t;
 
 typedef struct {

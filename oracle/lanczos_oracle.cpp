@@ -66,11 +66,11 @@ int oracle_gsaes(int n, const double* K, const double* S, double* evals, double*
 // <v|H|v> after truncation (up to max_restart each); returns the number of restarts run
 // or a negative error.  Final vector written to out_k/out_v (capacity nd).
 int oracle_lanczos(const oracle_ham* H, uint64_t init_det, int64_t nd, int nl, int max_restart,
-                   int64_t hpsi_cap, double cut, int nthreads,
+                   int64_t hpsi_cap, double cut, int nthreads, double e_prev_init,
                    double* ritz, double* e_trunc, double* e_final,
                    uint64_t* out_k, double* out_v, int64_t* out_len) {
     const double deltaE = 1e-5;
-    double previousE = 0.0, nearestE = -1000000.0;
+    double previousE = e_prev_init, nearestE = -1000000.0;
     std::vector<Vec> K(nl, Vec(nd));
     Vec v(nd);
     v.k[0] = init_det; v.v[0] = 1.0; v.len = 1;

@@ -79,8 +79,8 @@ def lib():
         L.oracle_gsaes.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.oracle_lanczos.restype = C.c_int
         L.oracle_lanczos.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_int64,
-                                     C.c_double, C.c_int, _f64p, _f64p, C.POINTER(C.c_double), _u64p, _f64p,
-                                     C.POINTER(C.c_int64)]
+                                     C.c_double, C.c_int, C.c_double, _f64p, _f64p, C.POINTER(C.c_double), _u64p,
+                                     _f64p, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -167,12 +167,12 @@ class Oracle:
         return lib().oracle_energy(self._h, keys, vals, len(keys), nthreads)
 
     def lanczos(self, init_det: int, nd: int, nl: int, restarts: int, hpsi_cap=60_000_000,
-                cut=1e-5, nthreads=0):
+                cut=1e-5, nthreads=0, e_prev_init=0.0):
         ritz, et = np.zeros(restarts), np.zeros(restarts)
         ef = C.c_double()
         ok, ov = np.zeros(nd, np.uint64), np.zeros(nd)
         olen = C.c_int64()
-        r = lib().oracle_lanczos(self._h, init_det, nd, nl, restarts, hpsi_cap, cut, nthreads,
+        r = lib().oracle_lanczos(self._h, init_det, nd, nl, restarts, hpsi_cap, cut, nthreads, e_prev_init,
                                  ritz, et, C.byref(ef), ok, ov, C.byref(olen))
         if r < 0:
             raise RuntimeError("oracle_lanczos failed (%d)" % r)

@@ -55,7 +55,7 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
     CK(sfci_vec_upload(v, &init_det, &one, 1));
 
     const double deltaE = p->delta_e;
-    double previousE = 0.0, nearestE = -1000000.0;
+    double previousE = p->e_prev_init, nearestE = -1000000.0;
     std::vector<double> KM(nl * nl, 0.0), S(nl * nl, 0.0);
     auto hv = [&](const sfci_vec* in, sfci_vec* out, double* alpha) {
         auto t = clk::now();

@@ -46,7 +46,7 @@ class KernelStatT(C.Structure):
 
 class LanczosParamsT(C.Structure):
     _fields_ = [("nd", C.c_int64), ("nl", C.c_int), ("num_restart", C.c_int), ("hpsi_cap", C.c_int64),
-                ("delta_e", C.c_double), ("verbose", C.c_int)]
+                ("delta_e", C.c_double), ("verbose", C.c_int), ("e_prev_init", C.c_double)]
 
 
 class LanczosResultT(C.Structure):
@@ -269,8 +269,9 @@ class Context:
         self._check(lib().sfci_lincomb(self._h, C.cast(arr, C.c_void_p), _ptr(c), len(vecs), out._h))
 
     def ground_state(self, init_det: int, nd: int, nl: int, num_restart: int, hpsi_cap: int = 60_000_000,
-                     delta_e: float = 1e-5, verbose: bool = False, v_out: Optional[Vector] = None) -> dict:
-        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0)
+                     delta_e: float = 1e-5, verbose: bool = False, v_out: Optional[Vector] = None,
+                     e_prev_init: float = 0.0) -> dict:
+        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0, e_prev_init)
         ritz, et = np.zeros(num_restart), np.zeros(num_restart)
         r = LanczosResultT()
         self._check(lib().sfci_ground_state(self._h, init_det, C.byref(p), _ptr(ritz), _ptr(et), C.byref(r),

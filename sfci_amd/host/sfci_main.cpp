@@ -92,7 +92,7 @@ int main(int argc, char** argv) {
             return 1;
         }
         std::cout << "E_hf:" << ehf << std::endl;
-        sfci_lanczos_params_t p{Nd, (int)Nl, (int)num_restart, hpsi_cap, 1e-5, 1};
+        sfci_lanczos_params_t p{Nd, (int)Nl, (int)num_restart, hpsi_cap, 1e-5, 1, 0.0};
         std::vector<double> ritz(num_restart), et(num_restart);
         sfci_lanczos_result_t r;
         rc = sfci_ground_state(ctx, init_hf, &p, ritz.data(), et.data(), &r, nullptr);

@@ -211,9 +211,10 @@ def test_vector_ops_bit_exact(sys1, ctx1):
 def test_lanczos_matches_oracle(sys1, orc1, ctx1):
     nd, nl, nr = 3000, 8, 3
     log("lanczos gpu")
-    g = ctx1.ground_state(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000)
+    g = ctx1.ground_state(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000, e_prev_init=float("inf"))
     log("lanczos oracle")
-    o = orc1.lanczos(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000)
+    o = orc1.lanczos(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000, e_prev_init=float("inf"))
+    assert g["restarts"] >= 2
     assert g["restarts"] == o["restarts"]
     assert np.allclose(g["ritz"], o["ritz"], rtol=0, atol=1e-9)
     assert abs(g["e_final"] - o["e_final"]) < 1e-9
