@@ -76,18 +76,33 @@ def cpu_baseline(w, keys, vals, n_conn, target_s=15.0, max_threads=16):
 def time_to_e(ctx, w, nd, args):
     """The reference's ground-state solve (truncated Lanczos, restarts until the Ritz
     value drops by less than 1e-5 Ha) on the GPU; optionally the CPU oracle's solve."""
+    mixed = args.precision == "mixed"
+    fp64_ref = None
+    if mixed:
+        # the plain FP64 solve first: the mixed solve's energy is judged against it
+        t = time.perf_counter()
+        r64 = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap,
+                               e_prev_init=float("inf"))
+        fp64_ref = dict(seconds=time.perf_counter() - t, e_final=r64["e_final"], restarts=r64["restarts"])
+        log("fp64 solve: %.2f s, E = %.12f" % (fp64_ref["seconds"], r64["e_final"]))
+        ctx.set_precision(True)
     t = time.perf_counter()
     # the synthetic workloads have positive energies; starting the stopping test from
     # +inf (instead of the reference's 0.0) makes the solve run until two restarts agree
     r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, e_prev_init=float("inf"))
     dt = time.perf_counter() - t
+    ctx.set_precision(False)
     res = dict(metric="time-to-E (truncated Lanczos until two restarts' Ritz values differ by < 1e-5 Ha)",
                value=dt, unit="s",
-               higher_is_better=False, dtype="f64", data="synthetic",
+               higher_is_better=False, dtype="f32/f64" if mixed else "f64", data="synthetic",
                config=dict(workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, Nl=%d, restarts<=%d"
                            % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, w["nl"], w["restarts"])),
                restarts=r["restarts"], ritz=list(r["ritz"]), e_final=r["e_final"], n_hv=r["n_hv"],
                ms_hv_total=r["ms_hv"])
+    if fp64_ref:
+        res["fp64_solve"] = fp64_ref
+        res["energy_error_vs_fp64"] = r["e_final"] - fp64_ref["e_final"]
+        res["speedup_vs_fp64"] = fp64_ref["seconds"] / dt
     if args.cpu:
         from oracle import oracle as O
         orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
@@ -116,6 +131,8 @@ def main():
     ap.add_argument("--time-to-e", action="store_true",
                     help="run the whole ground-state solve instead (GPU, and the CPU oracle with --cpu)")
     ap.add_argument("--cpu", action="store_true", help="with --time-to-e: also time the CPU oracle's solve")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="H.v candidate precision: fp64, or mixed (FP32 products, exact sums, FP64 vectors)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: multi-GPU sharding of H.v is described in DESIGN.md and not built",
@@ -136,6 +153,9 @@ def main():
     n_conn = info["n_conn"]
     v = build_input(ctx, w, w["init_hf"], nd, args.cap)
     out = ctx.vector(args.cap)
+    mixed = args.precision == "mixed"
+    ctx.set_precision(mixed)
+    rec = 12.0 if mixed else 16.0        # bytes of one candidate record (key + value)
     for i in range(args.warmup):
         ctx.hv(v, out, nd, 0.0, 1.0)
         log("warmup %d: %s" % (i, ctx.stats()))
@@ -164,26 +184,29 @@ def main():
                     frac=achieved / HBM_PEAK_GBS, traffic=None,
                     avg_launch_ms=d["ms"] / max(1, d["launches"]),
                     bytes_per_launch=d["bytes"] / max(1, d["launches"]))
-    # whole-H.v bytes of an ideal single-pass design: e * (N_in + 2 M_c + M_out), e = 16
+    # whole-H.v bytes of an ideal single-pass design: N_in and M_out at 16 bytes, the
+    # M_c candidate records written once and read once
     m_c = tot_cand / args.steps
     m_out = tot_out / args.steps
-    b_hv = 16.0 * (n_in + 2 * m_c + m_out)
+    b_hv = 16.0 * (n_in + m_out) + 2 * rec * m_c
     # sort + merge time per step (radix passes, their scans, and the exact reduce)
     t_sm = sum(ks[k]["ms"] for k in ("k_radix_hist", "k_radix_scatter", "scan", "k_reduce")) / args.steps
     res = dict(metric=METRIC, value=conn / (ms_step * 1e-3), unit="connections/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=ms_step, higher_is_better=True,
-               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               scaling="weak", vs_baseline=None, dtype="f32/f64" if mixed else "f64", data="synthetic",
                config=dict(workload="H.v, %d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, cap=%d"
                            % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, args.cap),
                            n_in=n_in, n_conn=n_conn, candidates=int(m_c), unique=int(tot_uniq / args.steps),
                            n_out=int(m_out)),
                roofline=roofline,
                determinants_merged_per_s=m_c / (t_sm * 1e-3 + 1e-30),
-               sort_merge_roofline_frac=16.0 * (m_c + m_out) / (t_sm * 1e-3 + 1e-30) / 1e9 / HBM_PEAK_GBS,
+               sort_merge_roofline_frac=(rec * m_c + 16.0 * m_out) / (t_sm * 1e-3 + 1e-30) / 1e9 / HBM_PEAK_GBS,
                hv_ideal_bytes_frac=b_hv / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                kernels={k: dict(ms_per_step=x["ms"] / args.steps, launches=x["launches"],
                                 gbs=(x["bytes"] / (x["ms"] * 1e-3) / 1e9) if x["ms"] > 0 else 0.0)
                         for k, x in ks.items() if x["launches"]})
+    if mixed:
+        res["precision"] = "FP32 candidate products, exact 128-bit fixed-point sums, FP64 vectors"
     if not args.no_cpu:
         keys, vals = v.download()
         res["cpu_baseline"] = cpu_baseline(w, keys, vals, n_conn)

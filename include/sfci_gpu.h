@@ -78,6 +78,15 @@ int sfci_info(const sfci_ctx* ctx, sfci_info_t* info);
 
 /* |y| > cut filter of H.v output; default 1e-5 (SpMSpVFinalResCut, src/main.cpp:196). */
 int sfci_set_cut(sfci_ctx* ctx, double cut);
+/* Candidate precision of H.v (no reference counterpart; the reference is FP64 only,
+ * src/all.h:76).  SFCI_PREC_FP64: every product in double.  SFCI_PREC_MIXED: each
+ * off-diagonal H_ij * coeff * c_j is rounded to float (product of the float-rounded
+ * factors) and moved as 12-byte records; sums stay exact, the diagonal term, the output
+ * vector and all Lanczos recurrences stay FP64. */
+#define SFCI_PREC_FP64 0
+#define SFCI_PREC_MIXED 1
+int sfci_set_precision(sfci_ctx* ctx, int precision);
+
 /* Device bytes the H.v candidate buffers of one target pass may use. */
 int sfci_set_workspace(sfci_ctx* ctx, uint64_t bytes);
 int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);

@@ -24,6 +24,10 @@
 //                         list entry, OffDiagGen_bit order), stable key sort and
 //                         sequential double merge.  Used to check mode 0 against the
 //                         reference's arithmetic within 1e-12 relative.
+//   mode 2 ("mixed")   -- mode 0 with each off-diagonal candidate formed in single
+//                         precision, (float)H_ij * (float)(coeff c_j), then widened;
+//                         the diagonal term stays (E_jj - lambda) * coeff c_j in double.
+//                         The GPU library's mixed-precision H.v must match it bit for bit.
 // Tie rule for every |c| truncation: larger |c| first, equal |c| -> smaller key first.
 // The reference's introsort leaves ties unordered (src/introsort.cpp:571-580).
 #include <algorithm>
@@ -402,12 +406,13 @@ int oracle_hv(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, in
         for (int64_t i = 0; i < ns; ++i) {
             det_t dk = src[i].k;
             double temp = coeff * src[i].v;
-            if (mode == 0) {
+            if (mode != 1) {
                 bool first = true;
                 connections_exact(H, dk, [&](det_t t, double hij) {
                     double val;
                     uint8_t tag = 0;
                     if (first) { val = (hij - lambda) * temp; tag = 1; first = false; }
+                    else if (mode == 2) val = (double)((float)hij * (float)temp);
                     else val = hij * temp;
                     if (!fx_ok(val)) bad = 1;
                     res[tid + (size_t)T * (alpha_rank(H, t) >> bb)].push_back({t, val, tag});
@@ -446,7 +451,7 @@ int oracle_hv(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, in
             size_t j = i;
             bool is_input = false;
             double y;
-            if (mode == 0) {
+            if (mode != 1) {
                 fx_t acc = 0;
                 for (; j < buf.size() && buf[j].k == buf[i].k; ++j) { acc += fx_from(buf[j].v); is_input |= buf[j].tag; }
                 y = fx_to(acc);
@@ -461,14 +466,14 @@ int oracle_hv(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, in
                 auto it = std::lower_bound(src.begin(), src.end(), buf[i].k,
                                            [](const KV& a, det_t k) { return a.k < k; });
                 double c = it->v;
-                if (mode == 0) diag_fx[b] += fx_from(y * c);
+                if (mode != 1) diag_fx[b] += fx_from(y * c);
                 else diag_lit[b] += y * c;
             }
             i = j;
         }
     }
     double diag;
-    if (mode == 0) {
+    if (mode != 1) {
         fx_t acc = 0;
         for (auto v : diag_fx) acc += v;
         diag = fx_to(acc);

@@ -221,7 +221,7 @@ void sfci_destroy(sfci_ctx* X) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     DevBuf* bufs[] = {&c.cand_k[0], &c.cand_k[1], &c.cand_v[0], &c.cand_v[1], &c.src_cnt, &c.src_off,
-                      &c.src_k, &c.src_v, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();
@@ -273,6 +273,12 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     s->ms_total = t.ms_total;
     s->n_fallback = t.n_fallback;
     s->hash_bits = t.hash_bits;
+    return SFCI_OK;
+}
+
+int sfci_set_precision(sfci_ctx* X, int precision) {
+    if (!X || (precision != SFCI_PREC_FP64 && precision != SFCI_PREC_MIXED)) return SFCI_E_ARG;
+    X->c.precision = precision;
     return SFCI_OK;
 }
 
@@ -425,6 +431,13 @@ int sfci_hv(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, int64_t nd, double l
 int sfci_energy(sfci_ctx* X, const sfci_vec* v, double* e) {
     if (!X || !v || !e || v->v.ctx != &X->c) return SFCI_E_ARG;
     Ctx* c = &X->c;
+    // the Rayleigh quotient is always formed from FP64 candidates, whatever the H.v precision
+    struct PrecisionGuard {
+        Ctx* c;
+        int saved;
+        ~PrecisionGuard() { c->precision = saved; }
+    } pg{c, c->precision};
+    c->precision = SFCI_PREC_FP64;
     return guarded(c, [&] {
         double d = 0.0;
         dev_hv(c, v->v.k, v->v.v, (size_t)v->v.len, (size_t)v->v.len, 0.0, 1.0, 0, nullptr, nullptr, &d);

@@ -310,19 +310,38 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
+// Candidate value: the Slater-Condon element times coeff * c_j, in the candidate
+// precision (double, or float products for the mixed-precision path).
+template <typename T>
+__device__ __forceinline__ T cand_val(double hij, double temp);
+template <>
+__device__ __forceinline__ double cand_val<double>(double hij, double temp) {
+    return hij * temp;
+}
+template <>
+__device__ __forceinline__ float cand_val<float>(double hij, double temp) {
+    return (float)hij * (float)temp;
+}
+
 // Candidate generation for the target rows [row_lo, row_hi): one workgroup per source.
-//   phase A  the source's own row: diagonal, spin-1 singles, spin-1 doubles
-//   phase B  spin-0 singles whose row is in range, each followed by its opposite-spin
-//            doubles (same row; ranks of both strings come from the LDS tables)
-//   phase C  spin-0 doubles, one row each (rank computed, then range-checked)
-// Records are written at `pos`, which starts at j * n_conn (single pass, off == nullptr)
-// or off[j] (per-source exclusive scan of k_count), so the layout is deterministic.
+//   A0  the diagonal: a tag-only record (value 0); its FP64 term (E_jj - lambda) * temp
+//       goes to dterm[j] and is added when the target is merged
+//   A1  spin-1 singles, A2  spin-1 doubles          (the source's own row)
+//   B1  spin-0 singles whose row is in range
+//   B2  their opposite-spin doubles (same rows; string ranks from the LDS tables)
+//   C   spin-0 doubles, one row each (rank computed, then range-checked)
+// Each part is its own loop, so a wavefront never mixes the O(n_occ) single-excitation
+// element with the O(1) double-excitation element.  Records are written at `pos`, which
+// starts at j * n_conn (single pass, off == nullptr) or off[j] (per-source exclusive scan
+// of k_count), so the layout is deterministic.
+template <typename T>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
                                                          double coeff, uint64_t row_lo, uint64_t row_hi,
                                                          const uint64_t* __restrict__ off,
                                                          const uint16_t* __restrict__ sdec_g, uint64_t* __restrict__ ok,
-                                                         double* __restrict__ ov, uint32_t* flags) {
+                                                         T* __restrict__ ov, double* __restrict__ dterm,
+                                                         uint32_t* flags) {
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
@@ -331,9 +350,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
     const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t one = 1;
     const uint64_t base_dense = row_lo * H.dim;
-    const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair, v = (uint32_t)H.v;
+    const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair;
     for (uint32_t x = tid; x < nsa; x += SFCI_BLOCK) sdec[x] = sdec_g[x];
     __syncthreads();
     bool range_bad = false;
@@ -342,68 +360,70 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
         const uint64_t d = s.key;
         const double temp = s.temp;
         uint64_t pos = off ? off[j] : (uint64_t)j * H.n_conn;
-        // ---- phase A
         if (s.row_in) {
-            const uint32_t nA = 1 + nsa + ndaa;
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
-            DivIter it(tid >= 1 + nsa ? tid - 1 - nsa : 0, SFCI_BLOCK, nvp);
-            for (uint32_t f = tid; f < nA; f += SFCI_BLOCK) {
-                uint64_t key;
-                double val;
-                if (f == 0) {
-                    key = (rowbase + s.rank[1]) << 1 | 1ull;
-                    val = s.diag_val;
-                } else if (f <= nsa) {
-                    uint16_t dd = sdec[f - 1];
-                    int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
-                    key = (rowbase + s.srank[1][f - 1]) << 1;
-                    val = single_elem(H, s, 2 * jo + 1, 2 * b + 1) * temp;
-                } else {
-                    uint32_t idx = f - 1 - nsa;
-                    if (it.q * nvp + it.r != idx) it = DivIter(idx, SFCI_BLOCK, nvp);
-                    uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
-                    int i = s.occ[1][op & 0xff], jo = s.occ[1][op >> 8];
-                    int a = s.vir[1][vp & 0xff], b = s.vir[1][vp >> 8];
-                    uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
-                    key = (rowbase + colex_rank(st, H.binom)) << 1;
-                    val = double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1) * temp;
-                    it.next();
-                }
-                range_bad |= !fx_in_range(val);
-                ok[pos + f] = key;
+            // A0
+            if (tid == 0) {
+                ok[pos] = (rowbase + s.rank[1]) << 1 | 1ull;
+                ov[pos] = (T)0;
+                dterm[j] = s.diag_val;
+            }
+            pos += 1;
+            // A1
+            for (uint32_t f = tid; f < nsa; f += SFCI_BLOCK) {
+                uint16_t dd = sdec[f];
+                int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
+                T val = cand_val<T>(single_elem(H, s, 2 * jo + 1, 2 * b + 1), temp);
+                range_bad |= !fx_in_range((double)val);
+                ok[pos + f] = (rowbase + s.srank[1][f]) << 1;
                 ov[pos + f] = val;
             }
-            pos += nA;
+            pos += nsa;
+            // A2
+            DivIter it(tid, SFCI_BLOCK, nvp);
+            for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
+                uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+                int i = s.occ[1][op & 0xff], jo = s.occ[1][op >> 8];
+                int a = s.vir[1][vp & 0xff], b = s.vir[1][vp >> 8];
+                uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                T val = cand_val<T>(double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1), temp);
+                range_bad |= !fx_in_range((double)val);
+                ok[pos + f] = (rowbase + colex_rank(st, H.binom)) << 1;
+                ov[pos + f] = val;
+            }
+            pos += ndaa;
         }
-        // ---- phase B
+        // B1
+        const uint32_t na = (uint32_t)s.n_alist;
+        for (uint32_t f = tid; f < na; f += SFCI_BLOCK) {
+            uint32_t sa = s.alist[f];
+            uint16_t da = sdec[sa];
+            int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
+            T val = cand_val<T>(single_elem(H, s, 2 * i, 2 * a), temp);
+            range_bad |= !fx_in_range((double)val);
+            ok[pos + f] = (s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1;
+            ov[pos + f] = val;
+        }
+        pos += na;
+        // B2
         {
-            const uint32_t per = 1 + nsa;
-            const uint32_t nB = (uint32_t)s.n_alist * per;
-            DivIter it(tid, SFCI_BLOCK, per);
+            const uint32_t nB = na * nsa;
+            DivIter it(tid, SFCI_BLOCK, nsa ? nsa : 1);
             for (uint32_t f = tid; f < nB; f += SFCI_BLOCK, it.next()) {
                 uint32_t sa = s.alist[it.q];
                 uint16_t da = sdec[sa];
                 int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
-                const uint64_t rowbase = s.srank[0][sa] * H.dim - base_dense;
-                uint64_t key;
-                double val;
-                if (it.r == 0) {
-                    key = (rowbase + s.rank[1]) << 1;
-                    val = single_elem(H, s, 2 * i, 2 * a) * temp;
-                } else {
-                    uint32_t sb = it.r - 1;
-                    uint16_t db = sdec[sb];
-                    int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
-                    key = (rowbase + s.srank[1][sb]) << 1;
-                    val = opposite_double(H, d, i, a, jo, b) * temp;
-                }
-                range_bad |= !fx_in_range(val);
-                ok[pos + f] = key;
+                uint32_t sb = it.r;
+                uint16_t db = sdec[sb];
+                int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
+                T val = cand_val<T>(opposite_double(H, d, i, a, jo, b), temp);
+                range_bad |= !fx_in_range((double)val);
+                ok[pos + f] = (s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1;
                 ov[pos + f] = val;
             }
             pos += nB;
         }
-        // ---- phase C
+        // C
         {
             const bool all_in = (off == nullptr);
             DivIter it(tid, SFCI_BLOCK, nvp);
@@ -411,7 +431,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 uint32_t f = f0 + tid;
                 bool have = false;
                 uint64_t key = 0;
-                double val = 0.0;
+                T val = (T)0;
                 if (f < ndaa) {
                     uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
                     int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
@@ -421,8 +441,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     have = all_in || (r0 >= row_lo && r0 < row_hi);
                     if (have) {
                         key = (r0 * H.dim - base_dense + s.rank[1]) << 1;
-                        val = double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b) * temp;
-                        range_bad |= !fx_in_range(val);
+                        val = cand_val<T>(double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b), temp);
+                        range_bad |= !fx_in_range((double)val);
                     }
                 }
                 it.next();
@@ -453,6 +473,43 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
         __syncthreads();
     }
     if (range_bad) atomicOr(flags, FLAG_RANGE);
+}
+
+// Finish one merged target: for a target that is an input determinant (tag), add the
+// FP64 diagonal term of its source and accumulate y_t * c_t into diag; round the exact
+// sum; decide the cut.  The determinant key is rebuilt from the dense index only when
+// needed (tagged or kept).
+__device__ __forceinline__ double finish_target(const HamDev& H, uint64_t dense, fx128 acc, uint32_t tag,
+                                                const uint64_t* __restrict__ ik, const double* __restrict__ iv,
+                                                const double* __restrict__ dterm, size_t n_in, double cut, bool& emit,
+                                                uint64_t& det, fx128& dacc, bool& bad_input) {
+    bool have_det = false;
+    size_t lo = 0;
+    bool found = false;
+    if (tag) {
+        uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+        det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
+              (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
+        have_det = true;
+        size_t hi = n_in;
+        while (lo < hi) {
+            size_t mid = (lo + hi) >> 1;
+            if (ik[mid] < det) lo = mid + 1;
+            else hi = mid;
+        }
+        found = lo < n_in && ik[lo] == det;
+        if (found) acc = fx_add(acc, fx_from(dterm[lo]));
+        else bad_input = true;
+    }
+    double y = fx_to(acc);
+    emit = fabs(y) > cut;
+    if (emit && !have_det) {
+        uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
+        det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
+              (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
+    }
+    if (found) dacc = fx_add(dacc, fx_from(y * iv[lo]));
+    return y;
 }
 
 // Exact per-target merge of sorted candidates (segmented reduction).
@@ -495,9 +552,11 @@ __device__ __forceinline__ SegVal seg_shfl_up(const SegVal& x, int off) {
     return r;
 }
 
+template <typename T>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t* __restrict__ ck,
-                                                       const double* __restrict__ cv, size_t n, uint64_t base_dense,
+                                                       const T* __restrict__ cv, size_t n, uint64_t base_dense,
                                                        const uint64_t* __restrict__ ik, const double* __restrict__ iv,
+                                                       const double* __restrict__ dterm,
                                                        size_t n_in, double cut, uint64_t* __restrict__ sk_out,
                                                        double* __restrict__ sv_out, unsigned long long* out_count,
                                                        size_t out_cap, unsigned long long* unique_count,
@@ -554,7 +613,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                 uint64_t key = raw >> 1;
                 bool head = (i == start) || (key != (ck[i - 1] >> 1));
                 bool ends = (i + 1 == end) || ((ck[i + 1] >> 1) != key);
-                fx128 x = fx_from(cv[i]);
+                fx128 x = fx_from((double)cv[i]);
                 uint32_t tg = (uint32_t)(raw & 1ull);
                 if (head) {
                     has_head = true;
@@ -614,25 +673,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                 double y = 0.0;
                 if (r < ne) {
                     ++uniq;
-                    y = fx_to(eval[r]);
-                    emit = fabs(y) > cut;
-                    if (emit || etag[r]) {
-                        uint64_t dense = ekey[r] + base_dense;
-                        uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-                        uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom);
-                        uint32_t s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
-                        det = spread_string(s0) | (spread_string(s1) << 1);
-                    }
-                    if (etag[r]) {
-                        size_t lo = 0, hi = n_in;
-                        while (lo < hi) {
-                            size_t mid = (lo + hi) >> 1;
-                            if (ik[mid] < det) lo = mid + 1;
-                            else hi = mid;
-                        }
-                        if (lo < n_in && ik[lo] == det) dacc = fx_add(dacc, fx_from(y * iv[lo]));
-                        else bad_input = true;
-                    }
+                    y = finish_target(H, ekey[r] + base_dense, eval[r], etag[r], ik, iv, dterm, n_in, cut, emit,
+                                      det, dacc, bad_input);
                 }
                 uint64_t bal = __ballot(emit);
                 if (bal) {
@@ -678,10 +720,12 @@ __global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int h
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
-    HamDev H, const uint64_t* __restrict__ ck, const double* __restrict__ cv, const uint64_t* __restrict__ bstart,
+    HamDev H, const uint64_t* __restrict__ ck, const T* __restrict__ cv, const uint64_t* __restrict__ bstart,
     const uint64_t* __restrict__ bend, uint64_t n_buckets, int hb, uint64_t base_dense,
-    const uint64_t* __restrict__ ik, const double* __restrict__ iv, size_t n_in, double cut,
+    const uint64_t* __restrict__ ik, const double* __restrict__ iv, const double* __restrict__ dterm, size_t n_in,
+    double cut,
     uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
     unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags) {
     __shared__ unsigned long long t_key[BR_SLOTS];
@@ -710,7 +754,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             uint64_t raw = ck[i];
             unsigned long long key = raw >> 1;
             uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
-            fx128 x = fx_from(cv[i]);
+            fx128 x = fx_from((double)cv[i]);
             int probe = 0;
             for (; probe < BR_SLOTS; ++probe) {
                 unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
@@ -741,25 +785,9 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             const unsigned long long key = t_key[slot];
             if (key != BR_EMPTY) {
                 ++uniq;
-                y = fx_to(fx128{t_lo[slot], t_hi[slot]});
-                const bool tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                emit = fabs(y) > cut;
-                if (emit || tag) {
-                    uint64_t dense = key + base_dense;
-                    uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-                    det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
-                          (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
-                }
-                if (tag) {
-                    size_t lo = 0, hi = n_in;
-                    while (lo < hi) {
-                        size_t mid = (lo + hi) >> 1;
-                        if (ik[mid] < det) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    if (lo < n_in && ik[lo] == det) dacc = fx_add(dacc, fx_from(y * iv[lo]));
-                    else bad_input = true;
-                }
+                const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
+                y = finish_target(H, key + base_dense, fx128{t_lo[slot], t_hi[slot]}, tag, ik, iv, dterm, n_in, cut,
+                                  emit, det, dacc, bad_input);
             }
             uint64_t bal = __ballot(emit);
             if (bal) {
@@ -817,8 +845,9 @@ struct Timer {
     }
 };
 
-size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
-              size_t out_cap, uint64_t* ok, double* ov, double* diag) {
+template <typename T>
+static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda,
+                       double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag) {
     const HamDev& H = c->ham;
     auto& st = c->st;
     st = {};
@@ -837,6 +866,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         sv = tv;
     }
     st.n_src = ns;
+    double* dterm = c->dterm.get<double>(ns);
     // 2. plan passes over target rows
     const uint64_t n_rows = H.dim;
     const size_t rec = sizeof(uint64_t) + sizeof(double);
@@ -906,14 +936,15 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         }
         if (n_pass == 0) continue;
         uint64_t* k0 = c->cand_k[0].get<uint64_t>(n_pass);
-        double* v0 = c->cand_v[0].get<double>(n_pass);
+        T* v0 = c->cand_v[0].get<T>(n_pass);
         uint64_t* k1 = c->cand_k[1].get<uint64_t>(n_pass);
-        double* v1 = c->cand_v[1].get<double>(n_pass);
+        T* v1 = c->cand_v[1].get<T>(n_pass);
         {
             Timer t(c->stream);
-            int tg = kt_begin(c, K_GENERATE, 16.0 * n_pass + 16.0 * ns);
-            k_generate<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo, row_hi,
-                                                                     off, c->d_sdec, k0, v0, c->d_flags);
+            int tg = kt_begin(c, K_GENERATE, (8.0 + sizeof(T)) * n_pass + 16.0 * ns);
+            k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
+                                                                        row_hi, off, c->d_sdec, k0, v0, dterm,
+                                                                        c->d_flags);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
             st.ms_generate += t.stop();
@@ -923,7 +954,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         // radix passes) merged in LDS tables; otherwise, or when a bucket overflows its
         // table, a full sort on the dense index followed by the segmented reduction.
         uint64_t *kd = k0, *kt = k1;
-        double *vd = v0, *vt = v1;
+        T *vd = v0, *vt = v1;
         int hb = 0;
         if (c->merge_mode == 2) {
             hb = c->merge_hash_bits;
@@ -940,7 +971,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         uint64_t* bend = nullptr;
         if (hashed) {
             Timer t(c->stream);
-            radix_partition_hash(c, &kd, &vd, kt, vt, n_pass, hb);
+            radix_partition_hash<T>(c, &kd, &vd, kt, vt, n_pass, hb);
             if (kd != k0) {
                 kt = k0;
                 vt = v0;
@@ -957,7 +988,7 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
         } else {
             Timer t(c->stream);
             int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
-            radix_sort_pairs(c, &kd, &vd, kt, vt, n_pass, bits);
+            radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
             if (kd != k0) {
                 kt = k0;
                 vt = v0;
@@ -970,17 +1001,18 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
             acc_clear(c, 1);
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
-                int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
-                k_bucket_reduce<<<256, BR_THREADS, 0, c->stream>>>(H, kd, vd, bstart, bend, nb, hb, row_lo * H.dim, sk,
-                                                                   sv, ns, cut_eff, stk, stv, d_out_count, stage_cap,
-                                                                   d_uniq, c->d_acc + 2, c->d_flags);
+                int tr = kt_begin(c, K_REDUCE, (8.0 + sizeof(T)) * n_pass);
+                k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, kd, vd, bstart, bend, nb, hb,
+                                                                      row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk,
+                                                                      stv, d_out_count, stage_cap, d_uniq,
+                                                                      c->d_acc + 2, c->d_flags);
                 kt_end(c, tr);
             } else {
                 int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
-                int tr = kt_begin(c, K_REDUCE, 16.0 * n_pass);
-                k_reduce<<<grid, SFCI_BLOCK, 0, c->stream>>>(H, kd, vd, n_pass, row_lo * H.dim, sk, sv, ns, cut_eff,
-                                                             stk, stv, d_out_count, stage_cap, d_uniq, c->d_acc + 2,
-                                                             c->d_flags);
+                int tr = kt_begin(c, K_REDUCE, (8.0 + sizeof(T)) * n_pass);
+                k_reduce<T><<<grid, SFCI_BLOCK, 0, c->stream>>>(H, kd, vd, n_pass, row_lo * H.dim, sk, sv, dterm, ns,
+                                                                cut_eff, stk, stv, d_out_count, stage_cap, d_uniq,
+                                                                c->d_acc + 2, c->d_flags);
                 kt_end(c, tr);
             }
             HIPCHK(hipGetLastError());
@@ -999,8 +1031,8 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
                     Timer ts(c->stream);
                     int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
                     uint64_t* k_in = kd;
-                    double* v_in = vd;
-                    radix_sort_pairs(c, &kd, &vd, kt, vt, n_pass, bits);
+                    T* v_in = vd;
+                    radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
                     if (kd != k_in) {
                         kt = k_in;
                         vt = v_in;
@@ -1073,6 +1105,12 @@ size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t 
     kt_resolve(c);
     st.ms_total = total.stop();
     return kept;
+}
+
+size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
+              size_t out_cap, uint64_t* ok, double* ov, double* diag) {
+    if (c->precision == 1) return dev_hv_t<float>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag);
+    return dev_hv_t<double>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag);
 }
 
 }  // namespace sfci

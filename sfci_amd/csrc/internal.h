@@ -121,7 +121,7 @@ struct Ctx {
     size_t ws_limit = 0;             // bytes for the candidate buffers of one pass
     // scratch
     DevBuf cand_k[2], cand_v[2];
-    DevBuf src_cnt, src_off, src_k, src_v;
+    DevBuf src_cnt, src_off, src_k, src_v, dterm;
     DevBuf stage_k, stage_v;         // merged H.v output before the top-cap select
     DevBuf tmp_k[2], tmp_v[2];
     DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
@@ -130,6 +130,7 @@ struct Ctx {
     DevBuf rowhist;
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
+    int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
     int merge_hash_bits = 0;      // distinct targets per candidate, from the last H.v (sizes the hash buckets)
     unsigned long long* d_acc = nullptr;   // 8 x 2 words of fx128 accumulators
@@ -157,7 +158,10 @@ struct Vec {
 void exclusive_scan_u32(Ctx* c, const uint32_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
 void exclusive_scan_u64(Ctx* c, const uint64_t* in, size_t n, uint64_t* out, uint64_t* total_dev);
 void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits);
-void radix_partition_hash(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int hb);
+template <typename T>
+void radix_sort_kv(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int bits);
+template <typename T>
+void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int hb);
 
 // vecops.hip
 // Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.

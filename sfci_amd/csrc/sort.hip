@@ -137,15 +137,15 @@ constexpr int SCATTER_THREADS = 512;
 constexpr int SCATTER_WAVES = SCATTER_THREADS / SFCI_WAVE;
 constexpr int SCATTER_ROUNDS = SORT_TILE / SCATTER_THREADS;
 
-template <class Digit>
+template <class Digit, typename T>
 __global__ __launch_bounds__(SCATTER_THREADS) void k_radix_scatter(const uint64_t* __restrict__ kin,
-                                                                   const double* __restrict__ vin,
+                                                                   const T* __restrict__ vin,
                                                                    uint64_t* __restrict__ kout,
-                                                                   double* __restrict__ vout, size_t n, Digit dg,
+                                                                   T* __restrict__ vout, size_t n, Digit dg,
                                                                    const uint64_t* __restrict__ off,
                                                                    const uint32_t* __restrict__ hist, size_t n_tiles) {
     __shared__ uint64_t s_key[SORT_TILE];
-    __shared__ double s_val[SORT_TILE];
+    __shared__ T s_val[SORT_TILE];
     __shared__ uint64_t s_gbase[RADIX];     // global start of each digit's run for this tile
     __shared__ uint32_t s_lstart[RADIX];    // tile-local start of each digit
     __shared__ uint32_t s_run[RADIX];       // items of each digit placed so far
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(SCATTER_THREADS) void k_radix_scatter(const uint64_
         int li = r * SCATTER_THREADS + tid;
         bool valid = li < n_here;
         uint64_t k = valid ? kin[base + li] : 0ull;
-        double v = valid ? vin[base + li] : 0.0;
+        T v = valid ? vin[base + li] : (T)0;
         uint32_t d = dg(k);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -218,17 +218,17 @@ __global__ __launch_bounds__(SCATTER_THREADS) void k_radix_scatter(const uint64_
     }
 }
 
-template <class MakeDigit>
-static void lsd_passes(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits,
+template <typename T, class MakeDigit>
+static void lsd_passes(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int bits,
                        MakeDigit make) {
     if (n <= 1 || bits <= 0) return;
     size_t n_tiles = (n + SORT_TILE - 1) / SORT_TILE;
     uint32_t* hist = c->scratch_hist.get<uint32_t>((size_t)RADIX * n_tiles);
     uint64_t* off = c->scratch_off.get<uint64_t>((size_t)RADIX * n_tiles + 1);
     uint64_t* kin = *keys;
-    double* vin = *vals;
+    T* vin = *vals;
     uint64_t* kout = ktmp;
-    double* vout = vtmp;
+    T* vout = vtmp;
     for (int shift = 0; shift < bits; shift += RADIX_BITS) {
         auto dg = make(shift);
         int t0 = kt_begin(c, K_RADIX_HIST, 8.0 * n + 4.0 * RADIX * n_tiles);
@@ -237,7 +237,7 @@ static void lsd_passes(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, d
         int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * n_tiles);
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
         kt_end(c, t1);
-        int t2 = kt_begin(c, K_RADIX_SCATTER, 32.0 * n);
+        int t2 = kt_begin(c, K_RADIX_SCATTER, 2.0 * (8.0 + sizeof(T)) * n);
         k_radix_scatter<<<n_tiles, SCATTER_THREADS, 0, c->stream>>>(kin, vin, kout, vout, n, dg, off, hist, n_tiles);
         kt_end(c, t2);
         HIPCHK(hipGetLastError());
@@ -251,14 +251,25 @@ static void lsd_passes(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, d
 // Sort n pairs by the low `bits` bits of the key (stable).  (*keys, *vals) hold the
 // input; on return they point at the sorted data, which is either the input buffers
 // or (ktmp, vtmp).
+template <typename T>
+void radix_sort_kv(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int bits) {
+    lsd_passes<T>(c, keys, vals, ktmp, vtmp, n, bits, [](int shift) { return DigitPlain{shift}; });
+}
+
 void radix_sort_pairs(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int bits) {
-    lsd_passes(c, keys, vals, ktmp, vtmp, n, bits, [](int shift) { return DigitPlain{shift}; });
+    radix_sort_kv<double>(c, keys, vals, ktmp, vtmp, n, bits);
 }
 
 // Group n candidate records by the hb-bit hash of their target (key >> 1): stable LSD
-// passes over the hash digits.  Same buffer convention as radix_sort_pairs.
-void radix_partition_hash(Ctx* c, uint64_t** keys, double** vals, uint64_t* ktmp, double* vtmp, size_t n, int hb) {
-    lsd_passes(c, keys, vals, ktmp, vtmp, n, hb, [hb](int shift) { return DigitHash{shift, hb}; });
+// passes over the hash digits.  Same buffer convention as radix_sort_kv.
+template <typename T>
+void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int hb) {
+    lsd_passes<T>(c, keys, vals, ktmp, vtmp, n, hb, [hb](int shift) { return DigitHash{shift, hb}; });
 }
+
+template void radix_sort_kv<double>(Ctx*, uint64_t**, double**, uint64_t*, double*, size_t, int);
+template void radix_sort_kv<float>(Ctx*, uint64_t**, float**, uint64_t*, float*, size_t, int);
+template void radix_partition_hash<double>(Ctx*, uint64_t**, double**, uint64_t*, double*, size_t, int);
+template void radix_partition_hash<float>(Ctx*, uint64_t**, float**, uint64_t*, float*, size_t, int);
 
 }  // namespace sfci

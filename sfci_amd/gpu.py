@@ -91,6 +91,7 @@ def lib():
         "sfci_set_workspace": (ip, [vp, u64]),
         "sfci_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
         "sfci_set_merge_mode": (ip, [vp, ip, ip]),
+        "sfci_set_precision": (ip, [vp, ip]),
         "sfci_sync": (ip, [vp]),
         "sfci_kernel_stats": (ip, [vp, C.POINTER(KernelStatT), ip, ip]),
         "sfci_kernel_name": (C.c_char_p, [ip]),
@@ -204,6 +205,9 @@ class Context:
 
     def set_workspace(self, nbytes: int):
         self._check(lib().sfci_set_workspace(self._h, int(nbytes)))
+
+    def set_precision(self, mixed: bool):
+        self._check(lib().sfci_set_precision(self._h, 1 if mixed else 0))
 
     def set_merge_mode(self, mode: int, hash_bits: int = 0):
         self._check(lib().sfci_set_merge_mode(self._h, mode, hash_bits))

@@ -219,3 +219,62 @@ def test_lanczos_matches_oracle(sys1, orc1, ctx1):
     assert np.allclose(g["ritz"], o["ritz"], rtol=0, atol=1e-9)
     assert abs(g["e_final"] - o["e_final"]) < 1e-9
     log("ritz gpu %r oracle %r" % (g["ritz"], o["ritz"]))
+
+
+@pytest.mark.parametrize("case", ["hf", "rand_shift", "multipass", "sort_merge", "hash_fallback"])
+def test_hv_mixed_bit_exact(case, sys1, orc1, ctx1):
+    # FP32 candidate products, exact sums: the GPU matches the oracle's mixed mode bit
+    # for bit on every merge path
+    log("mixed hv case %s" % case)
+    lam, coeff = 0.0, 1.0
+    if case == "hf":
+        k, v = np.array([sys1["init_hf"]], np.uint64), np.array([1.0])
+    else:
+        k, v = random_input(sys1, 3000, 41)
+        lam, coeff = 0.21, -0.8
+    ctx1.set_precision(True)
+    if case == "multipass":
+        ctx1.set_workspace(1 << 26)
+    elif case == "sort_merge":
+        ctx1.set_merge_mode(1, 0)
+    elif case == "hash_fallback":
+        ctx1.set_merge_mode(2, 1)
+    try:
+        gk, gv, gd = gpu_hv(ctx1, k, v, len(k), lam, coeff, 60_000_000)
+        st = ctx1.stats()
+    finally:
+        ctx1.set_precision(False)
+        ctx1.set_workspace(8 << 30)
+        ctx1.set_merge_mode(0, 0)
+    if case == "multipass":
+        assert st["n_passes"] > 1
+    if case == "hash_fallback":
+        assert st["n_fallback"] >= 1
+    ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=lam, coeff=coeff, cut=1e-5, mode=2)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits_of(gd) == bits_of(od)
+
+
+def test_energy_is_fp64_under_mixed(sys1, orc1, ctx1):
+    k, v = random_input(sys1, 1500, 12)
+    vec = ctx1.vector(len(k), k, v)
+    ctx1.set_precision(True)
+    try:
+        e = ctx1.energy(vec)
+    finally:
+        ctx1.set_precision(False)
+    assert bits_of(e) == bits_of(orc1.energy(k, v))
+
+
+def test_lanczos_mixed_energy_error(sys1, ctx1):
+    # mixed-precision H.v inside the FP64 Lanczos recurrences: the final FP64 energy
+    # stays within 1e-5 Ha of the FP64 solve
+    nd, nl, nr = 3000, 8, 3
+    g64 = ctx1.ground_state(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000, e_prev_init=float("inf"))
+    ctx1.set_precision(True)
+    try:
+        g32 = ctx1.ground_state(sys1["init_hf"], nd, nl, nr, hpsi_cap=200_000, e_prev_init=float("inf"))
+    finally:
+        ctx1.set_precision(False)
+    log("lanczos e64 %.12f mixed %.12f" % (g64["e_final"], g32["e_final"]))
+    assert abs(g64["e_final"] - g32["e_final"]) < 1e-5

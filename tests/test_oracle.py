@@ -252,3 +252,58 @@ def test_lanczos_full_space_reaches_fci(make_system, n_orb, n_elec, seed):
     r = o.lanczos(s["init_hf"], nd=len(dets), nl=12, restarts=6, cut=0.0)
     assert abs(r["e_final"] - e0) < 1e-9
     assert abs(r["ritz"][-1] - e0) < 1e-9
+
+
+def _dyadic_system(tmp_path, n_orb, n_elec, seed):
+    """A Hamiltonian whose integrals are multiples of 1/64: every Slater-Condon element
+    is then a short dyadic number, exactly representable in single precision."""
+    h, canon, ecore = I.synth_integrals(n_orb, seed)
+    h = np.round(h * 64) / 64
+    canon = [(round(v * 64) / 64, i, j, k, l) for v, i, j, k, l in canon]
+    fc, inp = str(tmp_path / "dy_fcidump"), str(tmp_path / "dy_IN")
+    I.write_fcidump(fc, h, canon, n_elec, round(ecore * 64) / 64)
+    I.convert_rhf(fc, inp)
+    return inp
+
+
+def test_hv_mixed_mode_exact_on_single_precision_data(tmp_path):
+    # When every H_ij and every coeff*c_j is exact in single precision, the mixed
+    # candidates equal the FP64 ones and both modes give the same bits.
+    inp = _dyadic_system(tmp_path, 8, 6, 5)
+    o = O.Oracle(inp, 16, 6)
+    dets = np.array(B.sz0_dets(8, 3), dtype=np.uint64)
+    rng = np.random.default_rng(4)
+    pick = np.sort(rng.choice(len(dets), size=600, replace=False))
+    k = dets[pick]
+    v = rng.integers(-16, 17, size=len(k)) / 16.0
+    a = o.hv(k, v, lam=0.25, coeff=0.5, cut=1e-5, mode=0)
+    b = o.hv(k, v, lam=0.25, coeff=0.5, cut=1e-5, mode=2)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
+    assert a[2] == b[2]
+    # a value that is not float-exact does change the mixed result
+    v2 = v.copy()
+    v2[0] = 0.1
+    c = o.hv(k, v2, lam=0.25, coeff=0.5, cut=0.0, mode=0)
+    d = o.hv(k, v2, lam=0.25, coeff=0.5, cut=0.0, mode=2)
+    assert not np.array_equal(c[1], d[1])
+
+
+def test_hv_mixed_mode_error_bound(make_system):
+    # Each mixed candidate carries at most ~1.5 ulp(float) relative error, so every
+    # output differs from FP64 by at most 2^-22 * sum_j |H_ij coeff c_j|.
+    s = make_system(5, 4, 7)
+    o = orc_of(s)
+    h, eri, ec, _ = I.read_fcidump(s["fcidump_path"])
+    dets, M = B.dense_h(h, eri, ec, 5, 2)
+    rng = np.random.default_rng(9)
+    vals = rng.normal(size=len(dets))
+    keys = np.array(dets, dtype=np.uint64)
+    a = o.hv(keys, vals, lam=0.3, coeff=-0.7, cut=0.0, mode=0)
+    b = o.hv(keys, vals, lam=0.3, coeff=-0.7, cut=0.0, mode=2)
+    bound = 2.0 ** -22 * (np.abs(M - np.diag(np.diag(M))) @ np.abs(0.7 * vals))
+    ya = dict(zip(a[0].tolist(), a[1].tolist()))
+    yb = dict(zip(b[0].tolist(), b[1].tolist()))
+    diff = [abs(ya.get(d, 0.0) - yb.get(d, 0.0)) - bound[i] for i, d in enumerate(dets)]
+    assert max(diff) <= 1e-300
+    assert any(ya[d] != yb[d] for d in ya)
