@@ -190,7 +190,8 @@ def main():
     m_out = tot_out / args.steps
     b_hv = 16.0 * (n_in + m_out) + 2 * rec * m_c
     # sort + merge time per step (radix passes, their scans, and the exact reduce)
-    t_sm = sum(ks[k]["ms"] for k in ("k_radix_hist", "k_radix_scatter", "scan", "k_reduce")) / args.steps
+    t_sm = sum(x["ms"] for k, x in ks.items()
+               if k.startswith(("k_radix_", "scan", "k_bucket_", "k_reduce"))) / args.steps
     res = dict(metric=METRIC, value=conn / (ms_step * 1e-3), unit="connections/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=ms_step, higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32/f64" if mixed else "f64", data="synthetic",

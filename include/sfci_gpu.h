@@ -100,7 +100,7 @@ int sfci_sync(sfci_ctx* ctx);
  * (no reference counterpart: the reference only prints wall-clock timers,
  * src/functions.cpp:800-801).  Kernel ids 0..SFCI_KERNEL_COUNT-1 are named by
  * sfci_kernel_name(); bytes are the algorithmic bytes each launch reads and writes. */
-#define SFCI_KERNEL_COUNT 8
+#define SFCI_KERNEL_COUNT 12
 typedef struct {
     double ms;
     uint64_t launches;
@@ -185,6 +185,73 @@ typedef struct {
  * after truncation.  v_out (optional, capacity >= nd) receives the final vector. */
 int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_lanczos_params_t* params, double* ritz,
                       double* e_trunc, sfci_lanczos_result_t* result, sfci_vec* v_out);
+
+/* ---- linear-solver products (the other callers of the H.v boundary) ------------------ */
+
+/* dst = top_{dst.cap}({|c| > cut} of src), key order: the cut-then-cap tail of
+ * spmspvm_bucket_linear (src/functions.cpp:1016-1062). */
+int sfci_shrink_cut(sfci_ctx* ctx, sfci_vec* dst, const sfci_vec* src, double cut);
+
+/* spmspvm_bucket_linear (src/functions.cpp:858-1094).  w = (H - lambda) in, every distinct
+ * target kept (no cut, no cap); hij[j] = <basis[j]|w> for j < n_basis (exact);
+ * w -= hij[n-1] basis[n-1]; w -= hij[n-2] basis[n-2] (when n_basis > 1, addTwo keeping
+ * zeros); out = top_{out.cap}({|w| > cut}), key order.  basis[n_basis-1] is normally `in`.
+ * The reference leaves Hij untouched for an empty input; here hij is zeroed. */
+int sfci_hv_linear(sfci_ctx* ctx, const sfci_vec* in, sfci_vec* out, double lambda, const sfci_vec* const* basis,
+                   int n_basis, double* hij);
+
+/* spmspvm_bucket_res_norm (src/functions.cpp:1096-1259).  w = (H - lambda) x - b, uncut;
+ * *res_norm = ||w|| (exact sum of squares); out = top_{out.cap}({|w| > cut}), key order. */
+int sfci_hv_residual(sfci_ctx* ctx, const sfci_vec* x, sfci_vec* out, double lambda, const sfci_vec* b,
+                     double* res_norm);
+
+/* dynVec::addThree (src/all.h:482-608): out = top_{out.cap}(a x + b y + c z), then
+ * entries with |c| <= 1e-14 dropped, key order.  out may alias an input. */
+int sfci_add_three(sfci_ctx* ctx, sfci_vec* out, const sfci_vec* x, const sfci_vec* y, const sfci_vec* z, double a,
+                   double b, double c);
+
+typedef struct {
+    int64_t nd;          /* capacity of every Krylov vector, of w, r0 and x_m (Nd) */
+    int num_iter;        /* Krylov dimension m: m + 1 products per restart (numIteration) */
+    int num_restart;     /* restarts (numRestart) */
+    int verbose;         /* 1: print the reference's per-restart lines */
+} sfci_linear_params_t;
+
+/* sparse_krylov_solve_linear (src/sparse_solver.cpp:561-731): solves (H - target_e) x = b
+ * from x0 = 0 with restarted Krylov cycles: K_0 = r0/|r0|; for i = 0..m,
+ * w = sfci_hv_linear(K_i, K_0..K_i), Hess(j,i) = Hess(i,j) = hij[j], S(i,j) = <K_j|K_i>,
+ * K_{i+1} = w/|w|; y = lstsq(S Hess[:, :m], |r0| S[:, 0]); x_m = top_Nd(x_m + sum_j y_j K_j);
+ * r0 = -top_Nd(cut((H - target_e) x_m - b)).  residual[r] receives the residual norm after
+ * restart r (capacity num_restart); x_out (capacity >= nd) the solution. */
+int sfci_linear_solve(sfci_ctx* ctx, const sfci_vec* b, double target_e, const sfci_linear_params_t* params,
+                      sfci_vec* x_out, double* residual);
+
+/* sparse_solve_SI (src/sparse_solver.cpp:733-772): one shift-invert step,
+ * v_out = x/|x| for x = sfci_linear_solve(b, target_e), *energy = <v_out|H|v_out>. */
+int sfci_shift_invert(sfci_ctx* ctx, const sfci_vec* b, double target_e, const sfci_linear_params_t* params,
+                      sfci_vec* v_out, double* energy, double* residual);
+
+/* minres_sp (src/minres.cpp:73-225): MINRES on (H - lambda) x = rhs from `guess`, every
+ * vector capped at nd by the reference's products (sfci_hv with coeff -1 / +1, addTwo,
+ * addThree).  Stops after `iters` steps or when the running residual estimate drops below
+ * `tol` (compared with the squared estimate, as the reference does).  *iters_done and
+ * *res2 (optional) receive the step count and the final squared estimate. */
+int sfci_minres(sfci_ctx* ctx, const sfci_vec* rhs, const sfci_vec* guess, sfci_vec* result, int64_t nd, int iters,
+                double tol, double lambda, int* iters_done, double* res2);
+
+/* calc_CISD (src/sparse_solver.cpp:15-114): the CISD block of ref_det (ref_det and every
+ * determinant coupled to it by a nonzero element), its dense Hamiltonian, eigenpair
+ * `state_idx` (0 = lowest).  out (capacity >= block size) receives the eigenvector in
+ * key order, normalized, its largest-|c| component (smallest key among ties) positive;
+ * *eigenvalue the eigenvalue; *block_size (optional) the block dimension. */
+int sfci_cisd(sfci_ctx* ctx, uint64_t ref_det, int state_idx, sfci_vec* out, double* eigenvalue,
+              int64_t* block_size);
+
+/* sparse_main (src/sparse_solver.cpp:774-809): CISD state 2 of init_det, normalized, then
+ * sfci_shift_invert at target_e.  (The reference's task=excited branch leaves this call
+ * commented out, src/main.cpp:235-242.) */
+int sfci_excited(sfci_ctx* ctx, uint64_t init_det, double target_e, const sfci_linear_params_t* params,
+                 sfci_vec* v_out, double* energy, double* e_cisd);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,14 @@ int oracle_gsaes(int n, const double* K, const double* S, double* evals, double*
     return 0;
 }
 
+// Least squares of the Krylov linear solver (gsaes.h lstsq); returns the rank.
+int oracle_lstsq(int m, int n, const double* A, const double* b, double* x) {
+    std::vector<double> a(A, A + (size_t)m * n), bb(b, b + m), xx;
+    int r = sfci_dense::lstsq(m, n, a, bb, xx);
+    std::copy(xx.begin(), xx.end(), x);
+    return r;
+}
+
 // sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548).
 // Start vector: init_det with coefficient 1.  Writes per-restart Ritz values and
 // <v|H|v> after truncation (up to max_restart each); returns the number of restarts run

@@ -81,6 +81,38 @@ def lib():
         L.oracle_lanczos.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_int64,
                                      C.c_double, C.c_int, C.c_double, _f64p, _f64p, C.POINTER(C.c_double), _u64p,
                                      _f64p, C.POINTER(C.c_int64)]
+        L.oracle_lstsq.restype = C.c_int
+        L.oracle_lstsq.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.oracle_n_conn.restype = C.c_uint64
+        L.oracle_n_conn.argtypes = [C.c_void_p]
+        L.oracle_hv_linear.restype = C.c_int
+        L.oracle_hv_linear.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, C.c_double, _u64p, _f64p, _i64p, C.c_int,
+                                       C.c_double, C.c_int64, _u64p, _f64p, C.POINTER(C.c_int64), _f64p]
+        L.oracle_hv_residual.restype = C.c_int
+        L.oracle_hv_residual.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, _u64p, _f64p, C.c_int64, C.c_double,
+                                         C.c_double, C.c_int64, _u64p, _f64p, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_double)]
+        L.oracle_add_three.restype = C.c_int64
+        L.oracle_add_three.argtypes = [_u64p, _f64p, C.c_int64, _u64p, _f64p, C.c_int64, _u64p, _f64p, C.c_int64,
+                                       C.c_double, C.c_double, C.c_double, C.c_int64, _u64p, _f64p]
+        L.oracle_linear_solve.restype = C.c_int
+        L.oracle_linear_solve.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, C.c_double, C.c_int64, C.c_int,
+                                          C.c_int, C.c_double, _u64p, _f64p, C.POINTER(C.c_int64), _f64p]
+        L.oracle_shift_invert.restype = C.c_int
+        L.oracle_shift_invert.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, C.c_double, C.c_int64, C.c_int,
+                                          C.c_int, C.c_double, _u64p, _f64p, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_double), _f64p]
+        L.oracle_cisd.restype = C.c_int
+        L.oracle_cisd.argtypes = [C.c_void_p, C.c_uint64, C.c_int, _u64p, _f64p, C.c_int64, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_double)]
+        L.oracle_excited.restype = C.c_int
+        L.oracle_excited.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_int64, C.c_int, C.c_int, C.c_double,
+                                     _u64p, _f64p, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.oracle_minres.restype = C.c_int
+        L.oracle_minres.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, _u64p, _f64p, C.c_int64, C.c_int64,
+                                    C.c_int, C.c_double, C.c_double, C.c_double, _u64p, _f64p,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -179,6 +211,96 @@ class Oracle:
         return dict(restarts=r, ritz=ritz[:r].copy(), e_trunc=et[:r].copy(), e_final=ef.value,
                     keys=ok[:olen.value].copy(), vals=ov[:olen.value].copy())
 
+    # --- linear-solver path (linear_oracle.cpp) ---------------------------------------
+    def n_conn(self) -> int:
+        return lib().oracle_n_conn(self._h)
+
+    def hv_linear(self, k, v, lam, basis, cut=1e-5, out_cap=60_000_000):
+        k, v = _u64(k), _f64(v)
+        offs = np.zeros(len(basis) + 1, np.int64)
+        for i, (bk, _) in enumerate(basis):
+            offs[i + 1] = offs[i] + len(bk)
+        bk = _u64(np.concatenate([b[0] for b in basis]) if offs[-1] else np.zeros(1, np.uint64))
+        bv = _f64(np.concatenate([b[1] for b in basis]) if offs[-1] else np.zeros(1))
+        bound = max(1, min(out_cap, len(k) * self.n_conn() + int(offs[-1])))
+        ok, ov = np.zeros(bound, np.uint64), np.zeros(bound)
+        olen = C.c_int64()
+        hij = np.zeros(len(basis))
+        rc = lib().oracle_hv_linear(self._h, k, v, len(k), lam, bk, bv, offs, len(basis), cut, out_cap, ok, ov,
+                                    C.byref(olen), hij)
+        if rc:
+            raise RuntimeError("oracle_hv_linear failed (%d)" % rc)
+        return ok[:olen.value].copy(), ov[:olen.value].copy(), hij
+
+    def hv_residual(self, xk, xv, bk, bv, lam, cut=1e-5, out_cap=60_000_000):
+        xk, xv, bk, bv = _u64(xk), _f64(xv), _u64(bk), _f64(bv)
+        bound = max(1, min(out_cap, len(xk) * self.n_conn() + len(bk)))
+        ok, ov = np.zeros(bound, np.uint64), np.zeros(bound)
+        olen, res = C.c_int64(), C.c_double()
+        rc = lib().oracle_hv_residual(self._h, xk, xv, len(xk), bk, bv, len(bk), lam, cut, out_cap, ok, ov,
+                                      C.byref(olen), C.byref(res))
+        if rc:
+            raise RuntimeError("oracle_hv_residual failed (%d)" % rc)
+        return ok[:olen.value].copy(), ov[:olen.value].copy(), res.value
+
+    def linear_solve(self, bk, bv, target_e, nd, num_iter, num_restart, cut=1e-5):
+        bk, bv = _u64(bk), _f64(bv)
+        xk, xv = np.zeros(nd, np.uint64), np.zeros(nd)
+        xl = C.c_int64()
+        res = np.zeros(num_restart)
+        rc = lib().oracle_linear_solve(self._h, bk, bv, len(bk), target_e, nd, num_iter, num_restart, cut, xk, xv,
+                                       C.byref(xl), res)
+        if rc:
+            raise RuntimeError("oracle_linear_solve failed (%d)" % rc)
+        return xk[:xl.value].copy(), xv[:xl.value].copy(), res
+
+    def shift_invert(self, bk, bv, target_e, nd, num_iter, num_restart, cut=1e-5):
+        bk, bv = _u64(bk), _f64(bv)
+        xk, xv = np.zeros(nd, np.uint64), np.zeros(nd)
+        xl, e = C.c_int64(), C.c_double()
+        res = np.zeros(num_restart)
+        rc = lib().oracle_shift_invert(self._h, bk, bv, len(bk), target_e, nd, num_iter, num_restart, cut, xk, xv,
+                                       C.byref(xl), C.byref(e), res)
+        if rc:
+            raise RuntimeError("oracle_shift_invert failed (%d)" % rc)
+        return xk[:xl.value].copy(), xv[:xl.value].copy(), e.value, res
+
+    def cisd(self, ref, state_idx):
+        cap = self.n_conn()
+        ok, ov = np.zeros(cap, np.uint64), np.zeros(cap)
+        n, e = C.c_int64(), C.c_double()
+        rc = lib().oracle_cisd(self._h, ref, state_idx, ok, ov, cap, C.byref(n), C.byref(e))
+        if rc:
+            raise RuntimeError("oracle_cisd failed (%d)" % rc)
+        return ok[:n.value].copy(), ov[:n.value].copy(), e.value
+
+    def excited(self, init_det, target_e, nd, num_iter, num_restart, cut=1e-5):
+        xk, xv = np.zeros(nd, np.uint64), np.zeros(nd)
+        xl, e, ec = C.c_int64(), C.c_double(), C.c_double()
+        rc = lib().oracle_excited(self._h, init_det, target_e, nd, num_iter, num_restart, cut, xk, xv,
+                                  C.byref(xl), C.byref(e), C.byref(ec))
+        if rc:
+            raise RuntimeError("oracle_excited failed (%d)" % rc)
+        return xk[:xl.value].copy(), xv[:xl.value].copy(), e.value, ec.value
+
+    def minres(self, rk, rv, gk, gv, nd, iters, tol, lam, cut=1e-5):
+        rk, rv, gk, gv = _u64(rk), _f64(rv), _u64(gk), _f64(gv)
+        xk, xv = np.zeros(nd, np.uint64), np.zeros(nd)
+        xl, it, r2 = C.c_int64(), C.c_int(), C.c_double()
+        rc = lib().oracle_minres(self._h, rk, rv, len(rk), gk, gv, len(gk), nd, iters, tol, lam, cut, xk, xv,
+                                 C.byref(xl), C.byref(it), C.byref(r2))
+        if rc:
+            raise RuntimeError("oracle_minres failed (%d)" % rc)
+        return xk[:xl.value].copy(), xv[:xl.value].copy(), it.value, r2.value
+
+
+def add_three(x, y, z, a, b, c, cap):
+    (xk, xv), (yk, yv), (zk, zv) = [(_u64(k), _f64(v)) for k, v in (x, y, z)]
+    n = max(1, len(xk) + len(yk) + len(zk))
+    ok, ov = np.zeros(n, np.uint64), np.zeros(n)
+    m = lib().oracle_add_three(xk, xv, len(xk), yk, yv, len(yk), zk, zv, len(zk), a, b, c, cap, ok, ov)
+    return ok[:m].copy(), ov[:m].copy()
+
 
 # --- sparse vector ops -------------------------------------------------------------
 def dot(ak, av, bk, bv) -> float:
@@ -217,6 +339,13 @@ def lincomb(vecs, coeffs, cap):
     nrm = C.c_double()
     m = lib().oracle_lincomb(kk, kv, offs, len(vecs), _f64(coeffs), cap, ok, ov, C.byref(nrm))
     return ok[:m].copy(), ov[:m].copy()
+
+
+def lstsq(A, b):
+    m, n = A.shape
+    x = np.zeros(n)
+    r = lib().oracle_lstsq(m, n, _f64(A.ravel()), _f64(b), x)
+    return x, r
 
 
 def gsaes(K, S):

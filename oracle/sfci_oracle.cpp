@@ -354,6 +354,13 @@ void oracle_bucket_params(const oracle_ham* h, uint64_t* block_dim, unsigned* bu
     *bucket_num = H->bucket_num;
 }
 
+// Connections per determinant, diagonal included: 1 + 2 k v + 2 C(k,2) C(v,2) + (k v)^2.
+uint64_t oracle_n_conn(const oracle_ham* hp) {
+    const Ham& H = *(const Ham*)hp;
+    uint64_t k = (uint64_t)H.n_alpha, v = (uint64_t)(H.n_so / 2) - k;
+    return 1 + 2 * k * v + 2 * (k * (k - 1) / 2) * (v * (v - 1) / 2) + (k * v) * (k * v);
+}
+
 double oracle_diag(const oracle_ham* h, uint64_t d) { return diag_exact(*(const Ham*)h, d); }
 double oracle_diag_literal(const oracle_ham* h, uint64_t d) { return diag_literal(*(const Ham*)h, d); }
 

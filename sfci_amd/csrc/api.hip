@@ -221,7 +221,8 @@ void sfci_destroy(sfci_ctx* X) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     DevBuf* bufs[] = {&c.cand_k[0], &c.cand_k[1], &c.cand_v[0], &c.cand_v[1], &c.src_cnt, &c.src_off,
-                      &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.lin_k[0], &c.lin_k[1], &c.lin_v[0],
+                      &c.lin_v[1], &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();
@@ -305,8 +306,12 @@ int sfci_kernel_stats(sfci_ctx* X, sfci_kernel_stat_t* out, int n, int reset) {
 }
 
 const char* sfci_kernel_name(int id) {
-    static const char* names[K_NUM] = {"k_generate", "k_count", "k_radix_hist", "k_radix_scatter",
-                                       "scan",       "k_reduce", "select",      "vecops"};
+    static const char* names[K_NUM] = {"k_generate",      "k_count",
+                                       "k_radix_hist<DigitHash>", "k_radix_scatter<DigitHash>",
+                                       "k_radix_hist<DigitPlain>", "k_radix_scatter<DigitPlain>",
+                                       "scan",            "k_bucket_bounds",
+                                       "k_bucket_reduce", "k_reduce",
+                                       "select",          "vecops"};
     return (id >= 0 && id < K_NUM) ? names[id] : "";
 }
 
@@ -514,6 +519,124 @@ int sfci_lincomb(sfci_ctx* X, const sfci_vec* const* vecs, const double* coef, i
     return guarded(c, [&] {
         size_t m = dev_lincomb(c, vs.data(), coef, n, (size_t)out->v.cap, out->v.k, out->v.v);
         out->v.len = (int64_t)m;
+    });
+}
+
+// Cut-then-cap tail of the linear-solver products (src/functions.cpp:1016-1062):
+// dst = top_{dst.cap}({|c| > cut} of src), key order.
+int sfci_shrink_cut(sfci_ctx* X, sfci_vec* dst, const sfci_vec* src, double cut) {
+    if (!X || !dst || !src || dst == src) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        size_t n = (size_t)src->v.len;
+        uint64_t* fk = c->lin_k[0].get<uint64_t>(std::max<size_t>(n, 1));
+        double* fv = c->lin_v[0].get<double>(std::max<size_t>(n, 1));
+        size_t f = dev_filter_above(c, src->v.k, src->v.v, n, cut, fk, fv);
+        dst->v.len = (int64_t)select_top(c, fk, fv, f, (size_t)dst->v.cap, dst->v.k, dst->v.v);
+    });
+}
+
+namespace {
+// The uncut product w = coeff (H - lambda) in, copied into lin_k[0]/lin_v[0] with room for
+// `extra` more entries; returns its length.
+size_t hv_all_into_lin0(Ctx* c, const Vec& in, double lambda, size_t extra, size_t* cap_out) {
+    double d = 0.0;
+    HvAll a;
+    dev_hv_all(c, in.k, in.v, (size_t)in.len, lambda, 1.0, &d, &a);
+    size_t cap = a.n + extra + 1;
+    uint64_t* k0 = c->lin_k[0].get<uint64_t>(cap);
+    double* v0 = c->lin_v[0].get<double>(cap);
+    c->lin_k[1].get<uint64_t>(cap);
+    c->lin_v[1].get<double>(cap);
+    dev_copy(c, k0, v0, a.k, a.v, a.n);
+    *cap_out = cap;
+    return a.n;
+}
+}  // namespace
+
+// spmspvm_bucket_linear (src/functions.cpp:858-1094): w = (H - lambda) in, uncut;
+// hij[j] = <basis[j] | w> for every j < n_basis; w -= hij[n-1] basis[n-1], then
+// w -= hij[n-2] basis[n-2]; out = top_{out.cap}({|w| > cut}), key order.
+int sfci_hv_linear(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, double lambda, const sfci_vec* const* basis,
+                   int n_basis, double* hij) {
+    if (!X || !in || !out || !basis || n_basis < 1 || !hij || out == in) return SFCI_E_ARG;
+    for (int j = 0; j < n_basis; ++j)
+        if (!basis[j] || basis[j] == out) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        for (int j = 0; j < n_basis; ++j) hij[j] = 0.0;
+        out->v.len = 0;
+        if (in->v.len == 0) return;
+        const Vec& bc = basis[n_basis - 1]->v;
+        const Vec* bp = n_basis > 1 ? &basis[n_basis - 2]->v : nullptr;
+        size_t cap = 0;
+        size_t m = hv_all_into_lin0(c, in->v, lambda, (size_t)bc.len + (bp ? (size_t)bp->len : 0), &cap);
+        uint64_t* k[2] = {(uint64_t*)c->lin_k[0].p, (uint64_t*)c->lin_k[1].p};
+        double* v[2] = {(double*)c->lin_v[0].p, (double*)c->lin_v[1].p};
+        for (int j = 0; j < n_basis; ++j)
+            hij[j] = dev_dot(c, k[0], v[0], m, basis[j]->v.k, basis[j]->v.v, (size_t)basis[j]->v.len);
+        m = dev_add_two(c, k[0], v[0], m, bc.k, bc.v, (size_t)bc.len, -hij[n_basis - 1], cap, k[1], v[1]);
+        int cur = 1;
+        if (bp) {
+            m = dev_add_two(c, k[1], v[1], m, bp->k, bp->v, (size_t)bp->len, -hij[n_basis - 2], cap, k[0], v[0]);
+            cur = 0;
+        }
+        size_t f = dev_filter_above(c, k[cur], v[cur], m, c->cut, k[1 - cur], v[1 - cur]);
+        out->v.len = (int64_t)select_top(c, k[1 - cur], v[1 - cur], f, (size_t)out->v.cap, out->v.k, out->v.v);
+    });
+}
+
+// spmspvm_bucket_res_norm (src/functions.cpp:1096-1259): w = (H - lambda) x - b, uncut;
+// *res = ||w||; out = top_{out.cap}({|w| > cut}), key order.
+int sfci_hv_residual(sfci_ctx* X, const sfci_vec* x, sfci_vec* out, double lambda, const sfci_vec* b, double* res) {
+    if (!X || !x || !out || !b || !res || out == x || out == b) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        size_t cap = 0;
+        size_t m = hv_all_into_lin0(c, x->v, lambda, (size_t)b->v.len, &cap);
+        uint64_t* k[2] = {(uint64_t*)c->lin_k[0].p, (uint64_t*)c->lin_k[1].p};
+        double* v[2] = {(double*)c->lin_v[0].p, (double*)c->lin_v[1].p};
+        m = dev_add_two(c, k[0], v[0], m, b->v.k, b->v.v, (size_t)b->v.len, -1.0, cap, k[1], v[1]);
+        *res = dev_norm(c, v[1], m);
+        size_t f = dev_filter_above(c, k[1], v[1], m, c->cut, k[0], v[0]);
+        out->v.len = (int64_t)select_top(c, k[0], v[0], f, (size_t)out->v.cap, out->v.k, out->v.v);
+    });
+}
+
+// calc_CISD (src/sparse_solver.cpp:15-114)
+int sfci_cisd(sfci_ctx* X, uint64_t ref_det, int state_idx, sfci_vec* out, double* eigenvalue, int64_t* block_size) {
+    if (!X || !out || !eigenvalue || state_idx < 0) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    if (popc64(ref_det & 0x5555555555555555ull) != c->ham.k || popc64(ref_det & 0xaaaaaaaaaaaaaaaaull) != c->ham.k ||
+        (c->n_so < 64 && (ref_det >> c->n_so))) {
+        c->last_error = "determinant does not match n_so / n_elec";
+        return SFCI_E_ARG;
+    }
+    return guarded(c, [&] {
+        size_t n = dev_cisd(c, ref_det, state_idx, (size_t)out->v.cap, out->v.k, out->v.v, eigenvalue);
+        out->v.len = (int64_t)n;
+        if (block_size) *block_size = (int64_t)n;
+    });
+}
+
+// dynVec::addThree (src/all.h:482-608): out = top_{out.cap}(a x + b y + c z), then the
+// entries with |c| <= 1e-14 dropped, key order.
+int sfci_add_three(sfci_ctx* X, sfci_vec* out, const sfci_vec* x, const sfci_vec* y, const sfci_vec* z, double a,
+                   double b, double cz) {
+    if (!X || !out || !x || !y || !z) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        const Vec* vs[3] = {&x->v, &y->v, &z->v};
+        const double co[3] = {a, b, cz};
+        size_t cap = std::max<size_t>((size_t)out->v.cap, 1);
+        uint64_t* k0 = c->lin_k[0].get<uint64_t>(cap);
+        double* v0 = c->lin_v[0].get<double>(cap);
+        uint64_t* k1 = c->lin_k[1].get<uint64_t>(cap);
+        double* v1 = c->lin_v[1].get<double>(cap);
+        size_t m = dev_lincomb(c, vs, co, 3, (size_t)out->v.cap, k0, v0);
+        size_t f = dev_filter_above(c, k0, v0, m, 1e-14, k1, v1);
+        dev_copy(c, out->v.k, out->v.v, k1, v1, f);
+        out->v.len = (int64_t)f;
     });
 }
 

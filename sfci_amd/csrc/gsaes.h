@@ -1,11 +1,17 @@
-// Dense generalized symmetric eigenproblem K x = lambda S x for the Nl x Nl projected
-// Lanczos matrices (the reference uses Eigen's GeneralizedSelfAdjointEigenSolver,
-// src/sparse_solver.cpp:287-288, 450-454): Cholesky S = L L^T, cyclic Jacobi on
-// C = L^-1 K L^-T, x = L^-T y.  Eigenvalues ascending; eigenvectors S-normalized with
-// the first nonzero component made positive.  Host code, header-only.
+// Dense kernels of the Krylov drivers, host code, header-only.
+//  gsaes  generalized symmetric eigenproblem K x = lambda S x for the Nl x Nl projected
+//         Lanczos matrices (the reference uses Eigen's GeneralizedSelfAdjointEigenSolver,
+//         src/sparse_solver.cpp:287-288, 450-454): Cholesky S = L L^T, cyclic Jacobi on
+//         C = L^-1 K L^-T, x = L^-T y.  Eigenvalues ascending; eigenvectors S-normalized
+//         with the first nonzero component made positive.
+//  lstsq  least-squares solution of the (n+1) x n system of the Krylov linear solver (the
+//         reference uses Eigen's colPivHouseholderQr().solve, src/sparse_solver.cpp:672):
+//         Householder QR with column pivoting (Businger-Golub), rank decided as Eigen
+//         does (|R_kk| > eps * min(m, n) * |R_00|), free variables set to zero.
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 namespace sfci_dense {
@@ -103,6 +109,61 @@ inline int gsaes(int n, const std::vector<double>& K, const std::vector<double>&
         for (int i = 0; i < n; ++i) evecs[i * n + jj] = sg * x[i];
     }
     return 0;
+}
+
+// min ||A x - b||_2 for A (m x n, row-major).  Returns the numerical rank.
+inline int lstsq(int m, int n, const std::vector<double>& A, const std::vector<double>& b, std::vector<double>& x) {
+    std::vector<double> R(A), y(b);
+    std::vector<int> perm(n);
+    for (int j = 0; j < n; ++j) perm[j] = j;
+    const int d = std::min(m, n);
+    double r00 = 0.0;
+    int rank = 0;
+    for (int k = 0; k < d; ++k) {
+        // pivot: the remaining column of largest norm (recomputed, no downdating)
+        int p = k;
+        double best = -1.0;
+        for (int j = k; j < n; ++j) {
+            double s = 0.0;
+            for (int i = k; i < m; ++i) s += R[i * n + j] * R[i * n + j];
+            if (s > best) { best = s; p = j; }
+        }
+        if (p != k) {
+            for (int i = 0; i < m; ++i) std::swap(R[i * n + k], R[i * n + p]);
+            std::swap(perm[k], perm[p]);
+        }
+        double nrm = std::sqrt(best);
+        if (k == 0) r00 = nrm;
+        if (!(nrm > std::numeric_limits<double>::epsilon() * d * r00) || nrm == 0.0) break;
+        ++rank;
+        double alpha = R[k * n + k] > 0 ? -nrm : nrm;
+        std::vector<double> v(m - k);
+        for (int i = k; i < m; ++i) v[i - k] = R[i * n + k];
+        v[0] -= alpha;
+        double vv = 0.0;
+        for (double t : v) vv += t * t;
+        if (vv > 0.0) {
+            for (int j = k; j < n; ++j) {
+                double s = 0.0;
+                for (int i = k; i < m; ++i) s += v[i - k] * R[i * n + j];
+                s = 2.0 * s / vv;
+                for (int i = k; i < m; ++i) R[i * n + j] -= s * v[i - k];
+            }
+            double s = 0.0;
+            for (int i = k; i < m; ++i) s += v[i - k] * y[i];
+            s = 2.0 * s / vv;
+            for (int i = k; i < m; ++i) y[i] -= s * v[i - k];
+        }
+    }
+    std::vector<double> z(n, 0.0);
+    for (int i = rank - 1; i >= 0; --i) {
+        double s = y[i];
+        for (int j = i + 1; j < rank; ++j) s -= R[i * n + j] * z[j];
+        z[i] = s / R[i * n + i];
+    }
+    x.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) x[perm[j]] = z[j];
+    return rank;
 }
 
 }  // namespace sfci_dense

@@ -711,12 +711,25 @@ constexpr int BR_THREADS = 512;
 constexpr int BR_SLOTS = 6144;                 // 6144 x 24 B = 144 KB of LDS per workgroup
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
-__global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t* __restrict__ bstart,
-                                uint64_t* __restrict__ bend) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint64_t h = target_hash(ck[i] >> 1, hb);
-        if (i == 0 || target_hash(ck[i - 1] >> 1, hb) != h) bstart[h] = i;
-        if (i + 1 == n || target_hash(ck[i + 1] >> 1, hb) != h) bend[h] = i + 1;
+// Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
+// spans [lower_bound(b), lower_bound(b + 1)): one binary search per bound instead of a
+// pass over the keys.
+__device__ __forceinline__ size_t hash_lower_bound(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t b) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = (lo + hi) >> 1;
+        if (target_hash(ck[mid] >> 1, hb) < b) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t n_buckets,
+                                uint64_t* __restrict__ bstart, uint64_t* __restrict__ bend) {
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_buckets;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        bstart[b] = hash_lower_bound(ck, n, hb, b);
+        bend[b] = b + 1 < n_buckets ? hash_lower_bound(ck, n, hb, b + 1) : n;
     }
 }
 
@@ -847,7 +860,7 @@ struct Timer {
 
 template <typename T>
 static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda,
-                       double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag) {
+                       double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag, HvAll* all) {
     const HamDev& H = c->ham;
     auto& st = c->st;
     st = {};
@@ -903,11 +916,16 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     st.n_passes = passes.size();
     // staging for merged output
     size_t stage_cap = std::max<size_t>(out_cap + (out_cap >> 2), (size_t)1 << 16);
+    if (all) {
+        // every distinct target is kept: size from the last observed distinct/candidate ratio
+        double est = (double)total_cand * std::min(1.0, c->unique_ratio * 1.15) + 65536.0;
+        stage_cap = std::max<size_t>(stage_cap, (size_t)std::min<double>(est, (double)total_cand + 1.0));
+    }
     if (c->stage_k.bytes / sizeof(uint64_t) > stage_cap) stage_cap = c->stage_k.bytes / sizeof(uint64_t);
     uint64_t* stk = c->stage_k.get<uint64_t>(stage_cap);
     double* stv = c->stage_v.get<double>(stage_cap);
-    // no output wanted (energy): nothing passes the cut
-    const double cut_eff = out_cap ? c->cut : INFINITY;
+    // no output wanted (energy): nothing passes the cut; all targets wanted: everything does
+    const double cut_eff = all ? -1.0 : (out_cap ? c->cut : INFINITY);
     fx128 diag_total = fx_zero();
     uint64_t uniq_host = 0;
     unsigned long long* d_out_count = (unsigned long long*)c->d_counter;
@@ -979,10 +997,9 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             const size_t nb = (size_t)1 << hb;
             bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
             bend = bstart + nb;
-            HIPCHK(hipMemsetAsync(bstart, 0, 2 * nb * sizeof(uint64_t), c->stream));
-            int tb = kt_begin(c, K_REDUCE, 8.0 * n_pass);
-            k_bucket_bounds<<<grid_for(c, (n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
-                kd, n_pass, hb, bstart, bend);
+            int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
+            k_bucket_bounds<<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
+                kd, n_pass, hb, nb, bstart, bend);
             kt_end(c, tb);
             st.ms_sort += t.stop();
         } else {
@@ -1001,7 +1018,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             acc_clear(c, 1);
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
-                int tr = kt_begin(c, K_REDUCE, (8.0 + sizeof(T)) * n_pass);
+                int tr = kt_begin(c, K_BUCKET_REDUCE, (8.0 + sizeof(T)) * n_pass);
                 k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, kd, vd, bstart, bend, nb, hb,
                                                                       row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk,
                                                                       stv, d_out_count, stage_cap, d_uniq,
@@ -1088,7 +1105,19 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     // 4. top out_cap by |y|, then key order
     size_t m = out_count_host;
     size_t kept = 0;
-    if (out_cap > 0 && m > 0) {
+    if (all) {
+        Timer t(c->stream);
+        uint64_t* tk = c->tmp_k[1].get<uint64_t>(std::max<size_t>(m, 1));
+        double* tv = c->tmp_v[1].get<double>(std::max<size_t>(m, 1));
+        uint64_t* kk = stk;
+        double* vv = stv;
+        radix_sort_pairs(c, &kk, &vv, tk, tv, m, c->n_so);
+        all->k = kk;
+        all->v = vv;
+        all->n = m;
+        kept = m;
+        st.ms_select += t.stop();
+    } else if (out_cap > 0 && m > 0) {
         Timer t(c->stream);
         uint64_t* sel_k = c->tmp_k[0].get<uint64_t>(m);
         double* sel_v = c->tmp_v[0].get<double>(m);
@@ -1109,8 +1138,39 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
 
 size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
               size_t out_cap, uint64_t* ok, double* ov, double* diag) {
-    if (c->precision == 1) return dev_hv_t<float>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag);
-    return dev_hv_t<double>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag);
+    if (c->precision == 1) return dev_hv_t<float>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag, nullptr);
+    return dev_hv_t<double>(c, ik, iv, n_in, nd, lambda, coeff, out_cap, ok, ov, diag, nullptr);
+}
+
+__global__ void k_fill(double* __restrict__ v, size_t n, double x) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = x;
+}
+
+void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const uint64_t** rk, const double** rv,
+                            const double** dterm) {
+    const HamDev& H = c->ham;
+    const size_t n = ns * H.n_conn;
+    double* ones = c->src_v.get<double>(ns);
+    uint64_t* k0 = c->cand_k[0].get<uint64_t>(n);
+    double* v0 = c->cand_v[0].get<double>(n);
+    double* dt = c->dterm.get<double>(ns);
+    k_fill<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(ones, ns, 1.0);
+    k_generate<double><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ones, ns, 0.0, 1.0, 0, H.dim, nullptr,
+                                                                      c->d_sdec, k0, v0, dt, c->d_flags);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (flags_read_clear(c) & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    *rk = k0;
+    *rv = v0;
+    *dterm = dt;
+}
+
+void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, double lambda, double coeff, double* diag,
+                HvAll* all) {
+    *all = HvAll{};
+    if (c->precision == 1) dev_hv_t<float>(c, ik, iv, n_in, n_in, lambda, coeff, 0, nullptr, nullptr, diag, all);
+    else dev_hv_t<double>(c, ik, iv, n_in, n_in, lambda, coeff, 0, nullptr, nullptr, diag, all);
 }
 
 }  // namespace sfci

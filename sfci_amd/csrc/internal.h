@@ -78,9 +78,13 @@ struct HamDev {
 enum KernelId : int {
     K_GENERATE = 0,
     K_COUNT,
-    K_RADIX_HIST,
-    K_RADIX_SCATTER,
+    K_PART_HIST,       // k_radix_hist<DigitHash>: hash-partition histogram
+    K_PART_SCATTER,    // k_radix_scatter<DigitHash>
+    K_SORT_HIST,       // k_radix_hist<DigitPlain>: key sort histogram
+    K_SORT_SCATTER,    // k_radix_scatter<DigitPlain>
     K_SCAN,
+    K_BUCKET_BOUNDS,
+    K_BUCKET_REDUCE,
     K_REDUCE,
     K_SELECT,
     K_VECOPS,
@@ -123,6 +127,7 @@ struct Ctx {
     DevBuf cand_k[2], cand_v[2];
     DevBuf src_cnt, src_off, src_k, src_v, dterm;
     DevBuf stage_k, stage_v;         // merged H.v output before the top-cap select
+    DevBuf lin_k[2], lin_v[2];       // uncut H.v results of the linear-solver products
     DevBuf tmp_k[2], tmp_v[2];
     DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
     DevBuf scratch_hist, scratch_off, scratch_small;
@@ -175,12 +180,32 @@ size_t dev_add_two(Ctx* c, const uint64_t* vk, const double* vv, size_t nv, cons
                    size_t nw, double a, size_t cap, uint64_t* ok, double* ov);
 size_t dev_lincomb(Ctx* c, const Vec* const* vecs, const double* coef, int n, size_t cap, uint64_t* ok, double* ov);
 void dev_copy(Ctx* c, uint64_t* dk, double* dv, const uint64_t* sk, const double* sv, size_t n);
+// Order-preserving compaction of the entries with |v| > cut (strict).  Returns the count.
+size_t dev_filter_above(Ctx* c, const uint64_t* k, const double* v, size_t n, double cut, uint64_t* ok, double* ov);
 
 // hv.hip
 // H.v with the reference's semantics; returns the output length written to (ok, ov)
 // (capacity out_cap, key-sorted) and the exact diag.
 size_t dev_hv(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda, double coeff,
               size_t out_cap, uint64_t* ok, double* ov, double* diag);
+// Every distinct target of coeff (H - lambda) v, no cut and no cap (exact zeros kept),
+// key-sorted.  The result lives in library buffers that the next H.v call reuses.
+struct HvAll {
+    const uint64_t* k = nullptr;
+    const double* v = nullptr;
+    size_t n = 0;
+};
+void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, double lambda, double coeff, double* diag,
+                HvAll* all);
+// cisd.hip: the CISD block eigenvector (state_idx, ascending) of ref_det into (ok, ov),
+// key order; returns the block size.
+size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_t* ok, double* ov,
+                double* eigenvalue);
+// Every connection of each of the ns sources with unit amplitude: record j * n_conn + e
+// holds ((dense target) << 1 | is_diagonal, H_target,source); the diagonal records carry
+// value 0 and dterm[j] = H_jj.  Buffers are the library's candidate buffers.
+void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const uint64_t** rk, const double** rv,
+                            const double** dterm);
 
 // kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
 // stream and folds the elapsed times into c->kclock.stat.

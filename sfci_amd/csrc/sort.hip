@@ -103,10 +103,12 @@ constexpr int SORT_TILE = SFCI_BLOCK * SORT_ROUNDS;   // 4096
 
 // digit of a key: plain bits, or bits of the target hash (key >> 1 hashed, hb bits)
 struct DigitPlain {
+    static constexpr int kHist = K_SORT_HIST, kScatter = K_SORT_SCATTER;
     int shift;
     __device__ __forceinline__ uint32_t operator()(uint64_t k) const { return (uint32_t)(k >> shift) & (RADIX - 1); }
 };
 struct DigitHash {
+    static constexpr int kHist = K_PART_HIST, kScatter = K_PART_SCATTER;
     int shift, hb;
     __device__ __forceinline__ uint32_t operator()(uint64_t k) const {
         return (uint32_t)(target_hash(k >> 1, hb) >> shift) & (RADIX - 1);
@@ -231,13 +233,13 @@ static void lsd_passes(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtm
     T* vout = vtmp;
     for (int shift = 0; shift < bits; shift += RADIX_BITS) {
         auto dg = make(shift);
-        int t0 = kt_begin(c, K_RADIX_HIST, 8.0 * n + 4.0 * RADIX * n_tiles);
+        int t0 = kt_begin(c, decltype(dg)::kHist, 8.0 * n + 4.0 * RADIX * n_tiles);
         k_radix_hist<<<n_tiles, SFCI_BLOCK, 0, c->stream>>>(kin, n, dg, hist, n_tiles);
         kt_end(c, t0);
         int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * n_tiles);
         exclusive_scan_u32(c, hist, (size_t)RADIX * n_tiles, off, nullptr);
         kt_end(c, t1);
-        int t2 = kt_begin(c, K_RADIX_SCATTER, 2.0 * (8.0 + sizeof(T)) * n);
+        int t2 = kt_begin(c, decltype(dg)::kScatter, 2.0 * (8.0 + sizeof(T)) * n);
         k_radix_scatter<<<n_tiles, SCATTER_THREADS, 0, c->stream>>>(kin, vin, kout, vout, n, dg, off, hist, n_tiles);
         kt_end(c, t2);
         HIPCHK(hipGetLastError());

@@ -289,6 +289,26 @@ size_t select_top(Ctx* c, const uint64_t* k, const double* v, size_t n, size_t c
     return cap;
 }
 
+// ---- magnitude filter --------------------------------------------------------------
+__global__ void k_flag_above(const double* __restrict__ v, size_t n, double cut, uint32_t* __restrict__ fl) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        fl[i] = fabs(v[i]) > cut ? 1u : 0u;
+}
+
+size_t dev_filter_above(Ctx* c, const uint64_t* k, const double* v, size_t n, double cut, uint64_t* ok, double* ov) {
+    if (n == 0) return 0;
+    uint32_t* fl = c->sel_b.get<uint32_t>(n);
+    uint64_t* pos = c->sel_c.get<uint64_t>(n + 1);
+    k_flag_above<<<grid_of(n), SFCI_BLOCK, 0, c->stream>>>(v, n, cut, fl);
+    exclusive_scan_u32(c, fl, n, pos, pos + n);
+    k_compact<<<grid_of(n), SFCI_BLOCK, 0, c->stream>>>(k, v, n, fl, pos, ok, ov);
+    HIPCHK(hipGetLastError());
+    uint64_t m = 0;
+    HIPCHK(hipMemcpyAsync(&m, pos + n, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return m;
+}
+
 // ---- add_two -----------------------------------------------------------------------
 // For each element of x, the lower bound in y and whether the key is present in y.
 __global__ void k_lower_bound(const uint64_t* __restrict__ xk, size_t nx, const uint64_t* __restrict__ yk, size_t ny,

@@ -49,6 +49,10 @@ class LanczosParamsT(C.Structure):
                 ("delta_e", C.c_double), ("verbose", C.c_int), ("e_prev_init", C.c_double)]
 
 
+class LinearParamsT(C.Structure):
+    _fields_ = [("nd", C.c_int64), ("num_iter", C.c_int), ("num_restart", C.c_int), ("verbose", C.c_int)]
+
+
 class LanczosResultT(C.Structure):
     _fields_ = [("n_restarts", C.c_int), ("e_final", C.c_double), ("ms_total", C.c_double),
                 ("ms_hv", C.c_double), ("n_hv", C.c_int64)]
@@ -115,6 +119,16 @@ def lib():
         "sfci_shrink_from": (ip, [vp, vp, vp]),
         "sfci_lincomb": (ip, [vp, vp, vp, ip, vp]),
         "sfci_ground_state": (ip, [vp, u64, C.POINTER(LanczosParamsT), vp, vp, C.POINTER(LanczosResultT), vp]),
+        "sfci_shrink_cut": (ip, [vp, vp, vp, dbl]),
+        "sfci_hv_linear": (ip, [vp, vp, vp, dbl, vp, ip, vp]),
+        "sfci_hv_residual": (ip, [vp, vp, vp, dbl, vp, C.POINTER(C.c_double)]),
+        "sfci_add_three": (ip, [vp, vp, vp, vp, vp, dbl, dbl, dbl]),
+        "sfci_linear_solve": (ip, [vp, vp, dbl, C.POINTER(LinearParamsT), vp, vp]),
+        "sfci_shift_invert": (ip, [vp, vp, dbl, C.POINTER(LinearParamsT), vp, C.POINTER(C.c_double), vp]),
+        "sfci_minres": (ip, [vp, vp, vp, vp, i64, ip, dbl, dbl, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "sfci_cisd": (ip, [vp, u64, ip, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "sfci_excited": (ip, [vp, u64, dbl, C.POINTER(LinearParamsT), vp, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -218,7 +232,7 @@ class Context:
         return {f: getattr(s, f) for f, _ in StatsT._fields_}
 
     def kernel_stats(self, reset: bool = False) -> dict:
-        n = 8
+        n = 12
         arr = (KernelStatT * n)()
         self._check(lib().sfci_kernel_stats(self._h, arr, n, 1 if reset else 0))
         return {lib().sfci_kernel_name(i).decode(): dict(ms=arr[i].ms, launches=arr[i].launches,
@@ -283,3 +297,55 @@ class Context:
         n = r.n_restarts
         return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), e_final=r.e_final,
                     ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv)
+
+    # --- linear-solver path --------------------------------------------------------
+    def shrink_cut(self, dst: Vector, src: Vector, cut: float):
+        self._check(lib().sfci_shrink_cut(self._h, dst._h, src._h, cut))
+
+    def hv_linear(self, vin: Vector, vout: Vector, lam: float, basis: Sequence[Vector]) -> np.ndarray:
+        arr = (C.c_void_p * len(basis))(*[b._h.value for b in basis])
+        hij = np.zeros(len(basis))
+        self._check(lib().sfci_hv_linear(self._h, vin._h, vout._h, lam, C.cast(arr, C.c_void_p), len(basis),
+                                         _ptr(hij)))
+        return hij
+
+    def hv_residual(self, x: Vector, out: Vector, lam: float, b: Vector) -> float:
+        r = C.c_double()
+        self._check(lib().sfci_hv_residual(self._h, x._h, out._h, lam, b._h, C.byref(r)))
+        return r.value
+
+    def add_three(self, out: Vector, x: Vector, y: Vector, z: Vector, a: float, b: float, c: float):
+        self._check(lib().sfci_add_three(self._h, out._h, x._h, y._h, z._h, a, b, c))
+
+    def linear_solve(self, b: Vector, target_e: float, nd: int, num_iter: int, num_restart: int,
+                     x_out: Vector, verbose: bool = False) -> np.ndarray:
+        p = LinearParamsT(nd, num_iter, num_restart, 1 if verbose else 0)
+        res = np.zeros(num_restart)
+        self._check(lib().sfci_linear_solve(self._h, b._h, target_e, C.byref(p), x_out._h, _ptr(res)))
+        return res
+
+    def shift_invert(self, b: Vector, target_e: float, nd: int, num_iter: int, num_restart: int,
+                     v_out: Vector, verbose: bool = False):
+        p = LinearParamsT(nd, num_iter, num_restart, 1 if verbose else 0)
+        res = np.zeros(num_restart)
+        e = C.c_double()
+        self._check(lib().sfci_shift_invert(self._h, b._h, target_e, C.byref(p), v_out._h, C.byref(e), _ptr(res)))
+        return e.value, res
+
+    def minres(self, rhs: Vector, guess: Vector, result: Vector, nd: int, iters: int, tol: float, lam: float):
+        it, r2 = C.c_int(), C.c_double()
+        self._check(lib().sfci_minres(self._h, rhs._h, guess._h, result._h, int(nd), iters, tol, lam,
+                                      C.byref(it), C.byref(r2)))
+        return it.value, r2.value
+
+    def cisd(self, ref_det: int, state_idx: int, out: Vector):
+        e, n = C.c_double(), C.c_int64()
+        self._check(lib().sfci_cisd(self._h, ref_det, state_idx, out._h, C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+    def excited(self, init_det: int, target_e: float, nd: int, num_iter: int, num_restart: int, v_out: Vector,
+                verbose: bool = False):
+        p = LinearParamsT(nd, num_iter, num_restart, 1 if verbose else 0)
+        e, ec = C.c_double(), C.c_double()
+        self._check(lib().sfci_excited(self._h, init_det, target_e, C.byref(p), v_out._h, C.byref(e), C.byref(ec)))
+        return e.value, ec.value
