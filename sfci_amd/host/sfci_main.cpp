@@ -2,14 +2,16 @@
 //
 // Reads ./sfci.input (key=value, split at the first '=', no trimming, unknown keys
 // ignored; src/main.cpp:128-151), loads the _IN integral file named by `filename`, and
-// for task=ground runs the truncated Lanczos solve from init_hf on the GPU.
+// for task=ground runs the truncated Lanczos solve from init_hf on the GPU; task=excited
+// runs sparse_main (src/sparse_solver.cpp:774-809): the CISD state 2 of init_hf, then one
+// shift-invert step at target_E with Nl Krylov products per restart.
 // Keys: task, Nd, Nl, num_restart, num_orb, num_e, filename, init_hf, coeff_cut,
 // target_E, num_thread, bucket_unit (the last two are accepted and unused: the GPU
 // path needs neither a thread count nor bucket over-allocation).  Additional keys:
 // hpsi_cap (default 60000000), device (default 0), workspace_gb.
 // Differences from the reference: coeff_cut is honoured (the reference overwrites it
-// with 1e-5, src/main.cpp:196), and task=excited reports that it is not available
-// (the reference's branch is empty, src/main.cpp:235-242).
+// with 1e-5, src/main.cpp:196), and task=excited runs the call the reference's branch
+// leaves commented out (src/main.cpp:235-242).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -83,7 +85,25 @@ int main(int argc, char** argv) {
               << "\ntarget energy:" << target_E << "\nSpMSpV final result cut:" << coeff_cut << "\n";
     std::cout << "\n****************************************************\n";
     if (task == "excited") {
-        std::cout << "task=excited is not available (the reference's branch is empty)\n";
+        sfci_vec* v = nullptr;
+        if ((rc = sfci_vec_create(ctx, Nd, &v))) {
+            fprintf(stderr, "vector allocation failed (%d)\n", rc);
+            sfci_destroy(ctx);
+            return 1;
+        }
+        sfci_linear_params_t lp{Nd, (int)Nl, (int)num_restart, 1};
+        double e = 0.0, e_cisd = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        rc = sfci_excited(ctx, init_hf, target_E, &lp, v, &e, &e_cisd);
+        if (rc) {
+            fprintf(stderr, "excited-state solve failed (%d): %s\n", rc, sfci_last_error(ctx));
+            sfci_vec_destroy(v);
+            sfci_destroy(ctx);
+            return 1;
+        }
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << "From CISD: " << e_cisd << "\nenergy after 0th SI:" << e << "\nSFCI time: " << dt << std::endl;
+        sfci_vec_destroy(v);
     } else if (task == "ground") {
         double ehf = 0.0;
         if ((rc = sfci_diag_element(ctx, init_hf, &ehf))) {

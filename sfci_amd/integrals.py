@@ -14,6 +14,13 @@ four two-electron blocks, one-electron lines emitted as ``(m, n)`` and, when ``m
 difference: the reference keeps its accumulator lists across files converted in one
 call (``convert_fcidump.py:33-35``); here every call converts one file from scratch.
 
+``convert_uhf`` restates ``conv_uhf`` (``util/convert_fcidump.py:153-276``) for the UHF
+FCIDUMP layout: after the header, the ``aa``, ``bb`` and ``ab`` two-electron blocks, the
+``a`` and ``b`` one-electron blocks and the core line, consecutive blocks separated by a
+line whose four indices are 0 and whose value is exactly 0.0.  The ``ab`` block serves
+both the ``ud`` and the ``du`` output blocks (``convert_fcidump.py:197-200``); the first
+block is the "u" spin.
+
 ``synth_fcidump`` writes the synthetic integrals of the benchmark workloads.  Draw
 order from ``numpy.random.default_rng(seed)``:
 
@@ -40,6 +47,9 @@ __all__ = [
     "write_fcidump",
     "synth_fcidump",
     "convert_rhf",
+    "convert_uhf",
+    "write_fcidump_uhf",
+    "synth_uhf_integrals",
     "read_fcidump",
     "WORKLOADS",
     "make_workload",
@@ -203,6 +213,136 @@ def convert_rhf(fcidump_path: str, out_path: Optional[str] = None) -> str:
             f.write("%d\n" % len(out))
             f.writelines(out)
     return out_path
+
+
+def convert_uhf(fcidump_path: str, out_path: Optional[str] = None) -> str:
+    """UHF FCIDUMP -> ``_IN`` with ``conv_uhf`` semantics (``util/convert_fcidump.py:153-276``)."""
+    if out_path is None:
+        out_path = fcidump_path.split("_")[0] + "_IN"
+    groups: List[list] = []
+    raw: List[list] = []
+    started = False
+    with open(fcidump_path) as f:
+        for line in f:
+            if started:
+                t = line.split()
+                if t[1] == "0" and t[2] == "0" and t[3] == "0" and t[4] == "0" and float(t[0]) == 0.0:
+                    groups.append(raw)
+                    raw = []
+                else:
+                    raw.append([t[0], int(t[1]), int(t[2]), int(t[3]), int(t[4])])
+            if line == " /\n":
+                started = True
+    groups.append(raw)
+    groups = groups[:3] + [groups[2]] + groups[3:]
+    labels = ["uu", "dd", "ud", "du", "u", "d", "c"]
+    seen: Dict[Tuple[int, int, int, int], bool] = {}
+
+    def emit(out, coeff, m, n, p, q):
+        key = (m, n, p, q)
+        if key not in seen:
+            out.append("%s %d %d %d %d\n" % (coeff, m, n, p, q))
+            seen[key] = True
+
+    blocks = []
+    for spin, rows in zip(labels, groups):
+        out: List[str] = []
+        if len(spin) == 2:
+            s1 = 1 if spin[0] == "u" else 0
+            s2 = 1 if spin[1] == "u" else 0
+            for c, i, j, k, l in rows:
+                m, n, p, q = (i << 1) + s1, (j << 1) + s1, (k << 1) + s2, (l << 1) + s2
+                if m != p and n != q:
+                    emit(out, c, m, n, p, q)
+                    emit(out, c, n, m, q, p)
+                    emit(out, c, p, q, m, n)
+                    emit(out, c, q, p, n, m)
+                if n != p and m != q:
+                    emit(out, c, n, m, p, q)
+                    emit(out, c, p, q, n, m)
+                    emit(out, c, m, n, q, p)
+                    emit(out, c, q, p, m, n)
+        elif spin == "c":
+            out = ["%s 0 0 0 0\n" % r[0] for r in rows]
+        else:
+            sp = 1 if spin == "u" else 0
+            for c, i, j, _, _ in rows:
+                m, n = (i << 1) + sp, (j << 1) + sp
+                out.append("%s %d %d 0 0\n" % (c, m, n))
+                if m != n:
+                    out.append("%s %d %d 0 0\n" % (c, n, m))
+        blocks.append(out)
+    with open(out_path, "w") as f:
+        for out in blocks:
+            f.write("%d\n" % len(out))
+            f.writelines(out)
+    return out_path
+
+
+def synth_uhf_integrals(n_orb: int, seed: int, delta: float = 0.02, **kw):
+    """UHF-style integrals: the RHF-style set of ``synth_integrals(n_orb, seed, **kw)``
+    plus spin-dependent perturbations drawn from ``default_rng(seed + 1000)``.
+    Returns ``(h_a, h_b, g_aa, g_bb, g_ab, ecore)`` with dense arrays; ``g_aa``/``g_bb``
+    keep the 8-fold symmetry, ``g_ab[i,j,k,l]`` only ``i <-> j`` and ``k <-> l``."""
+    h, canon, _ = synth_integrals(n_orb, seed, **kw)
+    n = n_orb
+    g = np.zeros((n, n, n, n))
+    for v, i, j, k, l in canon:
+        for a, b, c, d in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                           (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)):
+            g[a, b, c, d] = v
+    rng = np.random.default_rng(seed + 1000)
+
+    def sym2(x):
+        return 0.5 * (x + x.T)
+
+    def sym8(x):
+        y = x + x.transpose(1, 0, 2, 3)
+        y = y + y.transpose(0, 1, 3, 2)
+        y = y + y.transpose(2, 3, 0, 1)
+        return y / 8.0
+
+    def sym4(x):
+        y = x + x.transpose(1, 0, 2, 3)
+        y = y + y.transpose(0, 1, 3, 2)
+        return y / 4.0
+
+    h_a = h + delta * sym2(rng.normal(size=(n, n)))
+    h_b = h + delta * sym2(rng.normal(size=(n, n)))
+    g_aa = g + delta * sym8(rng.normal(size=(n, n, n, n)))
+    g_bb = g + delta * sym8(rng.normal(size=(n, n, n, n)))
+    g_ab = g + delta * sym4(rng.normal(size=(n, n, n, n)))
+    return h_a, h_b, g_aa, g_bb, g_ab, 1.25
+
+
+def write_fcidump_uhf(path: str, h_a, h_b, g_aa, g_bb, g_ab, n_elec: int, ecore: float) -> str:
+    """UHF FCIDUMP: ``aa`` and ``bb`` blocks over canonical ``i>=j, k>=l, ij>=kl``, the
+    ``ab`` block over every ``i>=j`` (a) and ``k>=l`` (b), then the ``a`` and ``b``
+    one-electron blocks (``i>=j``) and the core line, separated by zero lines."""
+    n = h_a.shape[0]
+    sep = "%s 0 0 0 0\n" % _fmt(0.0)
+    pairs = [(i, j) for i in range(n) for j in range(i + 1)]
+    with open(path, "w") as f:
+        f.write(" &FCI NORB=%3d,NELEC=%3d,MS2= 0,\n" % (n, n_elec))
+        f.write("  ORBSYM=" + ",".join(["1"] * n) + ",\n")
+        f.write("  ISYM=1,IUHF=1,\n")
+        f.write(" /\n")
+        for g in (g_aa, g_bb):
+            for a, (i, j) in enumerate(pairs):
+                for b in range(a + 1):
+                    k, l = pairs[b]
+                    f.write("%s %3d %3d %3d %3d\n" % (_fmt(g[i, j, k, l]), i + 1, j + 1, k + 1, l + 1))
+            f.write(sep)
+        for (i, j) in pairs:
+            for (k, l) in pairs:
+                f.write("%s %3d %3d %3d %3d\n" % (_fmt(g_ab[i, j, k, l]), i + 1, j + 1, k + 1, l + 1))
+        f.write(sep)
+        for h in (h_a, h_b):
+            for (i, j) in pairs:
+                f.write("%s %3d %3d   0   0\n" % (_fmt(h[i, j]), i + 1, j + 1))
+            f.write(sep)
+        f.write("%s   0   0   0   0\n" % _fmt(ecore))
+    return path
 
 
 def synth_fcidump(path: str, n_orb: int, n_elec: int, seed: int, **kw) -> str:

@@ -25,14 +25,21 @@ def _cre(d, p):
 
 
 def apply_h(h, eri, ecore, d, n_orb):
-    """Return {det: <det|H|d>} for one determinant."""
+    """Return {det: <det|H|d>} for one determinant.
+
+    ``h`` and ``eri`` are spatial arrays, or spin-resolved: ``h = {s: h_s}`` and
+    ``eri = {(s_pq, s_rs): eri_block}`` with s the spin bit of the spin orbitals."""
+    if not isinstance(h, dict):
+        h = {0: h, 1: h}
+    if not isinstance(eri, dict):
+        eri = {(a, b): eri for a in (0, 1) for b in (0, 1)}
     out = {d: ecore}
     nso = 2 * n_orb
     for p in range(nso):
         for q in range(nso):
             if (p & 1) != (q & 1):
                 continue
-            v = h[p >> 1, q >> 1]
+            v = h[p & 1][p >> 1, q >> 1]
             if v == 0:
                 continue
             d1, s1 = _ann(d, q)
@@ -59,7 +66,7 @@ def apply_h(h, eri, ecore, d, n_orb):
                 for p in range(nso):
                     if (p & 1) != (q & 1):
                         continue
-                    v = eri[p >> 1, q >> 1, r >> 1, s >> 1]
+                    v = eri[(p & 1, r & 1)][p >> 1, q >> 1, r >> 1, s >> 1]
                     if v == 0:
                         continue
                     d4, s4 = _cre(d3, p)

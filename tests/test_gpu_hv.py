@@ -278,3 +278,27 @@ def test_lanczos_mixed_energy_error(sys1, ctx1):
         ctx1.set_precision(False)
     log("lanczos e64 %.12f mixed %.12f" % (g64["e_final"], g32["e_final"]))
     assert abs(g64["e_final"] - g32["e_final"]) < 1e-5
+
+
+def test_hv_uhf_bit_exact(tmp_path):
+    # spin-dependent integrals through the UHF converter: all four two-electron spin
+    # blocks and both one-electron blocks differ
+    from sfci_amd import gpu, integrals as I
+    h_a, h_b, g_aa, g_bb, g_ab, ec = I.synth_uhf_integrals(10, 5)
+    fc = str(tmp_path / "u10_fcidump")
+    I.write_fcidump_uhf(fc, h_a, h_b, g_aa, g_bb, g_ab, 8, ec)
+    inp = I.convert_uhf(fc, str(tmp_path / "u10_IN"))
+    orc = O.Oracle(inp, 20, 8)
+    ctx = gpu.Context(inp, 20, 8)
+    try:
+        sysu = dict(n_orb=10, n_elec=8)
+        k, v = random_input(sysu, 2000, 51)
+        for mixed in (False, True):
+            log("uhf hv mixed=%s" % mixed)
+            ctx.set_precision(mixed)
+            gk, gv, gd = gpu_hv(ctx, k, v, len(k), 0.15, 0.9, 60_000_000)
+            ok, ov, od, _ = orc.hv(k, v, nd=len(k), lam=0.15, coeff=0.9, cut=1e-5, mode=2 if mixed else 0)
+            assert_same_vec(gk, gv, ok, ov)
+            assert bits_of(gd) == bits_of(od)
+    finally:
+        ctx.close()

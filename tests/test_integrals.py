@@ -6,6 +6,8 @@ import os
 import numpy as np
 import pytest
 
+import sq_bruteforce as B
+from oracle import oracle as O
 from sfci_amd import integrals as I
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -100,3 +102,81 @@ def test_real_molecule_c2_hf_energy(tmp_path):
     # symmetry-sparse: many connections have an exactly zero element
     k, v = o.connections(hf)
     assert (v == 0.0).sum() > 0 and len(k) == 1 + 2 * 4 * 22 + 2 * 6 * 231 + (4 * 22) ** 2
+
+
+def _uhf_system(tmp_path, n_orb, n_elec, seed, same_spin=False):
+    h_a, h_b, g_aa, g_bb, g_ab, ec = I.synth_uhf_integrals(n_orb, seed)
+    if same_spin:
+        h_b, g_aa, g_bb, g_ab = h_a, g_ab, g_ab, g_ab
+        g_ab = 0.5 * (g_ab + g_ab.transpose(2, 3, 0, 1))
+        g_aa = g_bb = g_ab
+    fc = str(tmp_path / ("uhf%d_fcidump" % n_orb))
+    I.write_fcidump_uhf(fc, h_a, h_b, g_aa, g_bb, g_ab, n_elec, ec)
+    inp = I.convert_uhf(fc, str(tmp_path / ("uhf%d_IN" % n_orb)))
+    return inp, (h_a, h_b, g_aa, g_bb, g_ab, ec)
+
+
+def test_convert_uhf_block_layout(tmp_path):
+    inp, _ = _uhf_system(tmp_path, 4, 4, 2)
+    lines = open(inp).read().split("\n")
+    counts, t = [], 0
+    while t < len(lines) and lines[t].strip():
+        c = int(lines[t])
+        counts.append(c)
+        blk = lines[t + 1:t + 1 + c]
+        for ln in blk:
+            v, m, n, p, q = ln.split()
+            m, n, p, q = int(m), int(n), int(p), int(q)
+            if len(counts) <= 2:
+                sp = {1: 1, 2: 0}[len(counts)]
+                assert (m & 1, n & 1, p & 1, q & 1) == (sp,) * 4
+            elif len(counts) <= 4:
+                # opposite-spin blocks hold both (ab|..) and their (..|ab) images
+                assert (m & 1, n & 1, p & 1, q & 1) in ((1, 1, 0, 0), (0, 0, 1, 1))
+        t += 1 + c
+    assert len(counts) == 7 and counts[6] == 1
+    # every aa / bb / ab spin-orbital key appears once; the du block adds nothing that the
+    # ud block's (p, q, m, n) images did not already hold
+    assert counts[3] == 0
+    n = 4
+    assert counts[4] == counts[5] == n * n
+
+
+def test_convert_uhf_equals_rhf_for_spin_free_integrals(tmp_path, make_system):
+    # the same spatial integrals written as an RHF and as a UHF FCIDUMP give the same H
+    h, canon, _ = I.synth_integrals(6, 4)
+    n = 6
+    g = np.zeros((n,) * 4)
+    for v, i, j, k, l in canon:
+        for a, b, c, d in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                           (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)):
+            g[a, b, c, d] = v
+    fr = str(tmp_path / "r_fcidump")
+    I.write_fcidump(fr, h, canon, 6, 1.25)
+    ir = I.convert_rhf(fr, str(tmp_path / "r_IN"))
+    fu = str(tmp_path / "u_fcidump")
+    I.write_fcidump_uhf(fu, h, h, g, g, g, 6, 1.25)
+    iu = I.convert_uhf(fu, str(tmp_path / "u_IN"))
+    orr, ou = O.Oracle(ir, 12, 6), O.Oracle(iu, 12, 6)
+    dets = np.array(B.sz0_dets(6, 3), dtype=np.uint64)
+    rng = np.random.default_rng(0)
+    k = np.sort(rng.choice(dets, size=150, replace=False))
+    v = rng.normal(size=len(k))
+    a, b = orr.hv(k, v, cut=0.0), ou.hv(k, v, cut=0.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_uhf_slater_condon_vs_second_quantized(tmp_path):
+    inp, (h_a, h_b, g_aa, g_bb, g_ab, ec) = _uhf_system(tmp_path, 5, 4, 3)
+    o = O.Oracle(inp, 10, 4)
+    hd = {1: h_a, 0: h_b}
+    ed = {(1, 1): g_aa, (0, 0): g_bb, (1, 0): g_ab, (0, 1): g_ab.transpose(2, 3, 0, 1)}
+    dets = B.sz0_dets(5, 2)
+    for d in dets[::7]:
+        ref = B.apply_h(hd, ed, ec, d, 5)
+        k, v = o.connections(d)
+        got = dict(zip(k.tolist(), v.tolist()))
+        for t, x in ref.items():
+            assert abs(got.get(t, 0.0) - x) < 1e-12, (d, t)
+        for t, x in got.items():
+            assert abs(ref.get(t, 0.0) - x) < 1e-12, (d, t)
