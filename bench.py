@@ -208,6 +208,12 @@ def main():
                         for k, x in ks.items() if x["launches"]})
     if mixed:
         res["precision"] = "FP32 candidate products, exact 128-bit fixed-point sums, FP64 vectors"
+    # energy readout <v|H|v> (calc_energy) on the same vector, outside the timed steps
+    ctx.sync()
+    t = time.perf_counter()
+    e = ctx.energy(v)
+    ctx.sync()
+    res["energy_readout"] = dict(ms=(time.perf_counter() - t) * 1e3, energy=e)
     if not args.no_cpu:
         keys, vals = v.download()
         res["cpu_baseline"] = cpu_baseline(w, keys, vals, n_conn)

@@ -581,14 +581,38 @@ int64_t oracle_lincomb(const uint64_t* kk, const double* kv, const int64_t* offs
     return (int64_t)m.size();
 }
 
-// calc_energy (src/functions.cpp:1272-1313): <v|H|v> = the diag output of H.v.
+// calc_energy (src/functions.cpp:1272-1313): <v|H|v> = sum_i c_i (c_i H_ii + sum_{t in v,
+// t != i} H_ti c_t), per source as the reference pulls it, with exact sums:
+// E_i = round(fx(c_i H_ii) + sum fx(H_ti c_t)), energy = round(sum fx(c_i E_i)).
 double oracle_energy(const oracle_ham* h, const uint64_t* k, const double* v, int64_t n, int nthreads) {
-    std::vector<uint64_t> ok(1);
-    std::vector<double> ov(1);
-    int64_t ol;
-    double d = 0.0;
-    oracle_hv(h, k, v, n, n, 0.0, 1.0, INFINITY, 0, ok.data(), ov.data(), &ol, &d, nthreads, 0, nullptr);
-    return d;
+    const Ham& H = *(const Ham*)h;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    std::vector<fx_t> part(nthreads, 0);
+#pragma omp parallel num_threads(nthreads)
+    {
+        fx_t mine = 0;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < n; ++i) {
+            const det_t d = k[i];
+            const double ci = v[i];
+            fx_t e = 0;
+            bool first = true;
+            connections_exact(H, d, [&](det_t t, double hij) {
+                if (first) {   // the diagonal comes first
+                    e += fx_from(ci * hij);
+                    first = false;
+                    return;
+                }
+                const uint64_t* it = std::lower_bound(k, k + n, t);
+                if (it != k + n && *it == t) e += fx_from(hij * v[it - k]);
+            });
+            mine += fx_from(ci * fx_to(e));
+        }
+        part[omp_get_thread_num()] = mine;
+    }
+    fx_t tot = 0;
+    for (auto p : part) tot += p;
+    return fx_to(tot);
 }
 
 }  // extern "C"

@@ -311,7 +311,8 @@ const char* sfci_kernel_name(int id) {
                                        "k_radix_hist<DigitPlain>", "k_radix_scatter<DigitPlain>",
                                        "scan",            "k_bucket_bounds",
                                        "k_bucket_reduce", "k_reduce",
-                                       "select",          "vecops"};
+                                       "select",          "vecops",
+                                       "k_energy"};
     return (id >= 0 && id < K_NUM) ? names[id] : "";
 }
 
@@ -436,18 +437,8 @@ int sfci_hv(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, int64_t nd, double l
 int sfci_energy(sfci_ctx* X, const sfci_vec* v, double* e) {
     if (!X || !v || !e || v->v.ctx != &X->c) return SFCI_E_ARG;
     Ctx* c = &X->c;
-    // the Rayleigh quotient is always formed from FP64 candidates, whatever the H.v precision
-    struct PrecisionGuard {
-        Ctx* c;
-        int saved;
-        ~PrecisionGuard() { c->precision = saved; }
-    } pg{c, c->precision};
-    c->precision = SFCI_PREC_FP64;
-    return guarded(c, [&] {
-        double d = 0.0;
-        dev_hv(c, v->v.k, v->v.v, (size_t)v->v.len, (size_t)v->v.len, 0.0, 1.0, 0, nullptr, nullptr, &d);
-        *e = d;
-    });
+    // FP64 throughout, whatever the H.v precision
+    return guarded(c, [&] { *e = dev_energy(c, v->v.k, v->v.v, (size_t)v->v.len); });
 }
 
 int sfci_diag_element(sfci_ctx* X, uint64_t det, double* e) {

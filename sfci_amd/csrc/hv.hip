@@ -475,6 +475,216 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
     if (range_bad) atomicOr(flags, FLAG_RANGE);
 }
 
+// ---- energy readout: calc_energy (src/functions.cpp:1272-1313) ---------------------
+// <v|H|v> = sum_i c_i (c_i H_ii + sum_{t in v, t != i} H_ti c_t), pulled per source: each
+// connection of source i is looked up in v (row segment of the target's spin-0 string,
+// then a binary search over the sorted spin-1 ranks of that row) and only the members
+// are evaluated.  Exact: E_i = round(fx(c_i H_ii) + sum fx(H_ti c_t)), energy =
+// round(sum fx(c_i E_i)).  No candidate list is written.
+__device__ __forceinline__ uint32_t seg_find(const uint32_t* __restrict__ r1, uint32_t lo, uint32_t hi, uint32_t b) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (r1[mid] < b) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(SFCI_BLOCK) void k_energy(HamDev Hg, const uint64_t* __restrict__ sk,
+                                                       const double* __restrict__ sv, size_t n_src,
+                                                       const uint32_t* __restrict__ row_off,
+                                                       const uint32_t* __restrict__ r1s, const double* __restrict__ cs,
+                                                       const uint16_t* __restrict__ sdec_g,
+                                                       unsigned long long* acc_out, uint32_t* flags) {
+    __shared__ SrcLds s;
+    __shared__ uint16_t sdec[MAX_SINGLES];
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
+    __shared__ uint16_t e_sa[MAX_SINGLES];
+    __shared__ uint32_t e_lo[MAX_SINGLES], e_hi[MAX_SINGLES];
+    __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
+    __shared__ uint64_t s_rlo[SFCI_BLOCK / SFCI_WAVE], s_rhi[SFCI_BLOCK / SFCI_WAVE];
+    __shared__ uint32_t s_own[2];
+    __shared__ int s_ne;
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair;
+    for (uint32_t x = tid; x < nsa; x += SFCI_BLOCK) sdec[x] = sdec_g[x];
+    __syncthreads();
+    bool bad = false;
+    fx128 total = fx_zero();   // thread 0: sum of fx(c_i E_i) over this block's sources
+    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+        load_source(H, s, sdec, sk[j], sv[j], 0.0, 0, H.dim, flags);
+        // spin-0 singles whose target row holds members of v (nsa <= SFCI_BLOCK)
+        {
+            bool have = false;
+            uint32_t lo = 0, hi = 0;
+            if ((uint32_t)tid < nsa) {
+                uint64_t r = s.srank[0][tid];
+                lo = row_off[r];
+                hi = row_off[r + 1];
+                have = hi > lo;
+            }
+            uint64_t bal = __ballot(have);
+            uint32_t my = (uint32_t)popc64(bal & lt);
+            if (lane == 0) s_wcnt[w] = (uint32_t)popc64(bal);
+            if (tid == 0) {
+                s_own[0] = row_off[s.rank[0]];
+                s_own[1] = row_off[s.rank[0] + 1];
+            }
+            __syncthreads();
+            uint32_t pre = 0, tot = 0;
+            for (int i2 = 0; i2 < SFCI_BLOCK / SFCI_WAVE; ++i2) {
+                if (i2 < w) pre += s_wcnt[i2];
+                tot += s_wcnt[i2];
+            }
+            if (have) {
+                e_sa[pre + my] = (uint16_t)tid;
+                e_lo[pre + my] = lo;
+                e_hi[pre + my] = hi;
+            }
+            if (tid == 0) s_ne = (int)tot;
+            __syncthreads();
+        }
+        const uint64_t d = s.key;
+        const uint32_t own_lo = s_own[0], own_hi = s_own[1];
+        const uint32_t rb = (uint32_t)s.rank[1];
+        fx128 acc = fx_zero();
+        if (tid == 0) acc = fx_from(s.diag_val);
+        // A1: spin-1 singles (own row)
+        for (uint32_t f = tid; f < nsa; f += SFCI_BLOCK) {
+            uint32_t b1 = (uint32_t)s.srank[1][f];
+            uint32_t x = seg_find(r1s, own_lo, own_hi, b1);
+            if (x < own_hi && r1s[x] == b1) {
+                uint16_t dd = sdec[f];
+                int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
+                double t = single_elem(H, s, 2 * jo + 1, 2 * b + 1) * cs[x];
+                bad |= !fx_in_range(t);
+                acc = fx_add(acc, fx_from(t));
+            }
+        }
+        // A2: spin-1 doubles (own row)
+        {
+            DivIter it(tid, SFCI_BLOCK, nvp);
+            for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
+                uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+                int i = s.occ[1][op & 0xff], jo = s.occ[1][op >> 8];
+                int a = s.vir[1][vp & 0xff], b = s.vir[1][vp >> 8];
+                uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                uint32_t b1 = (uint32_t)colex_rank(st, H.binom);
+                uint32_t x = seg_find(r1s, own_lo, own_hi, b1);
+                if (x < own_hi && r1s[x] == b1) {
+                    double t = double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1) * cs[x];
+                    bad |= !fx_in_range(t);
+                    acc = fx_add(acc, fx_from(t));
+                }
+            }
+        }
+        const uint32_t ne = (uint32_t)s_ne;
+        // B1: spin-0 singles, spin-1 string unchanged
+        for (uint32_t f = tid; f < ne; f += SFCI_BLOCK) {
+            uint32_t x = seg_find(r1s, e_lo[f], e_hi[f], rb);
+            if (x < e_hi[f] && r1s[x] == rb) {
+                uint16_t da = sdec[e_sa[f]];
+                int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
+                double t = single_elem(H, s, 2 * i, 2 * a) * cs[x];
+                bad |= !fx_in_range(t);
+                acc = fx_add(acc, fx_from(t));
+            }
+        }
+        // B2: opposite-spin doubles over the populated spin-0 rows
+        {
+            const uint32_t nB = ne * nsa;
+            DivIter it(tid, SFCI_BLOCK, nsa ? nsa : 1);
+            for (uint32_t f = tid; f < nB; f += SFCI_BLOCK, it.next()) {
+                uint32_t e = it.q, sb = it.r;
+                uint32_t b1 = (uint32_t)s.srank[1][sb];
+                uint32_t x = seg_find(r1s, e_lo[e], e_hi[e], b1);
+                if (x < e_hi[e] && r1s[x] == b1) {
+                    uint16_t da = sdec[e_sa[e]], db = sdec[sb];
+                    int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
+                    int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
+                    double t = opposite_double(H, d, i, a, jo, b) * cs[x];
+                    bad |= !fx_in_range(t);
+                    acc = fx_add(acc, fx_from(t));
+                }
+            }
+        }
+        // C: spin-0 doubles, spin-1 string unchanged
+        {
+            DivIter it(tid, SFCI_BLOCK, nvp);
+            for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
+                uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+                int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
+                int a = s.vir[0][vp & 0xff], b = s.vir[0][vp >> 8];
+                uint32_t st = s.str[0] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                uint64_t r0 = colex_rank(st, H.binom);
+                uint32_t lo = row_off[r0], hi = row_off[r0 + 1];
+                if (hi > lo) {
+                    uint32_t x = seg_find(r1s, lo, hi, rb);
+                    if (x < hi && r1s[x] == rb) {
+                        double t = double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b) * cs[x];
+                        bad |= !fx_in_range(t);
+                        acc = fx_add(acc, fx_from(t));
+                    }
+                }
+            }
+        }
+        // E_i = round(sum), then c_i E_i into the block total
+        for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) {
+            fx128 o;
+            o.lo = __shfl_down(acc.lo, off, SFCI_WAVE);
+            o.hi = __shfl_down(acc.hi, off, SFCI_WAVE);
+            acc = fx_add(acc, o);
+        }
+        if (lane == 0) {
+            s_rlo[w] = acc.lo;
+            s_rhi[w] = acc.hi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            fx128 t = fx_zero();
+            for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) t = fx_add(t, fx128{s_rlo[i], s_rhi[i]});
+            double ci_ei = s.temp * fx_to(t);
+            bad |= !fx_in_range(ci_ei);
+            total = fx_add(total, fx_from(ci_ei));
+        }
+        __syncthreads();
+    }
+    if (tid == 0) fx_atomic_add(acc_out, total);
+    if (bad) atomicOr(flags, FLAG_RANGE);
+}
+
+// membership index of v: spin-1 ranks sorted inside each spin-0 row, row r spanning
+// [row_off[r], row_off[r + 1])
+__global__ void k_dense_split(HamDev H, const uint64_t* __restrict__ dets, const double* __restrict__ c, size_t n,
+                              uint64_t* __restrict__ dk, double* __restrict__ dv) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t r0 = colex_rank(spin_string(dets[i], 0), H.binom);
+        uint64_t r1 = colex_rank(spin_string(dets[i], 1), H.binom);
+        dk[i] = r0 * H.dim + r1;
+        dv[i] = c[i];
+    }
+}
+
+__global__ void k_row_index(const uint64_t* __restrict__ dk, size_t n, uint64_t dim, uint32_t* __restrict__ row_off,
+                            uint32_t* __restrict__ r1s) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r <= dim; r += stride) {
+        // first entry with row >= r
+        size_t lo = 0, hi = n;
+        const uint64_t key = r * dim;
+        while (lo < hi) {
+            size_t mid = (lo + hi) >> 1;
+            if (dk[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        row_off[r] = (uint32_t)lo;
+    }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) r1s[i] = (uint32_t)(dk[i] % dim);
+}
+
 // Finish one merged target: for a target that is an input determinant (tag), add the
 // FP64 diagonal term of its source and accumulate y_t * c_t into diag; round the exact
 // sum; decide the cut.  The determinant key is rebuilt from the dense index only when
@@ -1164,6 +1374,35 @@ void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const uint64_
     *rk = k0;
     *rv = v0;
     *dterm = dt;
+}
+
+
+double dev_energy(Ctx* c, const uint64_t* k, const double* v, size_t n) {
+    const HamDev& H = c->ham;
+    if (n == 0) return 0.0;
+    if (n >= 0xffffffffull) throw HipError("energy: vector longer than 2^32 - 1", -4);
+    uint64_t* dk = c->tmp_k[0].get<uint64_t>(n);
+    double* dv = c->tmp_v[0].get<double>(n);
+    uint64_t* dk2 = c->tmp_k[1].get<uint64_t>(n);
+    double* dv2 = c->tmp_v[1].get<double>(n);
+    k_dense_split<<<grid_for(c, n, 16), SFCI_BLOCK, 0, c->stream>>>(H, k, v, n, dk, dv);
+    uint64_t* sk = dk;
+    double* sv = dv;
+    radix_sort_pairs(c, &sk, &sv, dk2, dv2, n, bits_for(H.dim * H.dim));
+    uint32_t* row_off = c->sel_b.get<uint32_t>(H.dim + 1);
+    uint32_t* r1s = c->sel_a.get<uint32_t>(n);
+    k_row_index<<<grid_for(c, std::max<size_t>(n, H.dim + 1), 16), SFCI_BLOCK, 0, c->stream>>>(sk, n, H.dim, row_off,
+                                                                                                r1s);
+    acc_clear(c, 4);
+    int te = kt_begin(c, K_ENERGY, 16.0 * n);
+    k_energy<<<grid_for(c, n), SFCI_BLOCK, 0, c->stream>>>(H, k, v, n, row_off, r1s, sv, c->d_sdec, c->d_acc + 8,
+                                                           c->d_flags);
+    kt_end(c, te);
+    HIPCHK(hipGetLastError());
+    double e = acc_read(c, 4);
+    if (flags_read_clear(c) & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    kt_resolve(c);
+    return e;
 }
 
 void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, double lambda, double coeff, double* diag,

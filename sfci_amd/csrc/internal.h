@@ -88,6 +88,7 @@ enum KernelId : int {
     K_REDUCE,
     K_SELECT,
     K_VECOPS,
+    K_ENERGY,
     K_NUM
 };
 
@@ -197,6 +198,8 @@ struct HvAll {
 };
 void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, double lambda, double coeff, double* diag,
                 HvAll* all);
+// <v|H|v> with calc_energy's per-source pull (no candidate list); v key-sorted.
+double dev_energy(Ctx* c, const uint64_t* k, const double* v, size_t n);
 // cisd.hip: the CISD block eigenvector (state_idx, ascending) of ref_det into (ok, ov),
 // key order; returns the block size.
 size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_t* ok, double* ov,

@@ -302,3 +302,15 @@ def test_hv_uhf_bit_exact(tmp_path):
             assert bits_of(gd) == bits_of(od)
     finally:
         ctx.close()
+
+
+def test_energy_pull_matches_oracle_larger(sys1, orc1, ctx1):
+    # 20000 determinants: most connections miss v, many rows are empty
+    for n, seed in [(20000, 61), (1, 62)]:
+        k, v = random_input(sys1, n, seed)
+        vec = ctx1.vector(n, k, v)
+        e = ctx1.energy(vec)
+        oe = orc1.energy(k, v)
+        log("energy n=%d gpu %.15f oracle %.15f" % (n, e, oe))
+        assert bits_of(e) == bits_of(oe)
+    assert ctx1.energy(ctx1.vector(4)) == 0.0
