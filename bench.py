@@ -191,7 +191,7 @@ def main():
     b_hv = 16.0 * (n_in + m_out) + 2 * rec * m_c
     # sort + merge time per step (radix passes, their scans, and the exact reduce)
     t_sm = sum(x["ms"] for k, x in ks.items()
-               if k.startswith(("k_radix_", "scan", "k_bucket_", "k_reduce"))) / args.steps
+               if k.startswith(("k_radix_", "k_part_", "scan", "k_bucket_", "k_reduce"))) / args.steps
     res = dict(metric=METRIC, value=conn / (ms_step * 1e-3), unit="connections/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=ms_step, higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32/f64" if mixed else "f64", data="synthetic",

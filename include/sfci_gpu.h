@@ -100,7 +100,7 @@ int sfci_sync(sfci_ctx* ctx);
  * (no reference counterpart: the reference only prints wall-clock timers,
  * src/functions.cpp:800-801).  Kernel ids 0..SFCI_KERNEL_COUNT-1 are named by
  * sfci_kernel_name(); bytes are the algorithmic bytes each launch reads and writes. */
-#define SFCI_KERNEL_COUNT 13
+#define SFCI_KERNEL_COUNT 14
 typedef struct {
     double ms;
     uint64_t launches;

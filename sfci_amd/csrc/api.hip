@@ -222,7 +222,9 @@ void sfci_destroy(sfci_ctx* X) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     DevBuf* bufs[] = {&c.cand_k[0], &c.cand_k[1], &c.cand_v[0], &c.cand_v[1], &c.src_cnt, &c.src_off,
                       &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.lin_k[0], &c.lin_k[1], &c.lin_v[0],
-                      &c.lin_v[1], &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.lin_v[1], &c.part_meta[0], &c.part_meta[1], &c.part_start[0], &c.part_start[1],
+                      &c.part_len[0], &c.part_len[1], &c.part_hbase[0], &c.part_hbase[1], &c.part_hstride[0],
+                      &c.part_hstride[1], &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();
@@ -307,12 +309,12 @@ int sfci_kernel_stats(sfci_ctx* X, sfci_kernel_stat_t* out, int n, int reset) {
 
 const char* sfci_kernel_name(int id) {
     static const char* names[K_NUM] = {"k_generate",      "k_count",
-                                       "k_radix_hist<DigitHash>", "k_radix_scatter<DigitHash>",
+                                       "k_part_hist<DigitHash>", "k_part_scatter<DigitHash>",
                                        "k_radix_hist<DigitPlain>", "k_radix_scatter<DigitPlain>",
                                        "scan",            "k_bucket_bounds",
                                        "k_bucket_reduce", "k_reduce",
                                        "select",          "vecops",
-                                       "k_energy"};
+                                       "k_energy",        "k_part_scatter_cur<DigitHash>"};
     return (id >= 0 && id < K_NUM) ? names[id] : "";
 }
 

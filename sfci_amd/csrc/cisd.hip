@@ -43,13 +43,14 @@ __global__ void k_block_dense(HamDev H, const uint64_t* __restrict__ dets, size_
 }
 
 // records of source row i (at i * n_conn + e) whose target lies in the block -> A[i][p]
-__global__ void k_block_fill(const uint64_t* __restrict__ rk, const double* __restrict__ rv, uint64_t n_conn, size_t n,
+__global__ void k_block_fill(const Rec64* __restrict__ rec, uint64_t n_conn, size_t n,
                              const uint64_t* __restrict__ sorted_dense, const double* __restrict__ sorted_idx,
                              const double* __restrict__ dterm, double* __restrict__ A) {
     const size_t total = n * n_conn;
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
         const size_t i = r / n_conn;
-        const uint64_t key = rk[r];
+        const Rec64 x = rec[r];
+        const uint64_t key = x.key;
         if (key & 1) {
             A[i * n + i] = dterm[i];
             continue;
@@ -61,7 +62,7 @@ __global__ void k_block_fill(const uint64_t* __restrict__ rk, const double* __re
             if (sorted_dense[mid] < t) lo = mid + 1;
             else hi = mid;
         }
-        if (lo < n && sorted_dense[lo] == t) A[i * n + (size_t)sorted_idx[lo]] = rv[r];
+        if (lo < n && sorted_dense[lo] == t) A[i * n + (size_t)sorted_idx[lo]] = x.val;
     }
 }
 
@@ -114,15 +115,14 @@ size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_
     double* sv = di;
     radix_sort_pairs(c, &sk, &sv, dk2, di2, n, bits_for(H.dim * H.dim));
     // 3. every connection of every block determinant; entries inside the block fill A
-    const uint64_t* rec_k;
-    const double* rec_v;
+    const Rec64* recs;
     const double* dterm;
-    dev_connection_records(c, dets, n, &rec_k, &rec_v, &dterm);
+    dev_connection_records(c, dets, n, &recs, &dterm);
     DevBuf A, W;
     double* a = A.get<double>(n * n);
     double* evals = W.get<double>(2 * n + 1);
     HIPCHK(hipMemsetAsync(a, 0, n * n * sizeof(double), c->stream));
-    k_block_fill<<<grid1(n * H.n_conn), SFCI_BLOCK, 0, c->stream>>>(rec_k, rec_v, H.n_conn, n, sk, sv, dterm, a);
+    k_block_fill<<<grid1(n * H.n_conn), SFCI_BLOCK, 0, c->stream>>>(recs, H.n_conn, n, sk, sv, dterm, a);
     HIPCHK(hipGetLastError());
     // 4. all eigenpairs (ascending); column state_idx of the column-major result
     RocblasHandle hdl;

@@ -339,8 +339,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
                                                          double coeff, uint64_t row_lo, uint64_t row_hi,
                                                          const uint64_t* __restrict__ off,
-                                                         const uint16_t* __restrict__ sdec_g, uint64_t* __restrict__ ok,
-                                                         T* __restrict__ ov, double* __restrict__ dterm,
+                                                         const uint16_t* __restrict__ sdec_g,
+                                                         typename RecOf<T>::type* __restrict__ out,
+                                                         double* __restrict__ dterm,
                                                          uint32_t* flags) {
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
@@ -364,8 +365,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
             // A0
             if (tid == 0) {
-                ok[pos] = (rowbase + s.rank[1]) << 1 | 1ull;
-                ov[pos] = (T)0;
+                out[pos] = make_rec((rowbase + s.rank[1]) << 1 | 1ull, (T)0);
                 dterm[j] = s.diag_val;
             }
             pos += 1;
@@ -375,8 +375,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
                 T val = cand_val<T>(single_elem(H, s, 2 * jo + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                ok[pos + f] = (rowbase + s.srank[1][f]) << 1;
-                ov[pos + f] = val;
+                out[pos + f] = make_rec((rowbase + s.srank[1][f]) << 1, val);
             }
             pos += nsa;
             // A2
@@ -388,8 +387,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
                 T val = cand_val<T>(double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                ok[pos + f] = (rowbase + colex_rank(st, H.binom)) << 1;
-                ov[pos + f] = val;
+                out[pos + f] = make_rec((rowbase + colex_rank(st, H.binom)) << 1, val);
             }
             pos += ndaa;
         }
@@ -401,8 +399,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
             T val = cand_val<T>(single_elem(H, s, 2 * i, 2 * a), temp);
             range_bad |= !fx_in_range((double)val);
-            ok[pos + f] = (s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1;
-            ov[pos + f] = val;
+            out[pos + f] = make_rec((s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1, val);
         }
         pos += na;
         // B2
@@ -418,8 +415,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
                 T val = cand_val<T>(opposite_double(H, d, i, a, jo, b), temp);
                 range_bad |= !fx_in_range((double)val);
-                ok[pos + f] = (s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1;
-                ov[pos + f] = val;
+                out[pos + f] = make_rec((s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1, val);
             }
             pos += nB;
         }
@@ -447,10 +443,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 }
                 it.next();
                 if (all_in) {
-                    if (have) {
-                        ok[pos + f] = key;
-                        ov[pos + f] = val;
-                    }
+                    if (have) out[pos + f] = make_rec(key, val);
                     continue;
                 }
                 uint64_t bal = __ballot(have);
@@ -462,10 +455,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     if (i2 < w) pre += s_wcnt[i2];
                     tot += s_wcnt[i2];
                 }
-                if (have) {
-                    ok[pos + pre + my] = key;
-                    ov[pos + pre + my] = val;
-                }
+                if (have) out[pos + pre + my] = make_rec(key, val);
                 pos += tot;
                 __syncthreads();
             }
@@ -683,6 +673,15 @@ __global__ void k_row_index(const uint64_t* __restrict__ dk, size_t n, uint64_t 
         row_off[r] = (uint32_t)lo;
     }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) r1s[i] = (uint32_t)(dk[i] % dim);
+}
+
+template <typename R, typename T>
+__global__ void k_rec_split(const R* __restrict__ rec, size_t n, uint64_t* __restrict__ k, T* __restrict__ v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const R x = rec[i];
+        k[i] = rec_key(x);
+        v[i] = rec_val(x);
+    }
 }
 
 // Finish one merged target: for a target that is an input determinant (tag), add the
@@ -918,23 +917,26 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
 // than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
 constexpr int BR_THREADS = 512;
-constexpr int BR_SLOTS = 6144;                 // 6144 x 24 B = 144 KB of LDS per workgroup
+constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
+constexpr int BR_BATCH = 4;                    // records per thread loaded ahead
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
 // spans [lower_bound(b), lower_bound(b + 1)): one binary search per bound instead of a
 // pass over the keys.
-__device__ __forceinline__ size_t hash_lower_bound(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t b) {
+template <typename R>
+__device__ __forceinline__ size_t hash_lower_bound(const R* __restrict__ rec, size_t n, int hb, uint64_t b) {
     size_t lo = 0, hi = n;
     while (lo < hi) {
         size_t mid = (lo + hi) >> 1;
-        if (target_hash(ck[mid] >> 1, hb) < b) lo = mid + 1;
+        if (target_hash(rec_key(rec[mid]) >> 1, hb) < b) lo = mid + 1;
         else hi = mid;
     }
     return lo;
 }
 
-__global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int hb, uint64_t n_buckets,
+template <typename R>
+__global__ void k_bucket_bounds(const R* __restrict__ ck, size_t n, int hb, uint64_t n_buckets,
                                 uint64_t* __restrict__ bstart, uint64_t* __restrict__ bend) {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_buckets;
          b += (uint64_t)gridDim.x * blockDim.x) {
@@ -943,17 +945,41 @@ __global__ void k_bucket_bounds(const uint64_t* __restrict__ ck, size_t n, int h
     }
 }
 
+// Exact accumulation without a carry chain: a term x (|x| < 2^42) is split into three
+// signed limbs with x * 2^84 = l2 * 2^84 + l1 * 2^42 + l0 truncated toward zero (the same
+// integer fx_from gives), using exact FP64 steps: l2 = trunc(x), l1 = trunc(frac * 2^42),
+// l0 = trunc(rest * 2^42).  Every limb has the sign of x and |l1|, |l0| < 2^42.  Each limb
+// is added to its own 64-bit LDS word with an atomic whose result is unused.  A target
+// receives fewer than 2^21 terms (at most one per source), so no limb sum overflows; the
+// limbs are recombined exactly when the target is finished.
+__device__ __forceinline__ void limb_split(double x, unsigned long long& l0, unsigned long long& l1,
+                                           unsigned long long& l2) {
+    if (!(fabs(x) < 4398046511104.0)) x = 0.0;   // out of range: flagged where the term was made
+    const double t2 = trunc(x);
+    const double r1 = (x - t2) * 0x1p42;
+    const double t1 = trunc(r1);
+    const double t0 = trunc((r1 - t1) * 0x1p42);
+    l2 = (unsigned long long)(long long)t2;
+    l1 = (unsigned long long)(long long)t1;
+    l0 = (unsigned long long)(long long)t0;
+}
+
+__device__ __forceinline__ fx128 limb_join(unsigned long long l0, unsigned long long l1, unsigned long long l2) {
+    fx128 t{l0, (long long)l0 < 0 ? ~0ull : 0ull};
+    t = fx_add(t, fx128{l1 << 42, (unsigned long long)((long long)l1 >> 22)});
+    return fx_add(t, fx128{0, l2 << 20});
+}
+
 template <typename T>
 __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
-    HamDev H, const uint64_t* __restrict__ ck, const T* __restrict__ cv, const uint64_t* __restrict__ bstart,
+    HamDev H, const typename RecOf<T>::type* __restrict__ rec, const uint64_t* __restrict__ bstart,
     const uint64_t* __restrict__ bend, uint64_t n_buckets, int hb, uint64_t base_dense,
     const uint64_t* __restrict__ ik, const double* __restrict__ iv, const double* __restrict__ dterm, size_t n_in,
     double cut,
     uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
     unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags) {
     __shared__ unsigned long long t_key[BR_SLOTS];
-    __shared__ unsigned long long t_lo[BR_SLOTS];
-    __shared__ unsigned long long t_hi[BR_SLOTS];
+    __shared__ unsigned long long t_l0[BR_SLOTS], t_l1[BR_SLOTS], t_l2[BR_SLOTS];
     __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];
     __shared__ int s_full;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
@@ -967,32 +993,55 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
         if (st >= en) continue;
         for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
             t_key[i] = BR_EMPTY;
-            t_lo[i] = 0;
-            t_hi[i] = 0;
+            t_l0[i] = 0;
+            t_l1[i] = 0;
+            t_l2[i] = 0;
         }
         for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) t_tag[i] = 0;
         if (tid == 0) s_full = 0;
         __syncthreads();
-        for (uint64_t i = st + tid; i < en; i += BR_THREADS) {
-            uint64_t raw = ck[i];
-            unsigned long long key = raw >> 1;
-            uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
-            fx128 x = fx_from((double)cv[i]);
-            int probe = 0;
-            for (; probe < BR_SLOTS; ++probe) {
-                unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
-                if (prev == BR_EMPTY || prev == key) break;
-                slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
+        bool full_here = false;
+        // BR_BATCH records per thread are loaded before any is inserted, so their global
+        // loads are in flight together
+        for (uint64_t i0 = st; i0 < en; i0 += (uint64_t)BR_BATCH * BR_THREADS) {
+            uint64_t rk[BR_BATCH];
+            T rv[BR_BATCH];
+#pragma unroll
+            for (int r = 0; r < BR_BATCH; ++r) {
+                const uint64_t i = i0 + (uint64_t)r * BR_THREADS + tid;
+                rk[r] = ~0ull;
+                rv[r] = (T)0;
+                if (i < en) {
+                    const auto x = rec[i];
+                    rk[r] = rec_key(x);
+                    rv[r] = rec_val(x);
+                }
             }
-            if (probe == BR_SLOTS) {
-                s_full = 1;
-                continue;
+#pragma unroll
+            for (int r = 0; r < BR_BATCH; ++r) {
+                if (rk[r] == ~0ull) continue;
+                const uint64_t raw = rk[r];
+                const unsigned long long key = raw >> 1;
+                uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
+                int probe = 0;
+                for (; probe < BR_SLOTS; ++probe) {
+                    unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
+                    if (prev == BR_EMPTY || prev == key) break;
+                    slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
+                }
+                if (probe == BR_SLOTS) {
+                    full_here = true;
+                    continue;
+                }
+                unsigned long long l0, l1, l2;
+                limb_split((double)rv[r], l0, l1, l2);
+                atomicAdd(&t_l0[slot], l0);
+                atomicAdd(&t_l1[slot], l1);
+                atomicAdd(&t_l2[slot], l2);
+                if (raw & 1ull) atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
             }
-            unsigned long long old = atomicAdd(&t_lo[slot], (unsigned long long)x.lo);
-            unsigned long long carry = (old + x.lo) < old ? 1ull : 0ull;
-            atomicAdd(&t_hi[slot], (unsigned long long)(x.hi + carry));
-            if (raw & 1ull) atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
         }
+        if (full_here) s_full = 1;
         __syncthreads();
         const bool full = s_full != 0;
         __syncthreads();
@@ -1009,8 +1058,8 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             if (key != BR_EMPTY) {
                 ++uniq;
                 const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                y = finish_target(H, key + base_dense, fx128{t_lo[slot], t_hi[slot]}, tag, ik, iv, dterm, n_in, cut,
-                                  emit, det, dacc, bad_input);
+                y = finish_target(H, key + base_dense, limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik, iv,
+                                  dterm, n_in, cut, emit, det, dacc, bad_input);
             }
             uint64_t bal = __ballot(emit);
             if (bal) {
@@ -1163,26 +1212,26 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             off = offs;
         }
         if (n_pass == 0) continue;
-        uint64_t* k0 = c->cand_k[0].get<uint64_t>(n_pass);
-        T* v0 = c->cand_v[0].get<T>(n_pass);
-        uint64_t* k1 = c->cand_k[1].get<uint64_t>(n_pass);
-        T* v1 = c->cand_v[1].get<T>(n_pass);
+        using R = typename RecOf<T>::type;
+        // A: generation output; B: the (gapped) first-level partition, also room for the
+        // key/value split of the sort path
+        const size_t capB = part_capacity(n_pass);
+        R* A = (R*)c->cand_k[0].get<uint8_t>(n_pass * sizeof(R));
+        R* B = (R*)c->cand_k[1].get<uint8_t>(std::max(capB * sizeof(R), n_pass * (sizeof(uint64_t) + sizeof(T))));
         {
             Timer t(c->stream);
-            int tg = kt_begin(c, K_GENERATE, (8.0 + sizeof(T)) * n_pass + 16.0 * ns);
+            int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
             k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
-                                                                        row_hi, off, c->d_sdec, k0, v0, dterm,
+                                                                        row_hi, off, c->d_sdec, A, dterm,
                                                                         c->d_flags);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
             st.ms_generate += t.stop();
         }
         st.n_candidates += n_pass;
-        // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (1-3
-        // radix passes) merged in LDS tables; otherwise, or when a bucket overflows its
-        // table, a full sort on the dense index followed by the segmented reduction.
-        uint64_t *kd = k0, *kt = k1;
-        T *vd = v0, *vt = v1;
+        // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (two
+        // partition passes) merged in LDS tables; otherwise, or when a partition region or
+        // a bucket overflows, a full sort on the dense index and the segmented reduction.
         int hb = 0;
         if (c->merge_mode == 2) {
             hb = c->merge_hash_bits;
@@ -1194,45 +1243,58 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             int down = hb & ~7;
             if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS * (double)(1ull << down)) hb = down;
         }
-        bool hashed = hb > 0;
+        bool hashed = hb > 0 && hb <= 16;
+        if (hb > 16) st.n_fallback += 1;
+        R* rd = A;        // the records in their current order
+        R* other = B;     // the free buffer
         uint64_t* bstart = nullptr;
         uint64_t* bend = nullptr;
         if (hashed) {
             Timer t(c->stream);
-            radix_partition_hash<T>(c, &kd, &vd, kt, vt, n_pass, hb);
-            if (kd != k0) {
-                kt = k0;
-                vt = v0;
-            }
-            const size_t nb = (size_t)1 << hb;
-            bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
-            bend = bstart + nb;
-            int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
-            k_bucket_bounds<<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
-                kd, n_pass, hb, nb, bstart, bend);
-            kt_end(c, tb);
-            st.ms_sort += t.stop();
-        } else {
-            Timer t(c->stream);
-            int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
-            radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
-            if (kd != k0) {
-                kt = k0;
-                vt = v0;
+            R* g = hash_partition<R>(c, A, B, n_pass, hb);
+            if (!g) {
+                hashed = false;
+                st.n_fallback += 1;
+            } else {
+                rd = g;
+                other = (g == A) ? B : A;
+                const size_t nb = (size_t)1 << hb;
+                bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
+                bend = bstart + nb;
+                int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
+                k_bucket_bounds<R><<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
+                    rd, n_pass, hb, nb, bstart, bend);
+                kt_end(c, tb);
             }
             st.ms_sort += t.stop();
         }
+        // sort path: split the records into keys and values in the free buffer, sort them
+        // with the record buffer as scratch
+        uint64_t* kd = nullptr;
+        T* vd = nullptr;
+        auto sort_path = [&]() {
+            Timer ts(c->stream);
+            kd = (uint64_t*)other;
+            vd = (T*)(kd + n_pass);
+            uint64_t* kt = (uint64_t*)rd;
+            T* vt = (T*)(kt + n_pass);
+            k_rec_split<R, T><<<grid_for(c, (n_pass + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
+                rd, n_pass, kd, vd);
+            int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
+            radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
+            st.ms_sort += ts.stop();
+        };
+        if (!hashed) sort_path();
         for (int attempt = 0;; ++attempt) {
             Timer t(c->stream);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
-                int tr = kt_begin(c, K_BUCKET_REDUCE, (8.0 + sizeof(T)) * n_pass);
-                k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, kd, vd, bstart, bend, nb, hb,
-                                                                      row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk,
-                                                                      stv, d_out_count, stage_cap, d_uniq,
-                                                                      c->d_acc + 2, c->d_flags);
+                int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
+                k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, rd, bstart, bend, nb, hb, row_lo * H.dim, sk,
+                                                                      sv, dterm, ns, cut_eff, stk, stv, d_out_count,
+                                                                      stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
                 kt_end(c, tr);
             } else {
                 int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
@@ -1255,17 +1317,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                     c->unique_ratio = std::min(1.0, c->unique_ratio * 2.0);
                     uint64_t restore[2] = {before, uniq_host};
                     HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
-                    Timer ts(c->stream);
-                    int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
-                    uint64_t* k_in = kd;
-                    T* v_in = vd;
-                    radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
-                    if (kd != k_in) {
-                        kt = k_in;
-                        vt = v_in;
-                    }
-                    st.ms_sort += ts.stop();
                     hashed = false;
+                    sort_path();
                     if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
                     continue;
                 }
@@ -1357,22 +1410,19 @@ __global__ void k_fill(double* __restrict__ v, size_t n, double x) {
         v[i] = x;
 }
 
-void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const uint64_t** rk, const double** rv,
-                            const double** dterm) {
+void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64** rec, const double** dterm) {
     const HamDev& H = c->ham;
     const size_t n = ns * H.n_conn;
     double* ones = c->src_v.get<double>(ns);
-    uint64_t* k0 = c->cand_k[0].get<uint64_t>(n);
-    double* v0 = c->cand_v[0].get<double>(n);
+    Rec64* r0 = (Rec64*)c->cand_k[0].get<uint8_t>(n * sizeof(Rec64));
     double* dt = c->dterm.get<double>(ns);
     k_fill<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(ones, ns, 1.0);
     k_generate<double><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ones, ns, 0.0, 1.0, 0, H.dim, nullptr,
-                                                                      c->d_sdec, k0, v0, dt, c->d_flags);
+                                                                      c->d_sdec, r0, dt, c->d_flags);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (flags_read_clear(c) & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
-    *rk = k0;
-    *rv = v0;
+    *rec = r0;
     *dterm = dt;
 }
 

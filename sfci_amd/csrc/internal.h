@@ -49,12 +49,43 @@ struct DevBuf {
     }
 };
 
+// One candidate record: (pass-local dense target << 1 | is_diagonal, value), stored as
+// one 16-byte (FP64) or 12-byte (FP32) structure so that a run of records is one
+// contiguous span of memory.
+struct alignas(16) Rec64 {
+    uint64_t key;
+    double val;
+};
+struct alignas(4) Rec32 {
+    uint32_t klo, khi;
+    float val;
+};
+template <typename T>
+struct RecOf;
+template <>
+struct RecOf<double> {
+    using type = Rec64;
+};
+template <>
+struct RecOf<float> {
+    using type = Rec32;
+};
+__host__ __device__ __forceinline__ uint64_t rec_key(const Rec64& r) { return r.key; }
+__host__ __device__ __forceinline__ uint64_t rec_key(const Rec32& r) { return (uint64_t)r.khi << 32 | r.klo; }
+__host__ __device__ __forceinline__ double rec_val(const Rec64& r) { return r.val; }
+__host__ __device__ __forceinline__ float rec_val(const Rec32& r) { return r.val; }
+__host__ __device__ __forceinline__ Rec64 make_rec(uint64_t k, double v) { return Rec64{k, v}; }
+__host__ __device__ __forceinline__ Rec32 make_rec(uint64_t k, float v) {
+    return Rec32{(uint32_t)k, (uint32_t)(k >> 32), v};
+}
+
 // Device-side error flags (bit set by kernels).
 enum : uint32_t {
     FLAG_RANGE = 1u,        // a term left the fixed-point range (|x| >= 2^42)
     FLAG_OUT_OVERFLOW = 2u, // merged-output staging buffer too small (host grows and retries)
     FLAG_BAD_INPUT = 4u,    // an input determinant was not found in its own (unsorted) vector
     FLAG_BUCKET_FULL = 8u,  // a hash bucket had more distinct targets than its LDS table holds
+    FLAG_REGION_FULL = 16u, // a first-level partition region received more records than it holds
 };
 
 // Fixed per-Hamiltonian quantities passed by value to the kernels.
@@ -89,6 +120,7 @@ enum KernelId : int {
     K_SELECT,
     K_VECOPS,
     K_ENERGY,
+    K_PART_SCATTER1,   // k_part_scatter_cur<DigitHash>: first-level partition with region cursors
     K_NUM
 };
 
@@ -129,6 +161,7 @@ struct Ctx {
     DevBuf src_cnt, src_off, src_k, src_v, dterm;
     DevBuf stage_k, stage_v;         // merged H.v output before the top-cap select
     DevBuf lin_k[2], lin_v[2];       // uncut H.v results of the linear-solver products
+    DevBuf part_meta[2], part_start[2], part_len[2], part_hbase[2], part_hstride[2];   // partition chunks
     DevBuf tmp_k[2], tmp_v[2];
     DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
     DevBuf scratch_hist, scratch_off, scratch_small;
@@ -168,6 +201,13 @@ template <typename T>
 void radix_sort_kv(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int bits);
 template <typename T>
 void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int hb);
+// Group the n records of A by the hb-bit target hash (8 < hb <= 16: two passes, A -> B
+// -> A; hb <= 8: one pass, A -> B), ascending in the hash, order inside a group
+// unspecified.  B holds part_capacity(n) records.  Returns the grouped records, or
+// nullptr when a pass-1 region overflowed (A is then unchanged).
+size_t part_capacity(size_t n);
+template <typename R>
+R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb);
 
 // vecops.hip
 // Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.
@@ -207,8 +247,7 @@ size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_
 // Every connection of each of the ns sources with unit amplitude: record j * n_conn + e
 // holds ((dense target) << 1 | is_diagonal, H_target,source); the diagonal records carry
 // value 0 and dterm[j] = H_jj.  Buffers are the library's candidate buffers.
-void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const uint64_t** rk, const double** rv,
-                            const double** dterm);
+void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64** rec, const double** dterm);
 
 // kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
 // stream and folds the elapsed times into c->kclock.stat.

@@ -7,6 +7,8 @@
 //             with a 64-lane digit match built from 8 ballots, wave counts are
 //             combined in wave order through LDS, so items keep their input order.
 // Passes run back to back on the stream with no host synchronization.
+#include <vector>
+
 #include "internal.h"
 
 namespace sfci {
@@ -268,6 +270,300 @@ template <typename T>
 void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* vtmp, size_t n, int hb) {
     lsd_passes<T>(c, keys, vals, ktmp, vtmp, n, hb, [hb](int shift) { return DigitHash{shift, hb}; });
 }
+
+
+// ---- hash partition of candidate records ----------------------------------------------
+// Groups records by the hb-bit target hash (hb <= 16) in at most two passes.  Pass 1
+// splits the whole list by the top 8 hash bits into 256 regions of fixed capacity
+// (expected size + 2% + 64k): each tile ranks its records in LDS, reserves one run per
+// digit with an atomic on the region cursor and writes the runs out, so the pass needs
+// no histogram.  Pass 2 splits every region by the remaining bits into a dense list
+// (histogram, scan, scatter over region-aligned chunks).  Neither pass is stable: records
+// of one target are summed exactly, in any order.  A chunk is at most one tile of one
+// region; its counter for digit d sits at hbase + d * hstride, so one flat exclusive scan
+// gives every output offset of pass 2.
+constexpr int PART_TILE = 4096;
+constexpr int PART_THREADS = 512;
+constexpr int PART_ITEMS = PART_TILE / PART_THREADS;   // 8
+constexpr int HIST_CHUNKS = 16;                        // chunks per histogram workgroup
+
+size_t part_capacity(size_t n) {
+    size_t cap1 = (size_t)((double)n / RADIX * 1.02) + 65536;
+    return (size_t)RADIX * cap1;
+}
+
+struct Chunks {
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint64_t* hbase;
+    const uint32_t* hstride;
+    size_t n;
+};
+
+// chunk table of regions [rstart[r], rstart[r] + rsize[r]); cbase[r] = first chunk of
+// region r (n_regions + 1 entries)
+__global__ void k_make_chunks(const uint64_t* __restrict__ rstart, const uint64_t* __restrict__ rsize,
+                              const uint64_t* __restrict__ cbase, int n_regions, size_t n_chunks,
+                              uint64_t* __restrict__ start, uint32_t* __restrict__ len, uint64_t* __restrict__ hbase,
+                              uint32_t* __restrict__ hstride) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks; c += (size_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = n_regions;   // last r with cbase[r] <= c
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (cbase[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t cl = c - cbase[lo];
+        const uint64_t off = cl * PART_TILE;
+        start[c] = rstart[lo] + off;
+        len[c] = (uint32_t)min((uint64_t)PART_TILE, rsize[lo] - off);
+        hbase[c] = (uint64_t)RADIX * cbase[lo] + cl;
+        hstride[c] = (uint32_t)(cbase[lo + 1] - cbase[lo]);
+    }
+}
+
+template <class Digit, typename R>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist(const R* __restrict__ rin, Chunks ch, Digit dg,
+                                                          uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[HIST_CHUNKS][RADIX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < HIST_CHUNKS * RADIX; i += SFCI_BLOCK) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const size_t c0 = (size_t)blockIdx.x * HIST_CHUNKS;
+    for (int cc = 0; cc < HIST_CHUNKS; ++cc) {
+        const size_t c = c0 + cc;
+        if (c >= ch.n) break;
+        const uint64_t st = ch.start[c];
+        const uint32_t len = ch.len[c];
+        for (uint32_t i = tid; i < len; i += SFCI_BLOCK) atomicAdd(&h[cc][dg(rec_key(rin[st + i]))], 1u);
+    }
+    __syncthreads();
+    // consecutive threads write consecutive chunks of one digit
+    for (int e = tid; e < HIST_CHUNKS * RADIX; e += SFCI_BLOCK) {
+        const int d = e / HIST_CHUNKS, cc = e % HIST_CHUNKS;
+        const size_t c = c0 + cc;
+        if (c < ch.n) hist[ch.hbase[c] + (uint64_t)d * ch.hstride[c]] = h[cc][d];
+    }
+}
+
+// exclusive scan of one counter per digit held by threads 0..RADIX-1 (all threads call)
+__device__ __forceinline__ uint32_t digit_scan(uint32_t h, uint32_t* s_wsum) {
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
+    uint32_t x = h;
+    if (tid < RADIX) {
+        for (int o = 1; o < SFCI_WAVE; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, SFCI_WAVE);
+            if (lane >= o) x += y;
+        }
+        if (lane == SFCI_WAVE - 1) s_wsum[w] = x;
+    }
+    __syncthreads();
+    uint32_t pre = 0;
+    if (tid < RADIX)
+        for (int j = 0; j < w; ++j) pre += s_wsum[j];
+    return pre + x - h;
+}
+
+// pass 2: chunk c's records go to off[hbase + d * hstride] + (rank among the chunk's digit d)
+template <class Digit, typename R>
+__global__ __launch_bounds__(PART_THREADS) void k_part_scatter(const R* __restrict__ rin, R* __restrict__ rout,
+                                                                Chunks ch, Digit dg, const uint64_t* __restrict__ off,
+                                                                const uint32_t* __restrict__ hist) {
+    __shared__ R s_rec[PART_TILE];
+    __shared__ uint8_t s_dig[PART_TILE];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    const size_t c = blockIdx.x;
+    const uint64_t st = ch.start[c];
+    const uint32_t len = ch.len[c];
+    const uint64_t hb0 = ch.hbase[c];
+    const uint32_t hs = ch.hstride[c];
+    R x[PART_ITEMS];
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) x[r] = rin[st + li];
+    }
+    const uint32_t h = tid < RADIX ? hist[hb0 + (uint64_t)tid * hs] : 0u;
+    const uint32_t ls = digit_scan(h, s_wsum);
+    if (tid < RADIX) {
+        s_lstart[tid] = ls;
+        s_gbase[tid] = off[hb0 + (uint64_t)tid * hs];
+        s_cnt[tid] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) {
+            const uint32_t d = dg(rec_key(x[r]));
+            const uint32_t p = s_lstart[d] + atomicAdd(&s_cnt[d], 1u);
+            s_rec[p] = x[r];
+            s_dig[p] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += PART_THREADS) {
+        const uint32_t d = s_dig[p];
+        rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
+    }
+}
+
+// pass 1: tile t = records [t * PART_TILE, ...) of rin; digit d's run is reserved on
+// cursor[d] and written only when it ends at or before rend[d]
+template <class Digit, typename R>
+__global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cur(const R* __restrict__ rin, size_t n,
+                                                                    R* __restrict__ rout, Digit dg,
+                                                                    unsigned long long* __restrict__ cursor,
+                                                                    const uint64_t* __restrict__ rend,
+                                                                    uint32_t* __restrict__ flags) {
+    __shared__ R s_rec[PART_TILE];
+    __shared__ uint8_t s_dig[PART_TILE];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX], s_ok[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * PART_TILE;
+    const uint32_t len = (uint32_t)min((size_t)PART_TILE, n - base);
+    R x[PART_ITEMS];
+    uint32_t dig[PART_ITEMS], rank[PART_ITEMS];
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) x[r] = rin[base + li];
+    }
+    if (tid < RADIX) s_cnt[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        dig[r] = 0;
+        rank[r] = 0;
+        if (li < len) {
+            dig[r] = dg(rec_key(x[r]));
+            rank[r] = atomicAdd(&s_cnt[dig[r]], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t h = tid < RADIX ? s_cnt[tid] : 0u;
+    const uint32_t ls = digit_scan(h, s_wsum);
+    if (tid < RADIX) {
+        s_lstart[tid] = ls;
+        uint32_t ok = 1;
+        if (h) {
+            const unsigned long long g = atomicAdd(&cursor[tid], (unsigned long long)h);
+            s_gbase[tid] = g;
+            if (g + h > rend[tid]) {
+                ok = 0;
+                atomicOr(flags, FLAG_REGION_FULL);
+            }
+        }
+        s_ok[tid] = ok;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) {
+            const uint32_t p = s_lstart[dig[r]] + rank[r];
+            s_rec[p] = x[r];
+            s_dig[p] = (uint8_t)dig[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += PART_THREADS) {
+        const uint32_t d = s_dig[p];
+        if (s_ok[d]) rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
+    }
+}
+
+template <typename R, class Digit>
+static void part_pass(Ctx* c, const R* rin, R* rout, const Chunks& ch, Digit dg, size_t n) {
+    uint32_t* hist = c->scratch_hist.get<uint32_t>((size_t)RADIX * ch.n);
+    uint64_t* off = c->scratch_off.get<uint64_t>((size_t)RADIX * ch.n + 1);
+    int t0 = kt_begin(c, K_PART_HIST, (double)sizeof(R) * n + 4.0 * RADIX * ch.n);
+    k_part_hist<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(rin, ch, dg, hist);
+    kt_end(c, t0);
+    int t1 = kt_begin(c, K_SCAN, 12.0 * RADIX * ch.n);
+    exclusive_scan_u32(c, hist, (size_t)RADIX * ch.n, off, nullptr);
+    kt_end(c, t1);
+    int t2 = kt_begin(c, K_PART_SCATTER, 2.0 * sizeof(R) * n);
+    k_part_scatter<<<(unsigned)ch.n, PART_THREADS, 0, c->stream>>>(rin, rout, ch, dg, off, hist);
+    kt_end(c, t2);
+    HIPCHK(hipGetLastError());
+}
+
+// chunk table of the given regions in c->part_*[slot]
+static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart, const std::vector<uint64_t>& rsize) {
+    const int nr = (int)rstart.size();
+    std::vector<uint64_t> meta(3 * nr + 1);
+    uint64_t nch = 0;
+    for (int r = 0; r < nr; ++r) {
+        meta[r] = rstart[r];
+        meta[nr + r] = rsize[r];
+        meta[2 * nr + r] = nch;
+        nch += (rsize[r] + PART_TILE - 1) / PART_TILE;
+    }
+    meta[3 * nr] = nch;
+    uint64_t* dmeta = c->part_meta[slot].get<uint64_t>(meta.size());
+    HIPCHK(hipMemcpyAsync(dmeta, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, c->stream));
+    const size_t n_ch = std::max<size_t>(nch, 1);
+    uint64_t* st = c->part_start[slot].get<uint64_t>(n_ch);
+    uint32_t* ln = c->part_len[slot].get<uint32_t>(n_ch);
+    uint64_t* hb = c->part_hbase[slot].get<uint64_t>(n_ch);
+    uint32_t* hs = c->part_hstride[slot].get<uint32_t>(n_ch);
+    if (nch) {
+        unsigned g = (unsigned)std::min<uint64_t>((nch + SFCI_BLOCK - 1) / SFCI_BLOCK, 4096);
+        k_make_chunks<<<g, SFCI_BLOCK, 0, c->stream>>>(dmeta, dmeta + nr, dmeta + 2 * nr, nr, nch, st, ln, hb, hs);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));   // meta lives on the host stack
+    return Chunks{st, ln, hb, hs, (size_t)nch};
+}
+
+template <typename R>
+R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
+    if (hb <= 0 || hb > 16) throw HipError("hash_partition: 1..16 hash bits", -1);
+    if (hb <= 8) {
+        Chunks ch = make_chunks(c, 0, {0}, {(uint64_t)n});
+        part_pass<R>(c, A, B, ch, DigitHash{0, hb}, n);
+        return B;
+    }
+    // pass 1: top 8 hash bits, fixed-capacity regions of B
+    const size_t cap1 = part_capacity(n) / RADIX;
+    std::vector<uint64_t> cur(2 * RADIX);
+    for (int d = 0; d < RADIX; ++d) {
+        cur[d] = (uint64_t)d * cap1;
+        cur[RADIX + d] = (uint64_t)(d + 1) * cap1;
+    }
+    uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX);
+    HIPCHK(hipMemcpyAsync(dcur, cur.data(), 2 * RADIX * 8, hipMemcpyHostToDevice, c->stream));
+    const size_t n_tiles = (n + PART_TILE - 1) / PART_TILE;
+    int t1 = kt_begin(c, K_PART_SCATTER1, 2.0 * sizeof(R) * n);
+    k_part_scatter_cur<<<(unsigned)n_tiles, PART_THREADS, 0, c->stream>>>(
+        A, n, B, DigitHash{hb - 8, hb}, (unsigned long long*)dcur, dcur + RADIX, c->d_flags);
+    kt_end(c, t1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cur.data(), dcur, RADIX * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint32_t fl = flags_read_clear(c);
+    if (fl & ~(uint32_t)FLAG_REGION_FULL) atomic_flags_restore(c, fl & ~(uint32_t)FLAG_REGION_FULL);
+    if (fl & FLAG_REGION_FULL) return nullptr;
+    // pass 2: each region by the remaining hash bits (the pass-1 bits above them are
+    // constant inside a region), into a dense list in A
+    std::vector<uint64_t> rstart(RADIX), rsize(RADIX);
+    for (int d = 0; d < RADIX; ++d) {
+        rstart[d] = (uint64_t)d * cap1;
+        rsize[d] = cur[d] - rstart[d];
+    }
+    Chunks ch2 = make_chunks(c, 0, rstart, rsize);
+    part_pass<R>(c, B, A, ch2, DigitHash{0, hb}, n);
+    return A;
+}
+
+template Rec64* hash_partition<Rec64>(Ctx*, Rec64*, Rec64*, size_t, int);
+template Rec32* hash_partition<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 
 template void radix_sort_kv<double>(Ctx*, uint64_t**, double**, uint64_t*, double*, size_t, int);
 template void radix_sort_kv<float>(Ctx*, uint64_t**, float**, uint64_t*, float*, size_t, int);

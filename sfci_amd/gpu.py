@@ -232,7 +232,7 @@ class Context:
         return {f: getattr(s, f) for f, _ in StatsT._fields_}
 
     def kernel_stats(self, reset: bool = False) -> dict:
-        n = 13
+        n = 14
         arr = (KernelStatT * n)()
         self._check(lib().sfci_kernel_stats(self._h, arr, n, 1 if reset else 0))
         return {lib().sfci_kernel_name(i).decode(): dict(ms=arr[i].ms, launches=arr[i].launches,
