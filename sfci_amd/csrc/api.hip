@@ -120,6 +120,28 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         if (op.empty()) op.push_back(0);
         if (vp.empty()) vp.push_back(0);
         if (sd.empty()) sd.push_back(0);
+        // One block for all four spin cases when the file is spin-free (RHF): the opposite-
+        // spin blocks agree entry for entry, and the same-spin blocks agree with them on
+        // every entry the Slater-Condon rules read (same-spin (a i|b j) is only read with
+        // a != b and i != j; the converter leaves a == b or i == j out of those blocks).
+        bool one_block = true;
+        for (size_t t = 0; t < N4 && one_block; ++t) one_block = g[N4 + t] == g[2 * N4 + t];
+        for (size_t a = 0; a < (size_t)N && one_block; ++a)
+            for (size_t i = 0; i < (size_t)N && one_block; ++i)
+                for (size_t b = 0; b < (size_t)N && one_block; ++b)
+                    for (size_t j = 0; j < (size_t)N; ++j) {
+                        if (a == b || i == j) continue;
+                        size_t t = ((a * N + i) * N + b) * N + j;
+                        if (g[t] != g[N4 + t] || g[3 * N4 + t] != g[N4 + t]) {
+                            one_block = false;
+                            break;
+                        }
+                    }
+        if (one_block) {
+            std::vector<double> g1(g.begin() + N4, g.begin() + 2 * N4);
+            g.swap(g1);
+        }
+        c.g_one_block = one_block;
         HIPCHK(hipMalloc(&c.d_h, h.size() * 8));
         HIPCHK(hipMalloc(&c.d_g, g.size() * 8));
         HIPCHK(hipMalloc(&c.d_binom, c.h_binom.size() * 8));
@@ -145,6 +167,7 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         H.dim = binom_u64(N, k);
         H.h = c.d_h;
         H.g = c.d_g;
+        H.g_stride = one_block ? 0 : N4;
         H.binom = c.d_binom;
         H.occ_pairs = c.d_occ_pairs;
         H.vir_pairs = c.d_vir_pairs;
@@ -224,7 +247,7 @@ void sfci_destroy(sfci_ctx* X) {
                       &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.lin_k[0], &c.lin_k[1], &c.lin_v[0],
                       &c.lin_v[1], &c.part_meta[0], &c.part_meta[1], &c.part_start[0], &c.part_start[1],
                       &c.part_len[0], &c.part_len[1], &c.part_hbase[0], &c.part_hbase[1], &c.part_hstride[0],
-                      &c.part_hstride[1], &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.part_hstride[1], &c.part_region[0], &c.part_region[1], &c.part_base, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();

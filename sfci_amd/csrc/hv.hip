@@ -24,7 +24,7 @@ __device__ __forceinline__ double g_so(const HamDev& H, int m, int n, int p, int
     if (((m ^ n) & 1) | ((p ^ q) & 1)) return 0.0;
     size_t N = (size_t)H.n_orb;
     size_t blk = (size_t)(((m & 1) << 1) | (p & 1));
-    return H.g[blk * N * N * N * N + (((size_t)(m >> 1) * N + (n >> 1)) * N + (p >> 1)) * N + (q >> 1)];
+    return H.g[blk * H.g_stride + (((size_t)(m >> 1) * N + (n >> 1)) * N + (p >> 1)) * N + (q >> 1)];
 }
 
 __device__ __forceinline__ double h_so(const HamDev& H, int m, int n) {
@@ -188,12 +188,13 @@ __device__ __forceinline__ double opposite_double(const HamDev& H, uint64_t d, i
         uint64_t d2 = d - (one << q) + (one << x);
         sg = sign_between(d, q, x) * sign_between(d2, n, y);
     }
-    const uint32_t N = (uint32_t)H.n_orb, N2 = N * N, N4 = N2 * N2;
+    const uint32_t N = (uint32_t)H.n_orb, N2 = N * N;
+    const uint32_t gs = (uint32_t)H.g_stride;
     double g;
     if (n_is_0)   // block (0,1): (a i | b j)
-        g = H.g[1u * N4 + ((uint32_t)a * N + (uint32_t)i) * N2 + (uint32_t)b * N + (uint32_t)j];
+        g = H.g[1u * gs + ((uint32_t)a * N + (uint32_t)i) * N2 + (uint32_t)b * N + (uint32_t)j];
     else          // block (1,0): (b j | a i)
-        g = H.g[2u * N4 + ((uint32_t)b * N + (uint32_t)j) * N2 + (uint32_t)a * N + (uint32_t)i];
+        g = H.g[2u * gs + ((uint32_t)b * N + (uint32_t)j) * N2 + (uint32_t)a * N + (uint32_t)i];
     // the other term is an exact zero; adding it can only change the sign of a zero
     // value, which contributes nothing to any sum
     return sg * g;
@@ -916,8 +917,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // which one workgroup merges in an LDS open-addressing table with exact 128-bit
 // accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
 // than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
-constexpr int BR_THREADS = 512;
+constexpr int BR_THREADS = 768;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
+static_assert(BR_SLOTS % BR_THREADS == 0, "finish loop covers the table in whole rounds");
 constexpr int BR_BATCH = 4;                    // records per thread loaded ahead
 constexpr unsigned long long BR_EMPTY = ~0ull;
 

@@ -96,6 +96,7 @@ struct HamDev {
     uint64_t dim;          // C(n_orb, k): strings per spin
     const double* h;       // [2][n_orb][n_orb]  h[s][a][i] = h(2a+s, 2i+s)
     const double* g;       // [4][n_orb^4]       block 2*s1+s2: (a i | b j) with (a,i) spin s1, (b,j) spin s2
+    uint64_t g_stride;     // elements between blocks: n_orb^4, or 0 when one block serves all four
     const uint64_t* binom; // [65][33]
     const uint16_t* occ_pairs;  // C(k,2) entries (lo byte, hi byte) = positions in the occupied list
     const uint16_t* vir_pairs;  // C(v,2) entries = positions in the virtual list
@@ -109,8 +110,8 @@ struct HamDev {
 enum KernelId : int {
     K_GENERATE = 0,
     K_COUNT,
-    K_PART_HIST,       // k_radix_hist<DigitHash>: hash-partition histogram
-    K_PART_SCATTER,    // k_radix_scatter<DigitHash>
+    K_PART_HIST,       // k_part_hist*<DigitHash>: hash-partition histogram
+    K_PART_SCATTER,    // k_part_scatter*<DigitHash>: second-level partition
     K_SORT_HIST,       // k_radix_hist<DigitPlain>: key sort histogram
     K_SORT_SCATTER,    // k_radix_scatter<DigitPlain>
     K_SCAN,
@@ -161,7 +162,8 @@ struct Ctx {
     DevBuf src_cnt, src_off, src_k, src_v, dterm;
     DevBuf stage_k, stage_v;         // merged H.v output before the top-cap select
     DevBuf lin_k[2], lin_v[2];       // uncut H.v results of the linear-solver products
-    DevBuf part_meta[2], part_start[2], part_len[2], part_hbase[2], part_hstride[2];   // partition chunks
+    DevBuf part_meta[2], part_start[2], part_len[2], part_hbase[2], part_hstride[2], part_region[2];   // partition chunks
+    DevBuf part_base;
     DevBuf tmp_k[2], tmp_v[2];
     DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
     DevBuf scratch_hist, scratch_off, scratch_small;
@@ -169,6 +171,7 @@ struct Ctx {
     DevBuf rowhist;
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
+    bool g_one_block = false;        // the four two-electron spin blocks share one table
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
     int merge_hash_bits = 0;      // distinct targets per candidate, from the last H.v (sizes the hash buckets)

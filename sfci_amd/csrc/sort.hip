@@ -297,6 +297,7 @@ struct Chunks {
     const uint32_t* len;
     const uint64_t* hbase;
     const uint32_t* hstride;
+    const uint32_t* region;
     size_t n;
 };
 
@@ -305,7 +306,7 @@ struct Chunks {
 __global__ void k_make_chunks(const uint64_t* __restrict__ rstart, const uint64_t* __restrict__ rsize,
                               const uint64_t* __restrict__ cbase, int n_regions, size_t n_chunks,
                               uint64_t* __restrict__ start, uint32_t* __restrict__ len, uint64_t* __restrict__ hbase,
-                              uint32_t* __restrict__ hstride) {
+                              uint32_t* __restrict__ hstride, uint32_t* __restrict__ region) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks; c += (size_t)gridDim.x * blockDim.x) {
         int lo = 0, hi = n_regions;   // last r with cbase[r] <= c
         while (hi - lo > 1) {
@@ -319,6 +320,7 @@ __global__ void k_make_chunks(const uint64_t* __restrict__ rstart, const uint64_
         len[c] = (uint32_t)min((uint64_t)PART_TILE, rsize[lo] - off);
         hbase[c] = (uint64_t)RADIX * cbase[lo] + cl;
         hstride[c] = (uint32_t)(cbase[lo + 1] - cbase[lo]);
+        region[c] = (uint32_t)lo;
     }
 }
 
@@ -514,12 +516,149 @@ static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart,
     uint32_t* ln = c->part_len[slot].get<uint32_t>(n_ch);
     uint64_t* hb = c->part_hbase[slot].get<uint64_t>(n_ch);
     uint32_t* hs = c->part_hstride[slot].get<uint32_t>(n_ch);
+    uint32_t* rg = c->part_region[slot].get<uint32_t>(n_ch);
     if (nch) {
         unsigned g = (unsigned)std::min<uint64_t>((nch + SFCI_BLOCK - 1) / SFCI_BLOCK, 4096);
-        k_make_chunks<<<g, SFCI_BLOCK, 0, c->stream>>>(dmeta, dmeta + nr, dmeta + 2 * nr, nr, nch, st, ln, hb, hs);
+        k_make_chunks<<<g, SFCI_BLOCK, 0, c->stream>>>(dmeta, dmeta + nr, dmeta + 2 * nr, nr, nch, st, ln, hb, hs, rg);
     }
     HIPCHK(hipStreamSynchronize(c->stream));   // meta lives on the host stack
-    return Chunks{st, ln, hb, hs, (size_t)nch};
+    return Chunks{st, ln, hb, hs, rg, (size_t)nch};
+}
+
+
+// ---- second level, chunk-major counters ----------------------------------------------
+// cnt[c * RADIX + d]: records of chunk c with digit d.  One workgroup per region walks
+// its chunks in order, leaving in coff[c * RADIX + d] the records of digit d in earlier
+// chunks of the same region, and in base[r * RADIX + d] the dense output start of
+// (region r, digit d).  A chunk then finds its 256 counters and offsets contiguously.
+template <class Digit, typename R>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist_cm(const R* __restrict__ rin, Chunks ch, Digit dg,
+                                                             uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t h[HIST_CHUNKS][RADIX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < HIST_CHUNKS * RADIX; i += SFCI_BLOCK) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const size_t c0 = (size_t)blockIdx.x * HIST_CHUNKS;
+    for (int cc = 0; cc < HIST_CHUNKS; ++cc) {
+        const size_t c = c0 + cc;
+        if (c >= ch.n) break;
+        const uint64_t st = ch.start[c];
+        const uint32_t len = ch.len[c];
+        for (uint32_t i = tid; i < len; i += SFCI_BLOCK) atomicAdd(&h[cc][dg(rec_key(rin[st + i]))], 1u);
+    }
+    __syncthreads();
+    for (int e = tid; e < HIST_CHUNKS * RADIX; e += SFCI_BLOCK) {
+        const size_t c = c0 + e / RADIX;
+        if (c < ch.n) cnt[c * RADIX + (e % RADIX)] = (&h[0][0])[e];
+    }
+}
+
+constexpr int COLSCAN_UNROLL = 8;
+
+__global__ __launch_bounds__(RADIX) void k_region_colscan(const uint32_t* __restrict__ cnt,
+                                                          const uint64_t* __restrict__ cbase,
+                                                          const uint64_t* __restrict__ dense_start,
+                                                          uint32_t* __restrict__ coff, uint64_t* __restrict__ base) {
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int r = blockIdx.x, d = threadIdx.x;
+    const uint64_t c0 = cbase[r], c1 = cbase[r + 1];
+    uint32_t run = 0;
+    uint64_t cc = c0;
+    for (; cc + COLSCAN_UNROLL <= c1; cc += COLSCAN_UNROLL) {
+        uint32_t v[COLSCAN_UNROLL];
+#pragma unroll
+        for (int u = 0; u < COLSCAN_UNROLL; ++u) v[u] = cnt[(cc + u) * RADIX + d];
+#pragma unroll
+        for (int u = 0; u < COLSCAN_UNROLL; ++u) {
+            coff[(cc + u) * RADIX + d] = run;
+            run += v[u];
+        }
+    }
+    for (; cc < c1; ++cc) {
+        const uint32_t v = cnt[cc * RADIX + d];
+        coff[cc * RADIX + d] = run;
+        run += v;
+    }
+    const uint32_t pre = digit_scan(run, s_wsum);
+    base[(uint64_t)r * RADIX + d] = dense_start[r] + pre;
+}
+
+template <class Digit, typename R>
+__global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cm(const R* __restrict__ rin, R* __restrict__ rout,
+                                                                   Chunks ch, Digit dg, const uint32_t* __restrict__ cnt,
+                                                                   const uint32_t* __restrict__ coff,
+                                                                   const uint64_t* __restrict__ base) {
+    __shared__ R s_rec[PART_TILE];
+    __shared__ uint8_t s_dig[PART_TILE];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    const size_t c = blockIdx.x;
+    const uint64_t st = ch.start[c];
+    const uint32_t len = ch.len[c];
+    const uint32_t rg = ch.region[c];
+    R x[PART_ITEMS];
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) x[r] = rin[st + li];
+    }
+    const uint32_t h = tid < RADIX ? cnt[c * RADIX + tid] : 0u;
+    const uint32_t ls = digit_scan(h, s_wsum);
+    if (tid < RADIX) {
+        s_lstart[tid] = ls;
+        s_gbase[tid] = base[(uint64_t)rg * RADIX + tid] + coff[c * RADIX + tid];
+        s_cnt[tid] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS + tid;
+        if (li < len) {
+            const uint32_t d = dg(rec_key(x[r]));
+            const uint32_t p = s_lstart[d] + atomicAdd(&s_cnt[d], 1u);
+            s_rec[p] = x[r];
+            s_dig[p] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += PART_THREADS) {
+        const uint32_t d = s_dig[p];
+        rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
+    }
+}
+
+// regions given by (rstart, rsize) with dense output starts dense_start: rin -> rout
+template <typename R, class Digit>
+static void part_pass_regions(Ctx* c, const R* rin, R* rout, const std::vector<uint64_t>& rstart,
+                              const std::vector<uint64_t>& rsize, Digit dg, size_t n) {
+    const int nr = (int)rstart.size();
+    Chunks ch = make_chunks(c, 0, rstart, rsize);
+    // meta of slot 0 holds cbase at [2 nr, 3 nr]; dense starts go to slot 1
+    const uint64_t* cbase = (const uint64_t*)c->part_meta[0].p + 2 * nr;
+    std::vector<uint64_t> ds(nr);
+    uint64_t acc = 0;
+    for (int r = 0; r < nr; ++r) {
+        ds[r] = acc;
+        acc += rsize[r];
+    }
+    uint64_t* dds = c->part_meta[1].get<uint64_t>(nr);
+    HIPCHK(hipMemcpyAsync(dds, ds.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
+    uint32_t* cnt = c->scratch_hist.get<uint32_t>((size_t)RADIX * std::max<size_t>(ch.n, 1));
+    uint32_t* coff = (uint32_t*)c->scratch_off.get<uint64_t>((size_t)RADIX * std::max<size_t>(ch.n, 1) / 2 + 1);
+    uint64_t* base = c->part_base.get<uint64_t>((size_t)RADIX * nr);
+    int t0 = kt_begin(c, K_PART_HIST, (double)sizeof(R) * n + 4.0 * RADIX * ch.n);
+    k_part_hist_cm<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(rin, ch, dg, cnt);
+    kt_end(c, t0);
+    int t1 = kt_begin(c, K_SCAN, 8.0 * RADIX * ch.n);
+    k_region_colscan<<<nr, RADIX, 0, c->stream>>>(cnt, cbase, dds, coff, base);
+    kt_end(c, t1);
+    int t2 = kt_begin(c, K_PART_SCATTER, 2.0 * sizeof(R) * n);
+    k_part_scatter_cm<<<(unsigned)ch.n, PART_THREADS, 0, c->stream>>>(rin, rout, ch, dg, cnt, coff, base);
+    kt_end(c, t2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));   // ds lives on the host stack
 }
 
 template <typename R>
@@ -557,8 +696,7 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
         rstart[d] = (uint64_t)d * cap1;
         rsize[d] = cur[d] - rstart[d];
     }
-    Chunks ch2 = make_chunks(c, 0, rstart, rsize);
-    part_pass<R>(c, B, A, ch2, DigitHash{0, hb}, n);
+    part_pass_regions<R>(c, B, A, rstart, rsize, DigitHash{0, hb}, n);
     return A;
 }
 
