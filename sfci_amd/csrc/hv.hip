@@ -917,9 +917,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // which one workgroup merges in an LDS open-addressing table with exact 128-bit
 // accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
 // than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
-constexpr int BR_THREADS = 768;
+constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
-static_assert(BR_SLOTS % BR_THREADS == 0, "finish loop covers the table in whole rounds");
 constexpr int BR_BATCH = 4;                    // records per thread loaded ahead
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
@@ -1051,12 +1050,12 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             any_full = true;
             continue;
         }
-        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // BR_SLOTS is a multiple of BR_THREADS
+        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // every lane reaches the ballot
             const int slot = s0 + tid;
             bool emit = false;
             uint64_t det = 0;
             double y = 0.0;
-            const unsigned long long key = t_key[slot];
+            const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
             if (key != BR_EMPTY) {
                 ++uniq;
                 const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;

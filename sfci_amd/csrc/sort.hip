@@ -527,10 +527,9 @@ static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart,
 
 
 // ---- second level, chunk-major counters ----------------------------------------------
-// cnt[c * RADIX + d]: records of chunk c with digit d.  One workgroup per region walks
-// its chunks in order, leaving in coff[c * RADIX + d] the records of digit d in earlier
-// chunks of the same region, and in base[r * RADIX + d] the dense output start of
-// (region r, digit d).  A chunk then finds its 256 counters and offsets contiguously.
+// cnt[c * RADIX + d]: records of chunk c with digit d.  One workgroup per region sums its
+// chunks' counters and sets cursor[r * RADIX + d] to the dense output start of (region r,
+// digit d); each chunk then reserves its run of digit d with one atomic on that cursor.
 template <class Digit, typename R>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist_cm(const R* __restrict__ rin, Chunks ch, Digit dg,
                                                              uint32_t* __restrict__ cnt) {
@@ -558,7 +557,7 @@ constexpr int COLSCAN_UNROLL = 8;
 __global__ __launch_bounds__(RADIX) void k_region_colscan(const uint32_t* __restrict__ cnt,
                                                           const uint64_t* __restrict__ cbase,
                                                           const uint64_t* __restrict__ dense_start,
-                                                          uint32_t* __restrict__ coff, uint64_t* __restrict__ base) {
+                                                          unsigned long long* __restrict__ cursor) {
     __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
     const int r = blockIdx.x, d = threadIdx.x;
     const uint64_t c0 = cbase[r], c1 = cbase[r + 1];
@@ -569,25 +568,18 @@ __global__ __launch_bounds__(RADIX) void k_region_colscan(const uint32_t* __rest
 #pragma unroll
         for (int u = 0; u < COLSCAN_UNROLL; ++u) v[u] = cnt[(cc + u) * RADIX + d];
 #pragma unroll
-        for (int u = 0; u < COLSCAN_UNROLL; ++u) {
-            coff[(cc + u) * RADIX + d] = run;
-            run += v[u];
-        }
+        for (int u = 0; u < COLSCAN_UNROLL; ++u) run += v[u];
     }
-    for (; cc < c1; ++cc) {
-        const uint32_t v = cnt[cc * RADIX + d];
-        coff[cc * RADIX + d] = run;
-        run += v;
-    }
+    for (; cc < c1; ++cc) run += cnt[cc * RADIX + d];
     const uint32_t pre = digit_scan(run, s_wsum);
-    base[(uint64_t)r * RADIX + d] = dense_start[r] + pre;
+    cursor[(uint64_t)r * RADIX + d] = dense_start[r] + pre;
 }
 
+// a chunk reserves one run per digit on its region's (digit) cursor
 template <class Digit, typename R>
 __global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cm(const R* __restrict__ rin, R* __restrict__ rout,
                                                                    Chunks ch, Digit dg, const uint32_t* __restrict__ cnt,
-                                                                   const uint32_t* __restrict__ coff,
-                                                                   const uint64_t* __restrict__ base) {
+                                                                   unsigned long long* __restrict__ cursor) {
     __shared__ R s_rec[PART_TILE];
     __shared__ uint8_t s_dig[PART_TILE];
     __shared__ uint64_t s_gbase[RADIX];
@@ -608,7 +600,7 @@ __global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cm(const R* __res
     const uint32_t ls = digit_scan(h, s_wsum);
     if (tid < RADIX) {
         s_lstart[tid] = ls;
-        s_gbase[tid] = base[(uint64_t)rg * RADIX + tid] + coff[c * RADIX + tid];
+        s_gbase[tid] = h ? atomicAdd(&cursor[(uint64_t)rg * RADIX + tid], (unsigned long long)h) : 0ull;
         s_cnt[tid] = 0;
     }
     __syncthreads();
@@ -646,16 +638,15 @@ static void part_pass_regions(Ctx* c, const R* rin, R* rout, const std::vector<u
     uint64_t* dds = c->part_meta[1].get<uint64_t>(nr);
     HIPCHK(hipMemcpyAsync(dds, ds.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
     uint32_t* cnt = c->scratch_hist.get<uint32_t>((size_t)RADIX * std::max<size_t>(ch.n, 1));
-    uint32_t* coff = (uint32_t*)c->scratch_off.get<uint64_t>((size_t)RADIX * std::max<size_t>(ch.n, 1) / 2 + 1);
-    uint64_t* base = c->part_base.get<uint64_t>((size_t)RADIX * nr);
+    unsigned long long* cursor = (unsigned long long*)c->part_base.get<uint64_t>((size_t)RADIX * nr);
     int t0 = kt_begin(c, K_PART_HIST, (double)sizeof(R) * n + 4.0 * RADIX * ch.n);
     k_part_hist_cm<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(rin, ch, dg, cnt);
     kt_end(c, t0);
-    int t1 = kt_begin(c, K_SCAN, 8.0 * RADIX * ch.n);
-    k_region_colscan<<<nr, RADIX, 0, c->stream>>>(cnt, cbase, dds, coff, base);
+    int t1 = kt_begin(c, K_SCAN, 4.0 * RADIX * ch.n);
+    k_region_colscan<<<nr, RADIX, 0, c->stream>>>(cnt, cbase, dds, cursor);
     kt_end(c, t1);
     int t2 = kt_begin(c, K_PART_SCATTER, 2.0 * sizeof(R) * n);
-    k_part_scatter_cm<<<(unsigned)ch.n, PART_THREADS, 0, c->stream>>>(rin, rout, ch, dg, cnt, coff, base);
+    k_part_scatter_cm<<<(unsigned)ch.n, PART_THREADS, 0, c->stream>>>(rin, rout, ch, dg, cnt, cursor);
     kt_end(c, t2);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));   // ds lives on the host stack
