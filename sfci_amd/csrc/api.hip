@@ -331,7 +331,7 @@ int sfci_kernel_stats(sfci_ctx* X, sfci_kernel_stat_t* out, int n, int reset) {
 }
 
 const char* sfci_kernel_name(int id) {
-    static const char* names[K_NUM] = {"k_generate",      "k_count",
+    static const char* names[K_NUM] = {"k_generate",      "k_rowgroup_hist+k_count_passes",
                                        "k_part_hist<DigitHash>", "k_part_scatter<DigitHash>",
                                        "k_radix_hist<DigitPlain>", "k_radix_scatter<DigitPlain>",
                                        "scan",            "k_bucket_bounds",

@@ -217,15 +217,36 @@ struct DivIter {
 
 // ---------------------------------------------------------------------------------
 // Per-source count of connections whose target row lies in [row_lo, row_hi).
-__global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev Hg, const uint64_t* __restrict__ sk, size_t n_src,
-                                                      uint64_t row_lo, uint64_t row_hi, uint32_t* __restrict__ cnt) {
+// Per-source candidate counts of every pass at once: pass p covers target rows
+// [bounds[p], bounds[p + 1]); cnt[p * n_src + j] receives source j's count in pass p
+// (the same weights as k_rowgroup_hist: own row 1 + singles + doubles of spin 1, a
+// spin-0 single 1 + spin-1 singles, a spin-0 double 1).
+constexpr int MAX_PASSES = 256;
+
+__device__ __forceinline__ int pass_of(const uint64_t* bounds, int n_pass, uint64_t r) {
+    int lo = 0, hi = n_pass;   // last p with bounds[p] <= r
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (bounds[mid] <= r) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const uint64_t* __restrict__ sk, size_t n_src,
+                                                             const uint64_t* __restrict__ bounds_g, int n_pass,
+                                                             uint32_t* __restrict__ cnt) {
     __shared__ uint64_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
-    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
+    __shared__ uint64_t bounds[MAX_PASSES + 1];
+    __shared__ uint32_t s_cnt[MAX_PASSES];
     __shared__ uint32_t s_str;
     __shared__ uint8_t s_occ[SFCI_MAX_K], s_vir[SFCI_MAX_ORB];
-    __shared__ uint32_t s_tot;
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
+    for (int i = threadIdx.x; i <= n_pass; i += SFCI_BLOCK) bounds[i] = bounds_g[i];
     for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+        for (int i = threadIdx.x; i < n_pass; i += SFCI_BLOCK) s_cnt[i] = 0;
+        __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t st = spin_string(sk[j], 0);
             s_str = st;
@@ -234,12 +255,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev Hg, const uint64_t*
                 if (st >> i & 1u) s_occ[a++] = (uint8_t)i;
                 else s_vir[b++] = (uint8_t)i;
             }
-            uint64_t r = colex_rank(st, H.binom);
-            // diagonal + spin-1 singles + spin-1 doubles keep the row
-            s_tot = (r >= row_lo && r < row_hi) ? (uint32_t)(1 + H.n_sa + H.n_daa) : 0u;
+            atomicAdd(&s_cnt[pass_of(bounds, n_pass, colex_rank(st, H.binom))], (uint32_t)(1 + H.n_sa + H.n_daa));
         }
         __syncthreads();
-        uint32_t my = 0;
         const uint64_t n_alpha = H.n_sa + H.n_daa;
         for (uint64_t e = threadIdx.x; e < n_alpha; e += SFCI_BLOCK) {
             uint32_t st;
@@ -247,7 +265,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev Hg, const uint64_t*
             if (e < H.n_sa) {
                 int ii = (int)(e / H.v), aa = (int)(e - (uint64_t)ii * H.v);
                 st = s_str ^ (1u << s_occ[ii]) ^ (1u << s_vir[aa]);
-                w = (uint32_t)(1 + H.n_sa);   // the single + its opposite-spin doubles
+                w = (uint32_t)(1 + H.n_sa);
             } else {
                 uint64_t idx = e - H.n_sa;
                 uint64_t ij = idx / H.n_vpair, ab = idx - ij * H.n_vpair;
@@ -256,13 +274,10 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count(HamDev Hg, const uint64_t*
                      (1u << s_vir[vp >> 8]);
                 w = 1;
             }
-            uint64_t r = colex_rank(st, H.binom);
-            if (r >= row_lo && r < row_hi) my += w;
+            atomicAdd(&s_cnt[pass_of(bounds, n_pass, colex_rank(st, H.binom))], w);
         }
-        for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) my += __shfl_down(my, off, SFCI_WAVE);
-        if ((threadIdx.x & (SFCI_WAVE - 1)) == 0) atomicAdd(&s_tot, my);
         __syncthreads();
-        if (threadIdx.x == 0) cnt[j] = s_tot;
+        for (int p = threadIdx.x; p < n_pass; p += SFCI_BLOCK) cnt[(size_t)p * n_src + j] = s_cnt[p];
         __syncthreads();
     }
 }
@@ -311,6 +326,15 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
+// Bounds-checked record store (the planning count sizes the buffer; a store beyond it is
+// dropped and reported).
+template <typename R>
+__device__ __forceinline__ bool put_rec(R* __restrict__ out, uint64_t cap, uint64_t i, const R& r) {
+    if (i >= cap) return false;
+    out[i] = r;
+    return true;
+}
+
 // Candidate value: the Slater-Condon element times coeff * c_j, in the candidate
 // precision (double, or float products for the mixed-precision path).
 template <typename T>
@@ -334,7 +358,7 @@ __device__ __forceinline__ float cand_val<float>(double hij, double temp) {
 // Each part is its own loop, so a wavefront never mixes the O(n_occ) single-excitation
 // element with the O(1) double-excitation element.  Records are written at `pos`, which
 // starts at j * n_conn (single pass, off == nullptr) or off[j] (per-source exclusive scan
-// of k_count), so the layout is deterministic.
+// of the pass counts), so the layout is deterministic.
 template <typename T>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
@@ -342,7 +366,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                                                          const uint64_t* __restrict__ off,
                                                          const uint16_t* __restrict__ sdec_g,
                                                          typename RecOf<T>::type* __restrict__ out,
-                                                         double* __restrict__ dterm,
+                                                         uint64_t out_cap, double* __restrict__ dterm,
                                                          uint32_t* flags) {
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
@@ -356,7 +380,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
     const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair;
     for (uint32_t x = tid; x < nsa; x += SFCI_BLOCK) sdec[x] = sdec_g[x];
     __syncthreads();
-    bool range_bad = false;
+    bool range_bad = false, wbad = false;
     for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
@@ -366,7 +390,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
             // A0
             if (tid == 0) {
-                out[pos] = make_rec((rowbase + s.rank[1]) << 1 | 1ull, (T)0);
+                wbad |= !put_rec(out, out_cap, pos, make_rec((rowbase + s.rank[1]) << 1 | 1ull, (T)0));
                 dterm[j] = s.diag_val;
             }
             pos += 1;
@@ -376,7 +400,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
                 T val = cand_val<T>(single_elem(H, s, 2 * jo + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                out[pos + f] = make_rec((rowbase + s.srank[1][f]) << 1, val);
+                wbad |= !put_rec(out, out_cap, pos + f, make_rec((rowbase + s.srank[1][f]) << 1, val));
             }
             pos += nsa;
             // A2
@@ -388,7 +412,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
                 T val = cand_val<T>(double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                out[pos + f] = make_rec((rowbase + colex_rank(st, H.binom)) << 1, val);
+                wbad |= !put_rec(out, out_cap, pos + f, make_rec((rowbase + colex_rank(st, H.binom)) << 1, val));
             }
             pos += ndaa;
         }
@@ -400,7 +424,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
             T val = cand_val<T>(single_elem(H, s, 2 * i, 2 * a), temp);
             range_bad |= !fx_in_range((double)val);
-            out[pos + f] = make_rec((s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1, val);
+            wbad |= !put_rec(out, out_cap, pos + f, make_rec((s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1, val));
         }
         pos += na;
         // B2
@@ -416,7 +440,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
                 T val = cand_val<T>(opposite_double(H, d, i, a, jo, b), temp);
                 range_bad |= !fx_in_range((double)val);
-                out[pos + f] = make_rec((s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1, val);
+                wbad |= !put_rec(out, out_cap, pos + f,
+                                 make_rec((s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1, val));
             }
             pos += nB;
         }
@@ -444,7 +469,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 }
                 it.next();
                 if (all_in) {
-                    if (have) out[pos + f] = make_rec(key, val);
+                    if (have) wbad |= !put_rec(out, out_cap, pos + f, make_rec(key, val));
                     continue;
                 }
                 uint64_t bal = __ballot(have);
@@ -456,7 +481,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     if (i2 < w) pre += s_wcnt[i2];
                     tot += s_wcnt[i2];
                 }
-                if (have) out[pos + pre + my] = make_rec(key, val);
+                if (have) wbad |= !put_rec(out, out_cap, pos + pre + my, make_rec(key, val));
                 pos += tot;
                 __syncthreads();
             }
@@ -1146,16 +1171,20 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec), (size_t)1 << 20);
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
     std::vector<std::pair<uint64_t, uint64_t>> passes;
+    std::vector<uint64_t> pass_count;   // candidates of each pass
     if (total_cand <= pass_cap) {
         passes.push_back({0, n_rows});
+        pass_count.push_back(total_cand);
     } else {
         int grp_shift = 0;
         while ((n_rows >> grp_shift) >= 4096) ++grp_shift;
         int n_grp = (int)((n_rows - 1) >> grp_shift) + 1;
         unsigned long long* hist = c->rowhist.get<unsigned long long>(n_grp);
         HIPCHK(hipMemsetAsync(hist, 0, n_grp * sizeof(unsigned long long), c->stream));
+        int tr = kt_begin(c, K_ROWGROUP, 8.0 * ns);
         k_rowgroup_hist<<<grid_for(c, ns), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
             H, sk, ns, grp_shift, n_grp, hist);
+        kt_end(c, tr);
         std::vector<unsigned long long> hh(n_grp);
         HIPCHK(hipMemcpyAsync(hh.data(), hist, n_grp * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1166,14 +1195,31 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             if (acc + hh[g] > pass_cap) {
                 uint64_t hi = (uint64_t)g << grp_shift;
                 passes.push_back({lo, hi});
+                pass_count.push_back(acc);
                 lo = hi;
                 acc = 0;
             }
             acc += hh[g];
         }
         passes.push_back({lo, n_rows});
+        pass_count.push_back(acc);
     }
     st.n_passes = passes.size();
+    uint32_t* pcnt = nullptr;
+    if (passes.size() > 1) {
+        if (passes.size() > (size_t)MAX_PASSES) throw HipError("more target passes than supported; raise the workspace", -4);
+        std::vector<uint64_t> bounds(passes.size() + 1);
+        for (size_t p = 0; p < passes.size(); ++p) bounds[p] = passes[p].first;
+        bounds[passes.size()] = n_rows;
+        uint64_t* dbounds = c->part_meta[1].get<uint64_t>(bounds.size());
+        HIPCHK(hipMemcpyAsync(dbounds, bounds.data(), bounds.size() * 8, hipMemcpyHostToDevice, c->stream));
+        pcnt = c->src_cnt.get<uint32_t>(passes.size() * ns);
+        int tcn = kt_begin(c, K_ROWGROUP, 8.0 * ns + 4.0 * passes.size() * ns);
+        k_count_passes<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ns, dbounds, (int)passes.size(), pcnt);
+        kt_end(c, tcn);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));   // bounds live on the host stack
+    }
     // staging for merged output
     size_t stage_cap = std::max<size_t>(out_cap + (out_cap >> 2), (size_t)1 << 16);
     if (all) {
@@ -1192,25 +1238,20 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     unsigned long long* d_uniq = d_out_count + 1;
     HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
     uint64_t out_count_host = 0;
-    for (auto& ps : passes) {
-        const uint64_t row_lo = ps.first, row_hi = ps.second;
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+        const uint64_t row_lo = passes[pi].first, row_hi = passes[pi].second;
         if (row_lo >= row_hi) continue;
-        // 3a. count + offsets
+        // 3a. this pass's candidate count and per-source offsets
+        const uint64_t n_pass = pass_count[pi];
         const uint64_t* off = nullptr;
-        uint64_t n_pass;
-        if (passes.size() == 1) {
-            n_pass = total_cand;
-        } else {
-            uint32_t* cnt = c->src_cnt.get<uint32_t>(ns);
+        if (passes.size() > 1) {
             uint64_t* offs = c->src_off.get<uint64_t>(ns + 1);
-            int tc = kt_begin(c, K_COUNT, 12.0 * ns);
-            k_count<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ns, row_lo, row_hi, cnt);
-            kt_end(c, tc);
-            uint64_t* tot_dev = offs + ns;
-            exclusive_scan_u32(c, cnt, ns, offs, tot_dev);
-            HIPCHK(hipMemcpyAsync(&n_pass, tot_dev, 8, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
+            exclusive_scan_u32(c, pcnt + pi * ns, ns, offs, offs + ns);
             off = offs;
+            uint64_t tot = 0;
+            HIPCHK(hipMemcpyAsync(&tot, offs + ns, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (tot != n_pass) throw HipError("per-source pass counts disagree with the planning histogram", -1);
         }
         if (n_pass == 0) continue;
         using R = typename RecOf<T>::type;
@@ -1223,8 +1264,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             Timer t(c->stream);
             int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
             k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
-                                                                        row_hi, off, c->d_sdec, A, dterm,
-                                                                        c->d_flags);
+                                                                        row_hi, off, c->d_sdec, A, n_pass,
+                                                                        dterm, c->d_flags);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
             st.ms_generate += t.stop();
@@ -1362,6 +1403,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     *diag = fx_to(diag_total);
     uint32_t fl = flags_read_clear(c);
     if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    if (fl & FLAG_GEN_OVERFLOW) throw HipError("generation exceeded the planning count", -1);
     if (fl & FLAG_OUT_OVERFLOW) throw HipError("merged-output staging overflow", -3);
     if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
     st.n_unique = uniq;
@@ -1419,10 +1461,12 @@ void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64**
     double* dt = c->dterm.get<double>(ns);
     k_fill<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(ones, ns, 1.0);
     k_generate<double><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ones, ns, 0.0, 1.0, 0, H.dim, nullptr,
-                                                                      c->d_sdec, r0, dt, c->d_flags);
+                                                                      c->d_sdec, r0, n, dt, c->d_flags);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (flags_read_clear(c) & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    const uint32_t fl = flags_read_clear(c);
+    if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    if (fl & FLAG_GEN_OVERFLOW) throw HipError("generation exceeded its buffer", -1);
     *rec = r0;
     *dterm = dt;
 }

@@ -86,6 +86,7 @@ enum : uint32_t {
     FLAG_BAD_INPUT = 4u,    // an input determinant was not found in its own (unsorted) vector
     FLAG_BUCKET_FULL = 8u,  // a hash bucket had more distinct targets than its LDS table holds
     FLAG_REGION_FULL = 16u, // a first-level partition region received more records than it holds
+    FLAG_GEN_OVERFLOW = 32u, // generation produced more records than the planning count
 };
 
 // Fixed per-Hamiltonian quantities passed by value to the kernels.
@@ -109,7 +110,7 @@ struct HamDev {
 // Per-kernel timing with HIP events on the library stream (resolved after each call).
 enum KernelId : int {
     K_GENERATE = 0,
-    K_COUNT,
+    K_ROWGROUP,        // k_rowgroup_hist + k_count_passes: pass planning and per-source pass counts
     K_PART_HIST,       // k_part_hist*<DigitHash>: hash-partition histogram
     K_PART_SCATTER,    // k_part_scatter*<DigitHash>: second-level partition
     K_SORT_HIST,       // k_radix_hist<DigitPlain>: key sort histogram
