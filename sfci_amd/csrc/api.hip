@@ -137,6 +137,18 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
                             break;
                         }
                     }
+        // opposite-spin blocks with the spin-1 virtual index fastest: the layout the
+        // opposite-spin doubles read with consecutive lanes on consecutive spin-1 virtuals
+        const size_t n_gt = one_block ? 1 : 2;
+        std::vector<double> gt(n_gt * N4);
+        for (size_t a = 0; a < (size_t)N; ++a)
+            for (size_t i = 0; i < (size_t)N; ++i)
+                for (size_t j = 0; j < (size_t)N; ++j)
+                    for (size_t b = 0; b < (size_t)N; ++b) {
+                        const size_t t = ((a * N + i) * N + j) * N + b;
+                        gt[t] = g[N4 + ((a * N + i) * N + b) * N + j];                           // (a i|b j), (0,1)
+                        if (n_gt == 2) gt[N4 + t] = g[2 * N4 + ((b * N + j) * N + a) * N + i];   // (b j|a i), (1,0)
+                    }
         if (one_block) {
             std::vector<double> g1(g.begin() + N4, g.begin() + 2 * N4);
             g.swap(g1);
@@ -144,6 +156,8 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         c.g_one_block = one_block;
         HIPCHK(hipMalloc(&c.d_h, h.size() * 8));
         HIPCHK(hipMalloc(&c.d_g, g.size() * 8));
+        HIPCHK(hipMalloc(&c.d_gt, gt.size() * 8));
+        HIPCHK(hipMemcpy(c.d_gt, gt.data(), gt.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&c.d_binom, c.h_binom.size() * 8));
         HIPCHK(hipMalloc(&c.d_occ_pairs, op.size() * 2));
         HIPCHK(hipMalloc(&c.d_vir_pairs, vp.size() * 2));
@@ -168,6 +182,8 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         H.h = c.d_h;
         H.g = c.d_g;
         H.g_stride = one_block ? 0 : N4;
+        H.gt = c.d_gt;
+        H.gt_stride = n_gt == 2 ? N4 : 0;
         H.binom = c.d_binom;
         H.occ_pairs = c.d_occ_pairs;
         H.vir_pairs = c.d_vir_pairs;
@@ -252,7 +268,7 @@ void sfci_destroy(sfci_ctx* X) {
                       &c.scratch_off, &c.scratch_small, &c.rowhist};
     for (DevBuf* b : bufs) b->release();
     for (auto& b : c.scratch_scan) b.release();
-    void* ptrs[] = {c.d_h, c.d_g, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_sdec, c.d_acc, c.d_flags, c.d_counter};
+    void* ptrs[] = {c.d_h, c.d_g, c.d_gt, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_sdec, c.d_acc, c.d_flags, c.d_counter};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.stream) (void)hipStreamDestroy(c.stream);

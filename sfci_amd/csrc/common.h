@@ -68,6 +68,37 @@ __host__ __device__ __forceinline__ fx128 fx_from(double x) {
     return (b >> 63) ? fx_neg(r) : r;
 }
 
+// Limb form of the same fixed-point value for short exact sums: a term x with |x| < 2^42
+// contributes l2 = trunc(x), l1 = trunc(frac(x) * 2^42), l0 = trunc(rest * 2^42) (exact
+// FP64 steps), each with the sign of x and |l1|, |l0| < 2^42, so that l2 2^84 + l1 2^42 +
+// l0 is the integer fx_from(x) represents.  Up to 2^21 terms sum without overflow;
+// fxl_join gives the fx128 value.
+struct FxLimbs {
+    long long l0, l1, l2;
+};
+
+__host__ __device__ __forceinline__ FxLimbs fxl_zero() { return FxLimbs{0, 0, 0}; }
+
+// adds x; returns false (adding nothing) when |x| >= 2^42 or x is not finite
+__host__ __device__ __forceinline__ bool fxl_add(FxLimbs& a, double x) {
+    if (!(fabs(x) < 4398046511104.0)) return false;
+    const double t2 = trunc(x);
+    const double r1 = (x - t2) * 0x1p42;
+    const double t1 = trunc(r1);
+    const double t0 = trunc((r1 - t1) * 0x1p42);
+    a.l2 += (long long)t2;
+    a.l1 += (long long)t1;
+    a.l0 += (long long)t0;
+    return true;
+}
+
+__host__ __device__ __forceinline__ fx128 fxl_join(const FxLimbs& a) {
+    const uint64_t l0 = (uint64_t)a.l0, l1 = (uint64_t)a.l1, l2 = (uint64_t)a.l2;
+    fx128 t{l0, a.l0 < 0 ? ~0ull : 0ull};
+    t = fx_add(t, fx128{l1 << 42, (uint64_t)(a.l1 >> 22)});
+    return fx_add(t, fx128{0, l2 << 20});
+}
+
 __host__ __device__ __forceinline__ int fx_clz64(uint64_t x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __clzll((long long)x);

@@ -106,19 +106,22 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
         s.srank[sp][x] = colex_rank(st, H.binom);
     }
     // exact diagonal: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)]
-    fx128 acc = fx_zero();
+    FxLimbs lacc = fxl_zero();
+    bool dok = true;
     const int n = s.n_occ_so;
     for (int t = tid; t < n * n; t += SFCI_BLOCK) {
         int a = t / n, b = t - a * n;
         int m = s.occ_so[a];
         if (a == b) {
-            acc = fx_add(acc, fx_from(h_so(H, m, m)));
+            dok &= fxl_add(lacc, h_so(H, m, m));
         } else {
             int p = s.occ_so[b];
-            acc = fx_add(acc, fx_from(0.5 * g_so(H, m, m, p, p)));
-            acc = fx_add(acc, fx_neg(fx_from(0.5 * g_so(H, m, p, p, m))));
+            dok &= fxl_add(lacc, 0.5 * g_so(H, m, m, p, p));
+            dok &= fxl_add(lacc, -(0.5 * g_so(H, m, p, p, m)));
         }
     }
+    if (!dok) atomicOr(flags, FLAG_RANGE);
+    fx128 acc = fxl_join(lacc);
     for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) {
         fx128 o;
         o.lo = __shfl_down(acc.lo, off, SFCI_WAVE);
@@ -147,14 +150,17 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
 
 // <D'|H|D>, D' = D with spin orbital n -> m
 __device__ __forceinline__ double single_elem(const HamDev& H, const SrcLds& s, int n, int m) {
-    fx128 acc = fx_from(h_so(H, m, n));
+    FxLimbs acc = fxl_zero();
+    bool ok = fxl_add(acc, h_so(H, m, n));
     for (int a = 0; a < s.n_occ_so; ++a) {
         int p = s.occ_so[a];
         if (p == n) continue;
-        acc = fx_add(acc, fx_from(g_so(H, m, n, p, p)));
-        acc = fx_add(acc, fx_neg(fx_from(g_so(H, m, p, p, n))));
+        ok &= fxl_add(acc, g_so(H, m, n, p, p));
+        ok &= fxl_add(acc, -g_so(H, m, p, p, n));
     }
-    return sign_between(s.key, m, n) * fx_to(acc);
+    // an integral outside the fixed-point range yields a NaN, which the caller's range
+    // check reports
+    return ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
 }
 
 // <D''|H|D>, D'' = D - {n,q} + {x,y}, n < q, x < y (spin orbitals)
@@ -188,13 +194,10 @@ __device__ __forceinline__ double opposite_double(const HamDev& H, uint64_t d, i
         uint64_t d2 = d - (one << q) + (one << x);
         sg = sign_between(d, q, x) * sign_between(d2, n, y);
     }
-    const uint32_t N = (uint32_t)H.n_orb, N2 = N * N;
-    const uint32_t gs = (uint32_t)H.g_stride;
-    double g;
-    if (n_is_0)   // block (0,1): (a i | b j)
-        g = H.g[1u * gs + ((uint32_t)a * N + (uint32_t)i) * N2 + (uint32_t)b * N + (uint32_t)j];
-    else          // block (1,0): (b j | a i)
-        g = H.g[2u * gs + ((uint32_t)b * N + (uint32_t)j) * N2 + (uint32_t)a * N + (uint32_t)i];
+    const uint32_t N = (uint32_t)H.n_orb;
+    // block (0,1): (a i | b j); block (1,0): (b j | a i); both stored as [a][i][j][b]
+    const double g = H.gt[(n_is_0 ? 0u : (uint32_t)H.gt_stride) +
+                          (((uint32_t)a * N + (uint32_t)i) * N + (uint32_t)j) * N + (uint32_t)b];
     // the other term is an exact zero; adding it can only change the sign of a zero
     // value, which contributes nothing to any sum
     return sg * g;

@@ -98,6 +98,9 @@ struct HamDev {
     const double* h;       // [2][n_orb][n_orb]  h[s][a][i] = h(2a+s, 2i+s)
     const double* g;       // [4][n_orb^4]       block 2*s1+s2: (a i | b j) with (a,i) spin s1, (b,j) spin s2
     uint64_t g_stride;     // elements between blocks: n_orb^4, or 0 when one block serves all four
+    const double* gt;      // [2][n_orb^4] opposite-spin blocks as [a][i][j][b], b fastest: gt[0] holds
+                           // (a i | b j) of block (0,1), gt[gt_stride] holds (b j | a i) of block (1,0)
+    uint64_t gt_stride;
     const uint64_t* binom; // [65][33]
     const uint16_t* occ_pairs;  // C(k,2) entries (lo byte, hi byte) = positions in the occupied list
     const uint16_t* vir_pairs;  // C(v,2) entries = positions in the virtual list
@@ -151,6 +154,7 @@ struct Ctx {
     HamDev ham{};
     double* d_h = nullptr;
     double* d_g = nullptr;
+    double* d_gt = nullptr;
     uint64_t* d_binom = nullptr;
     uint16_t* d_occ_pairs = nullptr;
     uint16_t* d_vir_pairs = nullptr;
