@@ -30,6 +30,18 @@ import numpy as np  # noqa: E402
 METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
     os.path.join(ROOT, "BASELINE.json")) else "H.v connections/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (fetch + write) measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on the default workload in FP64, scaled by the counters' reading of a plain
+# copy of known size (summary in profiles/); other workloads and precisions report null
+PMC_TRAFFIC = {
+    ("c2", "fp64"): {
+        "k_generate": 1.008e11,
+        "k_part_scatter_cur<DigitHash>": 1.679e11,
+        "k_part_scatter<DigitHash>": 1.694e11,
+        "k_part_hist<DigitHash>": 8.18e10,
+        "k_bucket_reduce": 8.19e10,
+    },
+}
 
 
 def log(*a):
@@ -180,8 +192,9 @@ def main():
     dom = max(ks.items(), key=lambda kv: kv[1]["ms"])
     dom_name, d = dom
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    traffic = PMC_TRAFFIC.get((args.workload, args.precision), {}).get(dom_name) if args.nd is None else None
     roofline = dict(bound="hbm", kernel=dom_name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic,
                     avg_launch_ms=d["ms"] / max(1, d["launches"]),
                     bytes_per_launch=d["bytes"] / max(1, d["launches"]))
     # whole-H.v bytes of an ideal single-pass design: N_in and M_out at 16 bytes, the
