@@ -94,14 +94,15 @@ def time_to_e(ctx, w, nd, args):
         # the plain FP64 solve first: the mixed solve's energy is judged against it
         t = time.perf_counter()
         r64 = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap,
-                               e_prev_init=float("inf"))
+                               e_prev_init=float("inf"), verbose=args.verbose)
         fp64_ref = dict(seconds=time.perf_counter() - t, e_final=r64["e_final"], restarts=r64["restarts"])
         log("fp64 solve: %.2f s, E = %.12f" % (fp64_ref["seconds"], r64["e_final"]))
         ctx.set_precision(True)
     t = time.perf_counter()
     # the synthetic workloads have positive energies; starting the stopping test from
     # +inf (instead of the reference's 0.0) makes the solve run until two restarts agree
-    r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, e_prev_init=float("inf"))
+    r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, e_prev_init=float("inf"),
+                         verbose=args.verbose)
     dt = time.perf_counter() - t
     ctx.set_precision(False)
     res = dict(metric="time-to-E (truncated Lanczos until two restarts' Ritz values differ by < 1e-5 Ha)",
@@ -143,6 +144,7 @@ def main():
     ap.add_argument("--time-to-e", action="store_true",
                     help="run the whole ground-state solve instead (GPU, and the CPU oracle with --cpu)")
     ap.add_argument("--cpu", action="store_true", help="with --time-to-e: also time the CPU oracle's solve")
+    ap.add_argument("--verbose", action="store_true", help="with --time-to-e: print the solver's per-step lines")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="H.v candidate precision: fp64, or mixed (FP32 products, exact sums, FP64 vectors)")
     args = ap.parse_args()

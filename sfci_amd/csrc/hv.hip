@@ -75,26 +75,36 @@ struct SrcLds {
 __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, uint64_t key, double temp,
                             double lambda, uint64_t row_lo, uint64_t row_hi, uint32_t* flags) {
     __shared__ uint64_t s_dlo[SFCI_BLOCK / SFCI_WAVE], s_dhi[SFCI_BLOCK / SFCI_WAVE];
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        s.key = key;
-        s.temp = temp;
+    __shared__ uint32_t s_acnt[SFCI_BLOCK / SFCI_WAVE];
+    const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), wv = tid / SFCI_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (wv == 0) {
+        // one lane per (spin) orbital: occupied / virtual lists and colex ranks by ballots
+        const bool occp = lane < 2 * H.n_orb && ((key >> lane) & 1ull);
+        const uint64_t bo = __ballot(occp);
+        if (occp) s.occ_so[popc64(bo & lt)] = (uint8_t)lane;
         for (int sp = 0; sp < 2; ++sp) {
-            uint32_t st = spin_string(key, sp);
-            s.str[sp] = st;
-            s.rank[sp] = colex_rank(st, H.binom);
-            int a = 0, b = 0;
-            for (int i = 0; i < H.n_orb; ++i) {
-                if (st >> i & 1u) s.occ[sp][a++] = (uint8_t)i;
-                else s.vir[sp][b++] = (uint8_t)i;
+            const uint32_t st = spin_string(key, sp);
+            const bool in = lane < H.n_orb;
+            const bool oc = in && ((st >> lane) & 1u);
+            const uint64_t b_oc = __ballot(oc), b_vi = __ballot(in && !oc);
+            const uint32_t t = (uint32_t)popc64(b_oc & lt);
+            if (oc) s.occ[sp][t] = (uint8_t)lane;
+            else if (in) s.vir[sp][popc64(b_vi & lt)] = (uint8_t)lane;
+            uint64_t r = oc ? H.binom[lane * 33 + t + 1] : 0ull;   // C(pos, index + 1)
+            for (int o = SFCI_WAVE / 2; o > 0; o >>= 1) r += __shfl_down(r, o, SFCI_WAVE);
+            if (lane == 0) {
+                s.str[sp] = st;
+                s.rank[sp] = r;
             }
         }
-        int n = 0;
-        for (int p = 0; p < 2 * H.n_orb; ++p)
-            if (key >> p & 1ull) s.occ_so[n++] = (uint8_t)p;
-        s.n_occ_so = n;
-        s.row_in = (s.rank[0] >= row_lo && s.rank[0] < row_hi) ? 1 : 0;
-        s.n_alist = 0;
+        if (lane == 0) {
+            s.key = key;
+            s.temp = temp;
+            s.n_occ_so = (int)popc64(bo);
+            s.row_in = (s.rank[0] >= row_lo && s.rank[0] < row_hi) ? 1 : 0;
+            s.n_alist = 0;
+        }
     }
     __syncthreads();
     // single-excitation target ranks, both spins
@@ -139,11 +149,20 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
         double dv = (fx_to(t) - lambda) * temp;
         if (!fx_in_range(dv)) atomicOr(flags, FLAG_RANGE);
         s.diag_val = dv;
-        // spin-0 singles whose target row is in range (ascending order)
-        int m = 0;
-        for (int x = 0; x < nsa; ++x)
-            if (s.srank[0][x] >= row_lo && s.srank[0][x] < row_hi) s.alist[m++] = (uint16_t)x;
-        s.n_alist = m;
+    }
+    // spin-0 singles whose target row is in range, ascending (nsa <= SFCI_BLOCK)
+    {
+        const bool keep = tid < nsa && s.srank[0][tid] >= row_lo && s.srank[0][tid] < row_hi;
+        const uint64_t bk = __ballot(keep);
+        if (lane == 0) s_acnt[wv] = (uint32_t)popc64(bk);
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int i = 0; i < SFCI_BLOCK / SFCI_WAVE; ++i) {
+            if (i < wv) pre += s_acnt[i];
+            tot += s_acnt[i];
+        }
+        if (keep) s.alist[pre + popc64(bk & lt)] = (uint16_t)tid;
+        if (tid == 0) s.n_alist = (int)tot;
     }
     __syncthreads();
 }
@@ -288,7 +307,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const ui
 // Histogram of connection counts over target-row groups (row >> grp_shift), for pass
 // planning.  Each thread handles one source at a time.
 __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const uint64_t* __restrict__ sk,
-                                                              size_t n_src, int grp_shift, int n_grp,
+                                                              size_t n_src, uint64_t row_lo, uint64_t row_hi,
+                                                              int grp_shift, int n_grp,
                                                               unsigned long long* __restrict__ hist) {
     __shared__ uint64_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
@@ -304,8 +324,11 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
             if (st >> i & 1u) occ[a++] = (uint8_t)i;
             else vir[b++] = (uint8_t)i;
         }
-        if (threadIdx.x == 0)
-            atomicAdd(&s_h[colex_rank(st, H.binom) >> grp_shift], (unsigned long long)(1 + H.n_sa + H.n_daa));
+        if (threadIdx.x == 0) {
+            const uint64_t r = colex_rank(st, H.binom);
+            if (r >= row_lo && r < row_hi)
+                atomicAdd(&s_h[(r - row_lo) >> grp_shift], (unsigned long long)(1 + H.n_sa + H.n_daa));
+        }
         const uint64_t n_alpha = H.n_sa + H.n_daa;
         for (uint64_t e = threadIdx.x; e < n_alpha; e += SFCI_BLOCK) {
             uint32_t s2;
@@ -321,7 +344,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
                 s2 = st ^ (1u << occ[op & 0xff]) ^ (1u << occ[op >> 8]) ^ (1u << vir[vp & 0xff]) ^ (1u << vir[vp >> 8]);
                 w = 1;
             }
-            atomicAdd(&s_h[colex_rank(s2, H.binom) >> grp_shift], w);
+            const uint64_t r = colex_rank(s2, H.binom);
+            if (r >= row_lo && r < row_hi) atomicAdd(&s_h[(r - row_lo) >> grp_shift], w);
         }
     }
     __syncthreads();
@@ -1172,6 +1196,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     const uint64_t n_rows = H.dim;
     const size_t rec = sizeof(uint64_t) + sizeof(double);
     size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec), (size_t)1 << 20);
+    if (c->merge_mode == 0) {
+        // keep the distinct targets of a pass within 2^16 LDS-table buckets at a load of
+        // at most 0.8, using the distinct/candidate ratio of the previous call
+        const double ratio = std::min(1.0, c->unique_ratio * 1.25);
+        const double bucket_cap = 0.8 * BR_SLOTS * 65536.0 / ratio;
+        if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
+    }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
     std::vector<std::pair<uint64_t, uint64_t>> passes;
     std::vector<uint64_t> pass_count;   // candidates of each pass
@@ -1179,30 +1210,58 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         passes.push_back({0, n_rows});
         pass_count.push_back(total_cand);
     } else {
-        int grp_shift = 0;
-        while ((n_rows >> grp_shift) >= 4096) ++grp_shift;
-        int n_grp = (int)((n_rows - 1) >> grp_shift) + 1;
-        unsigned long long* hist = c->rowhist.get<unsigned long long>(n_grp);
-        HIPCHK(hipMemsetAsync(hist, 0, n_grp * sizeof(unsigned long long), c->stream));
-        int tr = kt_begin(c, K_ROWGROUP, 8.0 * ns);
-        k_rowgroup_hist<<<grid_for(c, ns), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
-            H, sk, ns, grp_shift, n_grp, hist);
-        kt_end(c, tr);
-        std::vector<unsigned long long> hh(n_grp);
-        HIPCHK(hipMemcpyAsync(hh.data(), hist, n_grp * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        // candidates per row segment: a histogram over at most 4096 row groups, refined
+        // inside any group that alone exceeds the pass capacity, down to single rows
+        struct Seg {
+            uint64_t lo, hi, count;
+        };
+        auto hist_of = [&](uint64_t lo, uint64_t hi) {
+            int grp_shift = 0;
+            while (((hi - lo - 1) >> grp_shift) >= 4096) ++grp_shift;
+            const int n_grp = (int)((hi - lo - 1) >> grp_shift) + 1;
+            unsigned long long* hist = c->rowhist.get<unsigned long long>(n_grp);
+            HIPCHK(hipMemsetAsync(hist, 0, n_grp * sizeof(unsigned long long), c->stream));
+            int tr = kt_begin(c, K_ROWGROUP, 8.0 * ns);
+            k_rowgroup_hist<<<grid_for(c, ns), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
+                H, sk, ns, lo, hi, grp_shift, n_grp, hist);
+            kt_end(c, tr);
+            std::vector<unsigned long long> hh(n_grp);
+            HIPCHK(hipMemcpyAsync(hh.data(), hist, n_grp * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            std::vector<Seg> out(n_grp);
+            for (int g = 0; g < n_grp; ++g)
+                out[g] = Seg{lo + ((uint64_t)g << grp_shift), std::min(hi, lo + ((uint64_t)(g + 1) << grp_shift)),
+                             hh[g]};
+            return out;
+        };
+        std::vector<Seg> segs = hist_of(0, n_rows);
+        for (int round = 0; round < 8; ++round) {
+            bool refined = false;
+            std::vector<Seg> next;
+            for (const Seg& sg : segs) {
+                if (sg.count > pass_cap && sg.hi - sg.lo > 1) {
+                    std::vector<Seg> sub = hist_of(sg.lo, sg.hi);
+                    next.insert(next.end(), sub.begin(), sub.end());
+                    refined = true;
+                } else {
+                    next.push_back(sg);
+                }
+            }
+            segs.swap(next);
+            if (!refined) break;
+        }
         uint64_t lo = 0, acc = 0;
-        for (int g = 0; g < n_grp; ++g) {
-            if (hh[g] > pass_cap)
-                throw HipError("one row group exceeds the pass capacity; raise the workspace limit", -4);
-            if (acc + hh[g] > pass_cap) {
-                uint64_t hi = (uint64_t)g << grp_shift;
-                passes.push_back({lo, hi});
+        for (const Seg& sg : segs) {
+            if (sg.count > pass_cap)
+                throw HipError("one target row exceeds the pass capacity; raise the workspace limit", -4);
+            if (acc + sg.count > pass_cap) {
+                passes.push_back({lo, sg.lo});
                 pass_count.push_back(acc);
-                lo = hi;
+                lo = sg.lo;
                 acc = 0;
             }
-            acc += hh[g];
+            acc += sg.count;
         }
         passes.push_back({lo, n_rows});
         pass_count.push_back(acc);

@@ -66,11 +66,17 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
     };
     int r = 0;
     for (r = 0; r < p->num_restart; ++r) {
-        if (p->verbose) printf("\n*****************block %d starts*****************\n", r);
+        if (p->verbose) {
+            printf("\n*****************block %d starts*****************\n", r);
+            fflush(stdout);
+        }
         CK(sfci_vec_copy(K[0], v));
         double alpha = 0.0, beta = 0.0;
         CK(hv(v, u, &alpha));
-        if (p->verbose) printf("H * phi_0 len:%lld\n", (long long)sfci_vec_len(u));
+        if (p->verbose) {
+            printf("H * phi_0 len:%lld\n", (long long)sfci_vec_len(u));
+            fflush(stdout);
+        }
         KM[0] = alpha;
         for (int i = 1; i < nl; ++i) {
             auto t_it = clk::now();
@@ -92,8 +98,11 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
             CK(sfci_vec_copy(u, temp_u));
             CK(sfci_vec_copy(v, v_next));
             CK(sfci_vec_copy(K[i], v));
-            if (p->verbose) printf("iter %d: len %lld alpha %.12f beta %.12f time %.3f ms\n", i,
-                                   (long long)sfci_vec_len(v), alpha, beta, ms_since(t_it));
+            if (p->verbose) {
+                printf("iter %d: len %lld alpha %.12f beta %.12f time %.3f ms\n", i, (long long)sfci_vec_len(v),
+                       alpha, beta, ms_since(t_it));
+                fflush(stdout);
+            }
         }
         for (int i = 0; i < nl; ++i)
             for (int j = i; j < nl; ++j) {
@@ -127,6 +136,7 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
             printf("energy Psi_res after truncation:%.12f\n", et);
             printf("gsaes energy:%.12f\n", ev[nearest]);
             printf("*****************block %d ends*****************\n", r);
+            fflush(stdout);
         }
         std::fill(KM.begin(), KM.end(), 0.0);
         for (int i = 0; i < nl; ++i) CK(sfci_vec_clear(K[i]));

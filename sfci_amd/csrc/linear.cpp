@@ -97,6 +97,7 @@ extern "C" int sfci_linear_solve(sfci_ctx* ctx, const sfci_vec* b, double target
             printf("length of final result: %lld\n", (long long)sfci_vec_len(result));
             printf("residue after truncation: %.12g\n", rn);
             printf("*************************restart %d ends*************************\n", res);
+            fflush(stdout);
         }
     }
     CK(sfci_vec_copy(x_out, x_m));

@@ -314,3 +314,51 @@ def test_energy_pull_matches_oracle_larger(sys1, orc1, ctx1):
         log("energy n=%d gpu %.15f oracle %.15f" % (n, e, oe))
         assert bits_of(e) == bits_of(oe)
     assert ctx1.energy(ctx1.vector(4)) == 0.0
+
+
+def test_hv_multipass_heavy_row_group(make_system):
+    # every source sits in the four lowest spin-0 rows (18 orbitals, 5 electrons per spin:
+    # 8568 rows, planned in groups of 4 rows): the first group alone exceeds the pass
+    # capacity, so planning refines it to single rows
+    from sfci_amd import gpu
+    s = make_system(18, 10, 0)
+    strings = []
+    for occ in itertools.combinations(range(18), 5):
+        v = 0
+        for i in occ:
+            v |= 1 << i
+        strings.append(v)
+
+    def colex(v):
+        r, t = 0, 1
+        pos = 0
+        from math import comb
+        while v:
+            if v & 1:
+                r += comb(pos, t)
+                t += 1
+            v >>= 1
+            pos += 1
+        return r
+
+    by_rank = sorted(strings, key=colex)
+    spread = lambda v: sum(1 << (2 * i) for i in range(18) if (v >> i) & 1)
+    rows = [spread(v) for v in by_rank[:4]]
+    cols = [spread(v) << 1 for v in by_rank[::2]]
+    keys = np.sort(np.array([a | b for a in rows for b in cols], dtype=np.uint64))
+    rng = np.random.default_rng(77)
+    vals = rng.normal(size=len(keys)) * np.exp(-rng.uniform(0, 6, len(keys)))
+    log("heavy row group: %d sources" % len(keys))
+    orc = O.Oracle(s["in_path"], s["n_so"], s["n_elec"])
+    ctx = gpu.Context(s["in_path"], s["n_so"], s["n_elec"])
+    try:
+        ctx.set_workspace(320 << 20)
+        gk, gv, gd = gpu_hv(ctx, keys, vals, len(keys), 0.0, 1.0, 60_000_000)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    log("heavy row group: %d passes" % st["n_passes"])
+    assert st["n_passes"] > 1
+    ok, ov, od, _ = orc.hv(keys, vals, nd=len(keys), lam=0.0, coeff=1.0, cut=1e-5)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits_of(gd) == bits_of(od)
