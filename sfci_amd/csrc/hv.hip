@@ -66,6 +66,14 @@ struct SrcLds {
     int n_occ_so;
     uint64_t srank[2][MAX_SINGLES];   // rank of the spin-s string after single excitation e
     uint16_t alist[MAX_SINGLES];      // spin-0 singles whose target row is in the pass range
+    // opposite-spin double (spin-0 i -> a with spin-1 j -> b) factors, packed:
+    //   bw[e]  spin-1 single e: j*N + b (bits 0-9), min(j,b) (10-14), max(j,b) (15-19),
+    //          j (20-24), parity of its single-excitation sign in the source (bit 25)
+    //   aw[t]  alist entry t: a*N + i (bits 0-9), i (10-14), a (15-19), sign parity (bit 20)
+    //   akey[t] pass-local dense index of the entry's target row, column 0
+    uint32_t bw[MAX_SINGLES];
+    uint32_t aw[MAX_SINGLES];
+    uint64_t akey[MAX_SINGLES];
     int n_alist;
     int row_in;                       // the source's own row is in the pass range
 };
@@ -112,10 +120,19 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
     for (int e = tid; e < 2 * nsa; e += SFCI_BLOCK) {
         int sp = e >= nsa ? 1 : 0, x = e - sp * nsa;
         uint16_t dd = sdec[x];
-        uint32_t st = s.str[sp] ^ (1u << s.occ[sp][dd & 0xff]) ^ (1u << s.vir[sp][dd >> 8]);
+        const int o = s.occ[sp][dd & 0xff], v = s.vir[sp][dd >> 8];
+        uint32_t st = s.str[sp] ^ (1u << o) ^ (1u << v);
         s.srank[sp][x] = colex_rank(st, H.binom);
+        if (sp == 1) {
+            const uint32_t lo = o < v ? o : v, hi = o < v ? v : o;
+            const uint32_t par = sign_between(key, 2 * o + 1, 2 * v + 1) < 0.0 ? 1u : 0u;
+            s.bw[x] = ((uint32_t)o * (uint32_t)H.n_orb + (uint32_t)v) | (lo << 10) | (hi << 15) |
+                      ((uint32_t)o << 20) | (par << 25);
+        }
     }
-    // exact diagonal: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)]
+    // exact diagonal: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)], needed
+    // only in the pass that holds the source's own row (s.row_in is block-uniform)
+    if (s.row_in) {
     FxLimbs lacc = fxl_zero();
     bool dok = true;
     const int n = s.n_occ_so;
@@ -150,6 +167,7 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
         if (!fx_in_range(dv)) atomicOr(flags, FLAG_RANGE);
         s.diag_val = dv;
     }
+    }
     // spin-0 singles whose target row is in range, ascending (nsa <= SFCI_BLOCK)
     {
         const bool keep = tid < nsa && s.srank[0][tid] >= row_lo && s.srank[0][tid] < row_hi;
@@ -161,7 +179,15 @@ __device__ void load_source(const HamDev& H, SrcLds& s, const uint16_t* sdec, ui
             if (i < wv) pre += s_acnt[i];
             tot += s_acnt[i];
         }
-        if (keep) s.alist[pre + popc64(bk & lt)] = (uint16_t)tid;
+        if (keep) {
+            const uint32_t t = pre + (uint32_t)popc64(bk & lt);
+            const uint16_t da = sdec[tid];
+            const uint32_t i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
+            const uint32_t par = sign_between(key, 2 * i, 2 * a) < 0.0 ? 1u : 0u;
+            s.alist[t] = (uint16_t)tid;
+            s.aw[t] = (a * (uint32_t)H.n_orb + i) | (i << 10) | (a << 15) | (par << 20);
+            s.akey[t] = s.srank[0][tid] * H.dim - row_lo * H.dim;
+        }
         if (tid == 0) s.n_alist = (int)tot;
     }
     __syncthreads();
@@ -375,6 +401,54 @@ __device__ __forceinline__ float cand_val<float>(double hij, double temp) {
     return (float)hij * (float)temp;
 }
 
+constexpr int B2_UNROLL = 4;
+
+// Single-excitation elements of one source, n_items of them, each evaluated by a group of
+// 2^lg lanes (2^lg >= occupied spin orbitals, groups never span a wavefront): lane l of a
+// group adds the Coulomb and exchange terms of occupied orbitals l, l + 2^lg, ... in limb
+// form, the limbs are combined exactly across the group, and the group's first lane adds
+// h_mn and writes the record.  Same value as single_elem (an exact sum is order-free).
+// item(t, n, m, key) gives item t's annihilated/created spin orbitals and record key.
+template <typename T, typename Item>
+__device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, uint32_t n_items, int lg, double temp,
+                                             uint64_t pos, Item item, typename RecOf<T>::type* __restrict__ out,
+                                             uint64_t out_cap, bool& range_bad, bool& wbad) {
+    const int L = 1 << lg;
+    const int gl = threadIdx.x & (L - 1);
+    const uint32_t grp = threadIdx.x >> lg, n_grp = SFCI_BLOCK >> lg;
+    const int no = s.n_occ_so;
+    for (uint32_t t0 = 0; t0 < n_items; t0 += n_grp) {   // block-uniform trip count
+        const uint32_t t = t0 + grp;
+        const bool act = t < n_items;
+        int n = 0, m = 0;
+        uint64_t key = 0;
+        if (act) item(t, n, m, key);
+        FxLimbs acc = fxl_zero();
+        bool ok = true;
+        if (act) {
+            for (int a = gl; a < no; a += L) {
+                const int p = s.occ_so[a];
+                if (p == n) continue;
+                ok &= fxl_add(acc, g_so(H, m, n, p, p));
+                ok &= fxl_add(acc, -g_so(H, m, p, p, n));
+            }
+        }
+        for (int o = L >> 1; o > 0; o >>= 1) {   // at most 64 terms per limb: no overflow
+            acc.l0 += __shfl_xor(acc.l0, o, SFCI_WAVE);
+            acc.l1 += __shfl_xor(acc.l1, o, SFCI_WAVE);
+            acc.l2 += __shfl_xor(acc.l2, o, SFCI_WAVE);
+            ok = (__shfl_xor(ok ? 1 : 0, o, SFCI_WAVE) != 0) && ok;
+        }
+        if (act && gl == 0) {
+            ok &= fxl_add(acc, h_so(H, m, n));
+            const double hij = ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
+            const T val = cand_val<T>(hij, temp);
+            range_bad |= !fx_in_range((double)val);
+            wbad |= !put_rec(out, out_cap, pos + t, make_rec(key, val));
+        }
+    }
+}
+
 // Candidate generation for the target rows [row_lo, row_hi): one workgroup per source.
 //   A0  the diagonal: a tag-only record (value 0); its FP64 term (E_jj - lambda) * temp
 //       goes to dterm[j] and is added when the target is merged
@@ -412,6 +486,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
         const double temp = s.temp;
+        const int lg_occ = s.n_occ_so <= 1 ? 0 : min(6, 32 - (int)__clz(s.n_occ_so - 1));
         uint64_t pos = off ? off[j] : (uint64_t)j * H.n_conn;
         if (s.row_in) {
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
@@ -422,13 +497,14 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             }
             pos += 1;
             // A1
-            for (uint32_t f = tid; f < nsa; f += SFCI_BLOCK) {
-                uint16_t dd = sdec[f];
-                int jo = s.occ[1][dd & 0xff], b = s.vir[1][dd >> 8];
-                T val = cand_val<T>(single_elem(H, s, 2 * jo + 1, 2 * b + 1), temp);
-                range_bad |= !fx_in_range((double)val);
-                wbad |= !put_rec(out, out_cap, pos + f, make_rec((rowbase + s.srank[1][f]) << 1, val));
-            }
+            emit_singles<T>(H, s, nsa, lg_occ, temp, pos,
+                            [&](uint32_t f, int& n, int& m, uint64_t& key) {
+                                const uint16_t dd = sdec[f];
+                                n = 2 * s.occ[1][dd & 0xff] + 1;
+                                m = 2 * s.vir[1][dd >> 8] + 1;
+                                key = (rowbase + s.srank[1][f]) << 1;
+                            },
+                            out, out_cap, range_bad, wbad);
             pos += nsa;
             // A2
             DivIter it(tid, SFCI_BLOCK, nvp);
@@ -445,30 +521,54 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
         }
         // B1
         const uint32_t na = (uint32_t)s.n_alist;
-        for (uint32_t f = tid; f < na; f += SFCI_BLOCK) {
-            uint32_t sa = s.alist[f];
-            uint16_t da = sdec[sa];
-            int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
-            T val = cand_val<T>(single_elem(H, s, 2 * i, 2 * a), temp);
-            range_bad |= !fx_in_range((double)val);
-            wbad |= !put_rec(out, out_cap, pos + f, make_rec((s.srank[0][sa] * H.dim - base_dense + s.rank[1]) << 1, val));
-        }
+        emit_singles<T>(H, s, na, lg_occ, temp, pos,
+                        [&](uint32_t f, int& n, int& m, uint64_t& key) {
+                            const uint32_t aw = s.aw[f];
+                            n = 2 * (int)((aw >> 10) & 31u);
+                            m = 2 * (int)((aw >> 15) & 31u);
+                            key = (s.akey[f] + s.rank[1]) << 1;
+                        },
+                        out, out_cap, range_bad, wbad);
         pos += na;
-        // B2
+        // B2: <D''|H|D> = (-1)^p (a i|b j), p = parity of the spin-0 single's sign, of the
+        // spin-1 single's sign, and of whether 2i and 2a lie between 2j+1 and 2b+1 (moving
+        // the spin-0 electron first changes the occupation between the spin-1 pair by
+        // exactly those two positions); the same value as opposite_double.  B2_UNROLL
+        // integral loads are issued before any is used.
         {
             const uint32_t nB = na * nsa;
+            const uint32_t N2 = (uint32_t)H.n_orb * (uint32_t)H.n_orb, gts = (uint32_t)H.gt_stride;
             DivIter it(tid, SFCI_BLOCK, nsa ? nsa : 1);
-            for (uint32_t f = tid; f < nB; f += SFCI_BLOCK, it.next()) {
-                uint32_t sa = s.alist[it.q];
-                uint16_t da = sdec[sa];
-                int i = s.occ[0][da & 0xff], a = s.vir[0][da >> 8];
-                uint32_t sb = it.r;
-                uint16_t db = sdec[sb];
-                int jo = s.occ[1][db & 0xff], b = s.vir[1][db >> 8];
-                T val = cand_val<T>(opposite_double(H, d, i, a, jo, b), temp);
-                range_bad |= !fx_in_range((double)val);
-                wbad |= !put_rec(out, out_cap, pos + f,
-                                 make_rec((s.srank[0][sa] * H.dim - base_dense + s.srank[1][sb]) << 1, val));
+            for (uint32_t f0 = tid; f0 < nB; f0 += B2_UNROLL * SFCI_BLOCK) {
+                double g[B2_UNROLL];
+                uint32_t qa[B2_UNROLL], qb[B2_UNROLL];
+                bool neg[B2_UNROLL];
+#pragma unroll
+                for (int u = 0; u < B2_UNROLL; ++u) {
+                    qa[u] = it.q;
+                    qb[u] = it.r;
+                    it.next();
+                    g[u] = 0.0;
+                    neg[u] = false;
+                    if (f0 + (uint32_t)u * SFCI_BLOCK < nB) {
+                        const uint32_t aw = s.aw[qa[u]], bw = s.bw[qb[u]];
+                        const uint32_t i = (aw >> 10) & 31u, a = (aw >> 15) & 31u;
+                        const uint32_t lo = (bw >> 10) & 31u, hi = (bw >> 15) & 31u, j = (bw >> 20) & 31u;
+                        const uint32_t p = ((aw >> 20) ^ (bw >> 25) ^ (uint32_t)(i > lo && i <= hi) ^
+                                            (uint32_t)(a > lo && a <= hi)) & 1u;
+                        neg[u] = p != 0;
+                        g[u] = H.gt[(i <= j ? 0u : gts) + (aw & 1023u) * N2 + (bw & 1023u)];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < B2_UNROLL; ++u) {
+                    const uint32_t f = f0 + (uint32_t)u * SFCI_BLOCK;
+                    if (f < nB) {
+                        T val = cand_val<T>(neg[u] ? -g[u] : g[u], temp);
+                        range_bad |= !fx_in_range((double)val);
+                        wbad |= !put_rec(out, out_cap, pos + f, make_rec((s.akey[qa[u]] + s.srank[1][qb[u]]) << 1, val));
+                    }
+                }
             }
             pos += nB;
         }
@@ -486,9 +586,18 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
                     int a = s.vir[0][vp & 0xff], b = s.vir[0][vp >> 8];
                     uint32_t st = s.str[0] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
-                    uint64_t r0 = colex_rank(st, H.binom);
-                    have = all_in || (r0 >= row_lo && r0 < row_hi);
+                    // the ranks of k-strings whose highest orbital is m fill [C(m,k), C(m+1,k)):
+                    // targets whose class misses the pass range skip the full rank
+                    const int top = max(31 - (int)__clz(st), 0);
+                    const uint64_t cls_lo = H.binom[top * 33 + H.k];
+                    const uint64_t cls_hi = top + 1 < H.n_orb ? H.binom[(top + 1) * 33 + H.k] : H.dim;
+                    if (all_in || (cls_hi > row_lo && cls_lo < row_hi)) {
+                        const uint64_t r0 = colex_rank(st, H.binom);
+                        have = all_in || (r0 >= row_lo && r0 < row_hi);
+                        key = r0;
+                    }
                     if (have) {
+                        const uint64_t r0 = key;
                         key = (r0 * H.dim - base_dense + s.rank[1]) << 1;
                         val = cand_val<T>(double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b), temp);
                         range_bad |= !fx_in_range((double)val);
@@ -971,7 +1080,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
 constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
-constexpr int BR_BATCH = 4;                    // records per thread loaded ahead
+constexpr int BR_BATCH = 8;                    // records per thread loaded together
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
@@ -1054,45 +1163,74 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
         if (tid == 0) s_full = 0;
         __syncthreads();
         bool full_here = false;
-        // BR_BATCH records per thread are loaded before any is inserted, so their global
-        // loads are in flight together
-        for (uint64_t i0 = st; i0 < en; i0 += (uint64_t)BR_BATCH * BR_THREADS) {
-            uint64_t rk[BR_BATCH];
-            T rv[BR_BATCH];
+        // BR_BATCH records per thread per round, all loaded before any is inserted; full
+        // rounds load without bounds tests, so that their loads are in flight together
+        constexpr uint64_t STEP = (uint64_t)BR_BATCH * BR_THREADS;
+        auto load_full = [&](uint64_t i0, uint64_t* k_, T* v_) {
+#pragma unroll
+            for (int r = 0; r < BR_BATCH; ++r) {
+                const auto x = rec[i0 + (uint64_t)r * BR_THREADS + tid];
+                k_[r] = rec_key(x);
+                v_[r] = rec_val(x);
+            }
+        };
+        auto load_round = [&](uint64_t i0, uint64_t* k_, T* v_) {
 #pragma unroll
             for (int r = 0; r < BR_BATCH; ++r) {
                 const uint64_t i = i0 + (uint64_t)r * BR_THREADS + tid;
-                rk[r] = ~0ull;
-                rv[r] = (T)0;
+                k_[r] = ~0ull;
+                v_[r] = (T)0;
                 if (i < en) {
                     const auto x = rec[i];
-                    rk[r] = rec_key(x);
-                    rv[r] = rec_val(x);
+                    k_[r] = rec_key(x);
+                    v_[r] = rec_val(x);
                 }
             }
+        };
+        auto insert_round = [&](const uint64_t* k_, const T* v_) {
 #pragma unroll
             for (int r = 0; r < BR_BATCH; ++r) {
-                if (rk[r] == ~0ull) continue;
-                const uint64_t raw = rk[r];
+                if (k_[r] == ~0ull) continue;
+                const uint64_t raw = k_[r];
                 const unsigned long long key = raw >> 1;
                 uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
+                // most records meet a key already in the table (a target collects one
+                // record per connected source): a plain read finds it, and only an empty
+                // slot costs a compare-and-swap.  A slot's key never changes once set.
                 int probe = 0;
                 for (; probe < BR_SLOTS; ++probe) {
-                    unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
-                    if (prev == BR_EMPTY || prev == key) break;
+                    const unsigned long long cur =
+                        __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (cur == key) break;
+                    if (cur == BR_EMPTY) {
+                        const unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
+                        if (prev == BR_EMPTY || prev == key) break;
+                    }
                     slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
                 }
                 if (probe == BR_SLOTS) {
                     full_here = true;
                     continue;
                 }
+                // zero limbs are skipped (l2 is zero for every |term| < 1)
                 unsigned long long l0, l1, l2;
-                limb_split((double)rv[r], l0, l1, l2);
-                atomicAdd(&t_l0[slot], l0);
-                atomicAdd(&t_l1[slot], l1);
-                atomicAdd(&t_l2[slot], l2);
+                limb_split((double)v_[r], l0, l1, l2);
+                if (l0) atomicAdd(&t_l0[slot], l0);
+                if (l1) atomicAdd(&t_l1[slot], l1);
+                if (l2) atomicAdd(&t_l2[slot], l2);
                 if (raw & 1ull) atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
             }
+        };
+        uint64_t ka[BR_BATCH];
+        T va[BR_BATCH];
+        uint64_t i0 = st;
+        for (; i0 + STEP <= en; i0 += STEP) {
+            load_full(i0, ka, va);
+            insert_round(ka, va);
+        }
+        if (i0 < en) {
+            load_round(i0, ka, va);
+            insert_round(ka, va);
         }
         if (full_here) s_full = 1;
         __syncthreads();
