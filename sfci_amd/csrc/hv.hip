@@ -492,7 +492,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
             // A0
             if (tid == 0) {
-                wbad |= !put_rec(out, out_cap, pos, make_rec((rowbase + s.rank[1]) << 1 | 1ull, (T)0));
+                wbad |= !put_rec(out, out_cap, pos, make_rec((rowbase + s.rank[1]) << 1 | 1ull, tag_val<T>((uint32_t)j)));
                 dterm[j] = s.diag_val;
             }
             pos += 1;
@@ -842,7 +842,7 @@ __global__ void k_rec_split(const R* __restrict__ rec, size_t n, uint64_t* __res
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const R x = rec[i];
         k[i] = rec_key(x);
-        v[i] = rec_val(x);
+        v[i] = (rec_key(x) & 1ull) ? (T)0 : rec_val(x);   // a tag record's value is its source index
     }
 }
 
@@ -1081,6 +1081,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
 constexpr int BR_BATCH = 8;                    // records per thread loaded together
+constexpr int BR_TAGS = 512;                   // tagged targets per bucket finished from a list
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
@@ -1142,26 +1143,18 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags) {
     __shared__ unsigned long long t_key[BR_SLOTS];
     __shared__ unsigned long long t_l0[BR_SLOTS], t_l1[BR_SLOTS], t_l2[BR_SLOTS];
-    __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];
-    __shared__ int s_full;
+    __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];   // tagged, finished by search
+    __shared__ uint32_t t_tl[(BR_SLOTS + 31) / 32];    // tagged, finished from the tag list
+    __shared__ uint32_t s_tslot[BR_TAGS], s_tj[BR_TAGS];
+    __shared__ int s_full, s_ntag;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int slot_shift = 48 - hb;             // 16 hash bits below the bucket bits
     fx128 dacc = fx_zero();
     unsigned long long uniq = 0;
     bool bad_input = false, overflow = false, any_full = false;
-    for (uint64_t b = blockIdx.x; b < n_buckets; b += gridDim.x) {
-        const uint64_t st = bstart[b], en = bend[b];
-        if (st >= en) continue;
-        for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
-            t_key[i] = BR_EMPTY;
-            t_l0[i] = 0;
-            t_l1[i] = 0;
-            t_l2[i] = 0;
-        }
-        for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) t_tag[i] = 0;
-        if (tid == 0) s_full = 0;
-        __syncthreads();
+    // one bucket: merge its records [st, en) in the LDS table and finish its targets
+    auto merge_bucket = [&](const uint64_t st, const uint64_t en) {
         bool full_here = false;
         // BR_BATCH records per thread per round, all loaded before any is inserted; full
         // rounds load without bounds tests, so that their loads are in flight together
@@ -1212,18 +1205,49 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     full_here = true;
                     continue;
                 }
+                if (raw & 1ull) {
+                    // the target's own source: (slot, source index) into the tag list, or,
+                    // when that is full, a tag bit for the search in finish_target
+                    const int t = atomicAdd(&s_ntag, 1);
+                    if (t < BR_TAGS) {
+                        s_tslot[t] = slot;
+                        s_tj[t] = tag_index(v_[r]);
+                        atomicOr(&t_tl[slot >> 5], 1u << (slot & 31));
+                    } else {
+                        atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
+                    }
+                    continue;
+                }
                 // zero limbs are skipped (l2 is zero for every |term| < 1)
                 unsigned long long l0, l1, l2;
                 limb_split((double)v_[r], l0, l1, l2);
                 if (l0) atomicAdd(&t_l0[slot], l0);
                 if (l1) atomicAdd(&t_l1[slot], l1);
                 if (l2) atomicAdd(&t_l2[slot], l2);
-                if (raw & 1ull) atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
             }
         };
+        // the first round's loads are issued before the table is cleared
         uint64_t ka[BR_BATCH];
         T va[BR_BATCH];
-        uint64_t i0 = st;
+        if (st + STEP <= en) load_full(st, ka, va);
+        else load_round(st, ka, va);
+        for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
+            t_key[i] = BR_EMPTY;
+            t_l0[i] = 0;
+            t_l1[i] = 0;
+            t_l2[i] = 0;
+        }
+        for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) {
+            t_tag[i] = 0;
+            t_tl[i] = 0;
+        }
+        if (tid == 0) {
+            s_full = 0;
+            s_ntag = 0;
+        }
+        __syncthreads();
+        insert_round(ka, va);
+        uint64_t i0 = st + STEP;
         for (; i0 + STEP <= en; i0 += STEP) {
             load_full(i0, ka, va);
             insert_round(ka, va);
@@ -1238,20 +1262,9 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
         __syncthreads();
         if (full) {
             any_full = true;
-            continue;
+            return;
         }
-        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // every lane reaches the ballot
-            const int slot = s0 + tid;
-            bool emit = false;
-            uint64_t det = 0;
-            double y = 0.0;
-            const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
-            if (key != BR_EMPTY) {
-                ++uniq;
-                const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                y = finish_target(H, key + base_dense, limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik, iv,
-                                  dterm, n_in, cut, emit, det, dacc, bad_input);
-            }
+        auto emit_wave = [&](bool emit, uint64_t det, double y) {   // every lane of the wave
             uint64_t bal = __ballot(emit);
             if (bal) {
                 unsigned long long base = 0;
@@ -1267,8 +1280,59 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     }
                 }
             }
+        };
+        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // every lane reaches the ballot
+            const int slot = s0 + tid;
+            bool emit = false;
+            uint64_t det = 0;
+            double y = 0.0;
+            const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
+            if (key != BR_EMPTY) {
+                ++uniq;
+                if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
+                    const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
+                    y = finish_target(H, key + base_dense, limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik,
+                                      iv, dterm, n_in, cut, emit, det, dacc, bad_input);
+                }
+            }
+            emit_wave(emit, det, y);
+        }
+        // targets that are input determinants, from the tag list: the same finish as
+        // finish_target with the source index known (no search, no unranking)
+        const int ntag = min(s_ntag, BR_TAGS);
+        for (int t0 = 0; t0 < ntag; t0 += BR_THREADS) {
+            const int t = t0 + tid;
+            bool emit = false;
+            uint64_t det = 0;
+            double y = 0.0;
+            if (t < ntag) {
+                const uint32_t slot = s_tslot[t], j = s_tj[t];
+                const fx128 acc = fx_add(limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), fx_from(dterm[j]));
+                y = fx_to(acc);
+                emit = fabs(y) > cut;
+                det = ik[j];
+                dacc = fx_add(dacc, fx_from(y * iv[j]));
+            }
+            emit_wave(emit, det, y);
         }
         __syncthreads();
+    };
+    uint64_t b = blockIdx.x;
+    uint64_t st = 0, en = 0;
+    if (b < n_buckets) {
+        st = bstart[b];
+        en = bend[b];
+    }
+    for (; b < n_buckets; b += gridDim.x) {
+        // the next bucket's bounds are loaded while this one is merged
+        uint64_t nst = 0, nen = 0;
+        if (b + gridDim.x < n_buckets) {
+            nst = bstart[b + gridDim.x];
+            nen = bend[b + gridDim.x];
+        }
+        if (st < en) merge_bucket(st, en);
+        st = nst;
+        en = nen;
     }
     if (bad_input) atomicOr(flags, FLAG_BAD_INPUT);
     if (overflow) atomicOr(flags, FLAG_OUT_OVERFLOW);

@@ -78,6 +78,34 @@ __host__ __device__ __forceinline__ Rec64 make_rec(uint64_t k, double v) { retur
 __host__ __device__ __forceinline__ Rec32 make_rec(uint64_t k, float v) {
     return Rec32{(uint32_t)k, (uint32_t)(k >> 32), v};
 }
+// A diagonal (tag) record carries no value: its value field holds the bits of the source
+// index j (< 2^32), so that the merge finds the source's FP64 diagonal term and input
+// coefficient without a search.  Consumers that sum values treat tag records as zero.
+template <typename T>
+__host__ __device__ __forceinline__ T tag_val(uint32_t j);
+template <>
+__host__ __device__ __forceinline__ double tag_val<double>(uint32_t j) {
+    const uint64_t b = j;
+    double v;
+    __builtin_memcpy(&v, &b, 8);
+    return v;
+}
+template <>
+__host__ __device__ __forceinline__ float tag_val<float>(uint32_t j) {
+    float v;
+    __builtin_memcpy(&v, &j, 4);
+    return v;
+}
+__host__ __device__ __forceinline__ uint32_t tag_index(double v) {
+    uint64_t b;
+    __builtin_memcpy(&b, &v, 8);
+    return (uint32_t)b;
+}
+__host__ __device__ __forceinline__ uint32_t tag_index(float v) {
+    uint32_t b;
+    __builtin_memcpy(&b, &v, 4);
+    return b;
+}
 
 // Device-side error flags (bit set by kernels).
 enum : uint32_t {

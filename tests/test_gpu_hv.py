@@ -115,6 +115,31 @@ def test_hv_bit_exact(case, sys1, orc1, ctx1):
     assert st["n_out"] == len(ok)
 
 
+@pytest.mark.parametrize("hbits", [1, 2])
+def test_hv_bucket_tags_list_and_search(make_system, hbits):
+    # the whole 10-orbital / 4-electron space as input: every target is an input
+    # determinant.  With 2 buckets each holds about 1000 of them, more than the
+    # per-bucket tag list, so the rest are finished through the search; with 4 buckets
+    # the list holds them all.  No bucket overflows in either case.
+    from sfci_amd import gpu
+    sysS = make_system(10, 4, 7)
+    orcS = O.Oracle(sysS["in_path"], sysS["n_so"], sysS["n_elec"])
+    ctx = gpu.Context(sysS["in_path"], sysS["n_so"], sysS["n_elec"])
+    try:
+        k = all_dets(sysS["n_orb"], 2)
+        v = np.random.default_rng(11).normal(size=len(k))
+        ctx.set_merge_mode(2, hbits)
+        gk, gv, gd = gpu_hv(ctx, k, v, len(k), 0.3, 0.7, 60_000_000)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    log("tags: %d dets, hash bits %d, fallback %d" % (len(k), st["hash_bits"], st["n_fallback"]))
+    assert st["n_fallback"] == 0 and st["hash_bits"] == hbits
+    ok, ov, od, _ = orcS.hv(k, v, nd=len(k), lam=0.3, coeff=0.7, cut=1e-5)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits([gd])[0] == bits([od])[0]
+
+
 def test_hv_multipass_matches(sys1, orc1, ctx1):
     k, v = random_input(sys1, 4000, 7)
     ctx1.set_workspace(1 << 26)            # 64 MB: forces several target-row passes
