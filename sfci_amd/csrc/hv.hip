@@ -848,20 +848,19 @@ __global__ void k_rec_split(const R* __restrict__ rec, size_t n, uint64_t* __res
 
 // Finish one merged target: for a target that is an input determinant (tag), add the
 // FP64 diagonal term of its source and accumulate y_t * c_t into diag; round the exact
-// sum; decide the cut.  The determinant key is rebuilt from the dense index only when
-// needed (tagged or kept).
+// sum; decide the cut.  Kept targets are staged under their dense index (converted to
+// determinant keys once, after every pass); the key is rebuilt here only for the search of
+// a tagged target.
 __device__ __forceinline__ double finish_target(const HamDev& H, uint64_t dense, fx128 acc, uint32_t tag,
                                                 const uint64_t* __restrict__ ik, const double* __restrict__ iv,
                                                 const double* __restrict__ dterm, size_t n_in, double cut, bool& emit,
-                                                uint64_t& det, fx128& dacc, bool& bad_input) {
-    bool have_det = false;
+                                                fx128& dacc, bool& bad_input) {
     size_t lo = 0;
     bool found = false;
     if (tag) {
         uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-        det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
-              (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
-        have_det = true;
+        const uint64_t det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
+                             (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
         size_t hi = n_in;
         while (lo < hi) {
             size_t mid = (lo + hi) >> 1;
@@ -874,11 +873,6 @@ __device__ __forceinline__ double finish_target(const HamDev& H, uint64_t dense,
     }
     double y = fx_to(acc);
     emit = fabs(y) > cut;
-    if (emit && !have_det) {
-        uint64_t r0 = dense / H.dim, r1 = dense - r0 * H.dim;
-        det = spread_string(colex_unrank(r0, H.k, H.n_orb, H.binom)) |
-              (spread_string(colex_unrank(r1, H.k, H.n_orb, H.binom)) << 1);
-    }
     if (found) dacc = fx_add(dacc, fx_from(y * iv[lo]));
     return y;
 }
@@ -1044,8 +1038,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
                 double y = 0.0;
                 if (r < ne) {
                     ++uniq;
-                    y = finish_target(H, ekey[r] + base_dense, eval[r], etag[r], ik, iv, dterm, n_in, cut, emit,
-                                      det, dacc, bad_input);
+                    det = ekey[r] + base_dense;   // staged as the dense index
+                    y = finish_target(H, det, eval[r], etag[r], ik, iv, dterm, n_in, cut, emit, dacc, bad_input);
                 }
                 uint64_t bal = __ballot(emit);
                 if (bal) {
@@ -1082,6 +1076,7 @@ constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
 constexpr int BR_BATCH = 8;                    // records per thread loaded together
 constexpr int BR_TAGS = 512;                   // tagged targets per bucket finished from a list
+static_assert(BR_TAGS <= BR_THREADS, "the tag list is finished one entry per thread");
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
@@ -1147,6 +1142,8 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     __shared__ uint32_t t_tl[(BR_SLOTS + 31) / 32];    // tagged, finished from the tag list
     __shared__ uint32_t s_tslot[BR_TAGS], s_tj[BR_TAGS];
     __shared__ int s_full, s_ntag;
+    __shared__ uint32_t s_wtot[BR_THREADS / SFCI_WAVE];
+    __shared__ unsigned long long s_obase;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int slot_shift = 48 - hb;             // 16 hash bits below the bucket bits
@@ -1264,56 +1261,75 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             any_full = true;
             return;
         }
-        auto emit_wave = [&](bool emit, uint64_t det, double y) {   // every lane of the wave
-            uint64_t bal = __ballot(emit);
-            if (bal) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(out_count, (unsigned long long)popc64(bal));
-                base = __shfl(base, 0, SFCI_WAVE);
-                if (emit) {
-                    uint64_t pos = base + (uint64_t)popc64(bal & lt);
-                    if (pos < out_cap) {
-                        sk_out[pos] = det;
-                        sv_out[pos] = y;
-                    } else {
-                        overflow = true;
-                    }
-                }
-            }
-        };
-        for (int s0 = 0; s0 < BR_SLOTS; s0 += BR_THREADS) {   // every lane reaches the ballot
-            const int slot = s0 + tid;
-            bool emit = false;
-            uint64_t det = 0;
-            double y = 0.0;
+        // finish: every slot's target, and the tag list's, is rounded and cut; the kept
+        // ones (staged under their dense index) are written after one block-wide scan of
+        // the per-thread counts, with one global reservation per bucket
+        constexpr int FR = (BR_SLOTS + BR_THREADS - 1) / BR_THREADS;
+        bool em[FR + 1];
+        double yv[FR + 1];
+        uint64_t dk[FR + 1];
+        uint32_t ne = 0;
+#pragma unroll
+        for (int r = 0; r < FR; ++r) {
+            const int slot = r * BR_THREADS + tid;
+            em[r] = false;
+            yv[r] = 0.0;
+            dk[r] = 0;
             const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
             if (key != BR_EMPTY) {
                 ++uniq;
                 if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
                     const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                    y = finish_target(H, key + base_dense, limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik,
-                                      iv, dterm, n_in, cut, emit, det, dacc, bad_input);
+                    dk[r] = key + base_dense;
+                    yv[r] = finish_target(H, dk[r], limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik, iv,
+                                          dterm, n_in, cut, em[r], dacc, bad_input);
                 }
             }
-            emit_wave(emit, det, y);
+            ne += em[r] ? 1u : 0u;
         }
-        // targets that are input determinants, from the tag list: the same finish as
-        // finish_target with the source index known (no search, no unranking)
-        const int ntag = min(s_ntag, BR_TAGS);
-        for (int t0 = 0; t0 < ntag; t0 += BR_THREADS) {
-            const int t = t0 + tid;
-            bool emit = false;
-            uint64_t det = 0;
-            double y = 0.0;
-            if (t < ntag) {
-                const uint32_t slot = s_tslot[t], j = s_tj[t];
+        // targets that are input determinants, from the tag list (BR_TAGS <= BR_THREADS):
+        // the same finish as finish_target with the source index known, no search
+        {
+            const int ntag = min(s_ntag, BR_TAGS);
+            em[FR] = false;
+            yv[FR] = 0.0;
+            dk[FR] = 0;
+            if (tid < ntag) {
+                const uint32_t slot = s_tslot[tid], j = s_tj[tid];
                 const fx128 acc = fx_add(limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), fx_from(dterm[j]));
-                y = fx_to(acc);
-                emit = fabs(y) > cut;
-                det = ik[j];
-                dacc = fx_add(dacc, fx_from(y * iv[j]));
+                yv[FR] = fx_to(acc);
+                em[FR] = fabs(yv[FR]) > cut;
+                dk[FR] = t_key[slot] + base_dense;
+                dacc = fx_add(dacc, fx_from(yv[FR] * iv[j]));
+                ne += em[FR] ? 1u : 0u;
             }
-            emit_wave(emit, det, y);
+        }
+        uint32_t inc = ne;
+        for (int o = 1; o < SFCI_WAVE; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, SFCI_WAVE);
+            if (lane >= o) inc += t;
+        }
+        if (lane == SFCI_WAVE - 1) s_wtot[tid / SFCI_WAVE] = inc;
+        __syncthreads();
+        uint32_t pre = inc - ne, tot = 0;
+        for (int i = 0; i < BR_THREADS / SFCI_WAVE; ++i) {
+            const uint32_t wt = s_wtot[i];
+            if (i < tid / SFCI_WAVE) pre += wt;
+            tot += wt;
+        }
+        if (tid == 0) s_obase = tot ? atomicAdd(out_count, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        uint64_t pos = s_obase + pre;
+#pragma unroll
+        for (int r = 0; r <= FR; ++r) {
+            if (!em[r]) continue;
+            if (pos < out_cap) {
+                sk_out[pos] = dk[r];
+                sv_out[pos] = yv[r];
+            } else {
+                overflow = true;
+            }
+            ++pos;
         }
         __syncthreads();
     };
@@ -1340,6 +1356,20 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     fx_block_reduce_atomic<BR_THREADS>(dacc, diag_acc);
     for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) uniq += __shfl_down(uniq, off, SFCI_WAVE);
     if (lane == 0 && uniq) atomicAdd(unique_count, uniq);
+}
+
+// Staged outputs carry dense indices (spin-0 rank * dim + spin-1 rank); rewrite them as
+// determinant keys, with the binomial table in LDS.
+__global__ __launch_bounds__(SFCI_BLOCK) void k_dense_to_det(HamDev Hg, uint64_t* __restrict__ k, size_t n) {
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    for (int i = threadIdx.x; i < Hg.n_orb * 33; i += blockDim.x) s_binom[i] = Hg.binom[i];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t dense = k[i];
+        const uint64_t r0 = dense / Hg.dim, r1 = dense - r0 * Hg.dim;
+        k[i] = spread_string(colex_unrank(r0, Hg.k, Hg.n_orb, s_binom)) |
+               (spread_string(colex_unrank(r1, Hg.k, Hg.n_orb, s_binom)) << 1);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -1672,8 +1702,12 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
     st.n_unique = uniq;
     if (st.n_candidates) c->unique_ratio = std::max(1e-6, (double)uniq / (double)st.n_candidates);
-    // 4. top out_cap by |y|, then key order
+    // 4. staged dense indices -> determinant keys; top out_cap by |y|, then key order
     size_t m = out_count_host;
+    if (m > 0) {
+        k_dense_to_det<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(H, stk, m);
+        HIPCHK(hipGetLastError());
+    }
     size_t kept = 0;
     if (all) {
         Timer t(c->stream);
