@@ -384,7 +384,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
 template <typename R>
 __device__ __forceinline__ bool put_rec(R* __restrict__ out, uint64_t cap, uint64_t i, const R& r) {
     if (i >= cap) return false;
-    out[i] = r;
+    rec_store(out + i, r);
     return true;
 }
 

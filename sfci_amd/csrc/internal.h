@@ -78,6 +78,20 @@ __host__ __device__ __forceinline__ Rec64 make_rec(uint64_t k, double v) { retur
 __host__ __device__ __forceinline__ Rec32 make_rec(uint64_t k, float v) {
     return Rec32{(uint32_t)k, (uint32_t)(k >> 32), v};
 }
+// Record lists are streamed: the generation kernel writes every FP64 record once, with a
+// whole line per wavefront, through non-temporal stores, which keep the records from
+// evicting the integral tables that the kernel rereads.  (Non-temporal loads, and the
+// partition scatters' short runs, measured slower with the non-temporal forms, so they
+// use ordinary accesses.)
+typedef unsigned long long rec_u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void rec_store(Rec64* p, const Rec64& r) {
+    rec_u64x2 v;
+    v.x = r.key;
+    v.y = (unsigned long long)__double_as_longlong(r.val);
+    __builtin_nontemporal_store(v, (rec_u64x2*)p);
+}
+__device__ __forceinline__ void rec_store(Rec32* p, const Rec32& r) { *p = r; }
+
 // A diagonal (tag) record carries no value: its value field holds the bits of the source
 // index j (< 2^32), so that the merge finds the source's FP64 diagonal term and input
 // coefficient without a search.  Consumers that sum values treat tag records as zero.
