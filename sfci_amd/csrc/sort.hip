@@ -282,8 +282,8 @@ void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* 
 // of one target are summed exactly, in any order.  A chunk is at most one tile of one
 // region; its counter for digit d sits at hbase + d * hstride, so one flat exclusive scan
 // gives every output offset of pass 2.
-constexpr int PART_TILE = 4096;
-constexpr int PART_THREADS = 512;
+constexpr int PART_TILE = 8192;
+constexpr int PART_THREADS = 1024;
 constexpr int PART_ITEMS = PART_TILE / PART_THREADS;   // 8
 constexpr int HIST_CHUNKS = 16;                        // chunks per histogram workgroup
 
