@@ -265,7 +265,8 @@ void sfci_destroy(sfci_ctx* X) {
                       &c.part_len[0], &c.part_len[1], &c.part_hbase[0], &c.part_hbase[1], &c.part_hstride[0],
                       &c.part_hstride[1], &c.part_region[0], &c.part_region[1], &c.part_base, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
-                      &c.scratch_off, &c.scratch_small, &c.rowhist};
+                      &c.scratch_off, &c.scratch_small, &c.rowhist, &c.plan_k[0], &c.plan_k[1], &c.plan_v[0], &c.plan_v[1],
+                      &c.plan_fl, &c.plan_pos, &c.plan_first, &c.plan_str, &c.plan_cnt};
     for (DevBuf* b : bufs) b->release();
     for (auto& b : c.scratch_scan) b.release();
     void* ptrs[] = {c.d_h, c.d_g, c.d_gt, c.d_binom, c.d_occ_pairs, c.d_vir_pairs, c.d_sdec, c.d_acc, c.d_flags, c.d_counter};

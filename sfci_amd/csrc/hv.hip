@@ -265,10 +265,11 @@ struct DivIter {
 
 // ---------------------------------------------------------------------------------
 // Per-source count of connections whose target row lies in [row_lo, row_hi).
-// Per-source candidate counts of every pass at once: pass p covers target rows
-// [bounds[p], bounds[p + 1]); cnt[p * n_src + j] receives source j's count in pass p
-// (the same weights as k_rowgroup_hist: own row 1 + singles + doubles of spin 1, a
-// spin-0 single 1 + spin-1 singles, a spin-0 double 1).
+// Candidate counts of every pass at once, per distinct spin-0 string (every source with
+// that string has the same counts): pass p covers target rows [bounds[p], bounds[p + 1]);
+// cnt[p * n_row + u] receives the count of a source with string astr[u] in pass p (the
+// same weights as k_rowgroup_hist: own row 1 + singles + doubles of spin 1, a spin-0
+// single 1 + spin-1 singles, a spin-0 double 1).
 constexpr int MAX_PASSES = 256;
 
 __device__ __forceinline__ int pass_of(const uint64_t* bounds, int n_pass, uint64_t r) {
@@ -281,7 +282,7 @@ __device__ __forceinline__ int pass_of(const uint64_t* bounds, int n_pass, uint6
     return lo;
 }
 
-__global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const uint64_t* __restrict__ sk, size_t n_src,
+__global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const uint64_t* __restrict__ astr, size_t n_src,
                                                              const uint64_t* __restrict__ bounds_g, int n_pass,
                                                              uint32_t* __restrict__ cnt) {
     __shared__ uint64_t s_binom[LDS_BINOM];
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const ui
         for (int i = threadIdx.x; i < n_pass; i += SFCI_BLOCK) s_cnt[i] = 0;
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t st = spin_string(sk[j], 0);
+            uint32_t st = (uint32_t)astr[j];
             s_str = st;
             int a = 0, b = 0;
             for (int i = 0; i < H.n_orb; ++i) {
@@ -331,10 +332,11 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const ui
 }
 
 // Histogram of connection counts over target-row groups (row >> grp_shift), for pass
-// planning.  Each thread handles one source at a time.
-__global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const uint64_t* __restrict__ sk,
-                                                              size_t n_src, uint64_t row_lo, uint64_t row_hi,
-                                                              int grp_shift, int n_grp,
+// planning, per distinct spin-0 string astr[j] weighted by its number of sources
+// first[j + 1] - first[j].
+__global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const uint64_t* __restrict__ astr,
+                                                              const uint64_t* __restrict__ first, size_t n_src,
+                                                              uint64_t row_lo, uint64_t row_hi, int grp_shift, int n_grp,
                                                               unsigned long long* __restrict__ hist) {
     __shared__ uint64_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
@@ -343,7 +345,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
     for (int i = threadIdx.x; i < n_grp; i += SFCI_BLOCK) s_h[i] = 0;
     __syncthreads();
     for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
-        uint32_t st = spin_string(sk[j], 0);
+        const uint32_t st = (uint32_t)astr[j];
+        const unsigned long long m = first[j + 1] - first[j];   // sources with this string
         uint8_t occ[SFCI_MAX_K], vir[SFCI_MAX_ORB];
         int a = 0, b = 0;
         for (int i = 0; i < H.n_orb; ++i) {
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
         if (threadIdx.x == 0) {
             const uint64_t r = colex_rank(st, H.binom);
             if (r >= row_lo && r < row_hi)
-                atomicAdd(&s_h[(r - row_lo) >> grp_shift], (unsigned long long)(1 + H.n_sa + H.n_daa));
+                atomicAdd(&s_h[(r - row_lo) >> grp_shift], m * (unsigned long long)(1 + H.n_sa + H.n_daa));
         }
         const uint64_t n_alpha = H.n_sa + H.n_daa;
         for (uint64_t e = threadIdx.x; e < n_alpha; e += SFCI_BLOCK) {
@@ -371,12 +374,50 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
                 w = 1;
             }
             const uint64_t r = colex_rank(s2, H.binom);
-            if (r >= row_lo && r < row_hi) atomicAdd(&s_h[(r - row_lo) >> grp_shift], w);
+            if (r >= row_lo && r < row_hi) atomicAdd(&s_h[(r - row_lo) >> grp_shift], m * w);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n_grp; i += SFCI_BLOCK)
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Row table of the sources' spin-0 strings: keys = strings sorted ascending with the
+// source index as payload; fl[i] = 1 where a new string starts.
+__global__ void k_alpha_keys(const uint64_t* __restrict__ sk, size_t n, uint64_t* __restrict__ k, double* __restrict__ v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        k[i] = spin_string(sk[i], 0);
+        v[i] = (double)i;
+    }
+}
+
+__global__ void k_row_starts(const uint64_t* __restrict__ k, size_t n, uint32_t* __restrict__ fl) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        fl[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+}
+
+// pos = exclusive scan of fl: string u = pos[i] starts at sorted position i
+__global__ void k_row_table(const uint64_t* __restrict__ k, const uint32_t* __restrict__ fl,
+                            const uint64_t* __restrict__ pos, size_t n, uint64_t n_row, uint64_t* __restrict__ str,
+                            uint64_t* __restrict__ first) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (fl[i]) {
+            str[pos[i]] = k[i];
+            first[pos[i]] = i;
+        }
+        if (i == 0) first[n_row] = n;
+    }
+}
+
+// per-source pass counts from the per-string counts
+__global__ void k_expand_counts(const double* __restrict__ src_of, const uint32_t* __restrict__ fl,
+                                const uint64_t* __restrict__ pos, size_t n, uint64_t n_row, int n_pass,
+                                const uint32_t* __restrict__ cnt_row, uint32_t* __restrict__ cnt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = (size_t)src_of[i];
+        const uint64_t u = pos[i] + fl[i] - 1;
+        for (int p = 0; p < n_pass; ++p) cnt[(size_t)p * n + j] = cnt_row[(size_t)p * n_row + u];
+    }
 }
 
 // Bounds-checked record store (the planning count sizes the buffer; a store beyond it is
@@ -1436,6 +1477,32 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
     }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
+    // distinct spin-0 strings of the sources (planning work depends only on them)
+    uint64_t n_row_src = 0;
+    uint64_t *row_str = nullptr, *row_first = nullptr, *row_pos = nullptr;
+    uint32_t* row_fl = nullptr;
+    double* row_src = nullptr;
+    auto row_table = [&]() {
+        if (row_str) return;
+        uint64_t* ka = c->plan_k[0].get<uint64_t>(ns);
+        double* va = c->plan_v[0].get<double>(ns);
+        uint64_t* kb = c->plan_k[1].get<uint64_t>(ns);
+        double* vb = c->plan_v[1].get<double>(ns);
+        const int g = grid_for(c, (ns + SFCI_BLOCK - 1) / SFCI_BLOCK, 16);
+        k_alpha_keys<<<g, SFCI_BLOCK, 0, c->stream>>>(sk, ns, ka, va);
+        radix_sort_pairs(c, &ka, &va, kb, vb, ns, H.n_orb);
+        row_fl = c->plan_fl.get<uint32_t>(ns);
+        row_pos = c->plan_pos.get<uint64_t>(ns + 1);
+        k_row_starts<<<g, SFCI_BLOCK, 0, c->stream>>>(ka, ns, row_fl);
+        exclusive_scan_u32(c, row_fl, ns, row_pos, row_pos + ns);
+        HIPCHK(hipMemcpyAsync(&n_row_src, row_pos + ns, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        row_str = c->plan_str.get<uint64_t>(n_row_src);
+        row_first = c->plan_first.get<uint64_t>(n_row_src + 1);
+        k_row_table<<<g, SFCI_BLOCK, 0, c->stream>>>(ka, row_fl, row_pos, ns, n_row_src, row_str, row_first);
+        HIPCHK(hipGetLastError());
+        row_src = va;
+    };
     std::vector<std::pair<uint64_t, uint64_t>> passes;
     std::vector<uint64_t> pass_count;   // candidates of each pass
     if (total_cand <= pass_cap) {
@@ -1453,9 +1520,10 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             const int n_grp = (int)((hi - lo - 1) >> grp_shift) + 1;
             unsigned long long* hist = c->rowhist.get<unsigned long long>(n_grp);
             HIPCHK(hipMemsetAsync(hist, 0, n_grp * sizeof(unsigned long long), c->stream));
-            int tr = kt_begin(c, K_ROWGROUP, 8.0 * ns);
-            k_rowgroup_hist<<<grid_for(c, ns), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
-                H, sk, ns, lo, hi, grp_shift, n_grp, hist);
+            row_table();
+            int tr = kt_begin(c, K_ROWGROUP, 16.0 * n_row_src);
+            k_rowgroup_hist<<<grid_for(c, n_row_src), SFCI_BLOCK, n_grp * sizeof(unsigned long long), c->stream>>>(
+                H, row_str, row_first, n_row_src, lo, hi, grp_shift, n_grp, hist);
             kt_end(c, tr);
             std::vector<unsigned long long> hh(n_grp);
             HIPCHK(hipMemcpyAsync(hh.data(), hist, n_grp * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -1508,8 +1576,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         uint64_t* dbounds = c->part_meta[1].get<uint64_t>(bounds.size());
         HIPCHK(hipMemcpyAsync(dbounds, bounds.data(), bounds.size() * 8, hipMemcpyHostToDevice, c->stream));
         pcnt = c->src_cnt.get<uint32_t>(passes.size() * ns);
-        int tcn = kt_begin(c, K_ROWGROUP, 8.0 * ns + 4.0 * passes.size() * ns);
-        k_count_passes<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ns, dbounds, (int)passes.size(), pcnt);
+        row_table();
+        uint32_t* pcnt_row = c->plan_cnt.get<uint32_t>(passes.size() * n_row_src);
+        int tcn = kt_begin(c, K_ROWGROUP, 8.0 * n_row_src + 4.0 * passes.size() * (n_row_src + ns));
+        k_count_passes<<<grid_for(c, n_row_src), SFCI_BLOCK, 0, c->stream>>>(H, row_str, n_row_src, dbounds,
+                                                                             (int)passes.size(), pcnt_row);
+        k_expand_counts<<<grid_for(c, (ns + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
+            row_src, row_fl, row_pos, ns, n_row_src, (int)passes.size(), pcnt_row, pcnt);
         kt_end(c, tcn);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));   // bounds live on the host stack

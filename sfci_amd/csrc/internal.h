@@ -216,6 +216,7 @@ struct Ctx {
     DevBuf scratch_hist, scratch_off, scratch_small;
     DevBuf scratch_scan[8];
     DevBuf rowhist;
+    DevBuf plan_k[2], plan_v[2], plan_fl, plan_pos, plan_first, plan_str, plan_cnt;   // row-grouped planning
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
     bool g_one_block = false;        // the four two-electron spin blocks share one table
