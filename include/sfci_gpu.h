@@ -60,6 +60,9 @@ typedef struct {
     double ms_generate, ms_sort, ms_reduce, ms_select, ms_total;
     uint64_t n_fallback;    /* passes whose hash-bucket merge overflowed and were re-merged by full sort */
     int hash_bits;          /* hash bits of the last pass's bucket merge (0: full sort) */
+    uint64_t n_kept;        /* screened H.v: candidates kept for the merge (n_candidates = emitted) */
+    double screen_eps;      /* screened H.v: prefilter used (raised when the list would not fit) */
+    double ms_score;        /* screened H.v: scoring time */
 } sfci_hv_stats_t;
 
 /* Context from a reference _IN integral file: replaces load_integrals/ReadElements
@@ -100,7 +103,7 @@ int sfci_sync(sfci_ctx* ctx);
  * (no reference counterpart: the reference only prints wall-clock timers,
  * src/functions.cpp:800-801).  Kernel ids 0..SFCI_KERNEL_COUNT-1 are named by
  * sfci_kernel_name(); bytes are the algorithmic bytes each launch reads and writes. */
-#define SFCI_KERNEL_COUNT 14
+#define SFCI_KERNEL_COUNT 17
 typedef struct {
     double ms;
     uint64_t launches;
@@ -178,11 +181,18 @@ typedef struct {
     double ms_total;     /* wall time of the solve */
     double ms_hv;        /* wall time spent in sfci_hv */
     int64_t n_hv;        /* number of sfci_hv calls */
+    int n_screened;      /* restarts whose products were screened (sfci_set_screen) */
+    int n_reverts;       /* screened restarts rejected by the judge (see sfci_ground_state) */
 } sfci_lanczos_result_t;
 
 /* Runs from the single determinant init_det (coefficient 1).  ritz and e_trunc receive
  * one value per restart (capacity num_restart): the selected Ritz value and <v|H|v>
- * after truncation.  v_out (optional, capacity >= nd) receives the final vector. */
+ * after truncation.  v_out (optional, capacity >= nd) receives the final vector.
+ * With screening on (sfci_set_screen), the unscreened energy readout <v|H|v> judges every
+ * restart: a restart whose energy rises by more than 1e-9 Ha over the best so far is
+ * rejected, the best vector is restored and the rest of the solve runs unscreened; the
+ * solve stops when the judged energy drops by less than delta_e.  The context's screening
+ * setting is the same on return. */
 int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_lanczos_params_t* params, double* ritz,
                       double* e_trunc, sfci_lanczos_result_t* result, sfci_vec* v_out);
 
@@ -252,6 +262,76 @@ int sfci_cisd(sfci_ctx* ctx, uint64_t ref_det, int state_idx, sfci_vec* out, dou
  * commented out, src/main.cpp:235-242.) */
 int sfci_excited(sfci_ctx* ctx, uint64_t init_det, double target_e, const sfci_linear_params_t* params,
                  sfci_vec* v_out, double* energy, double* e_cisd);
+
+/* ---- learned candidate screener ------------------------------------------------------
+ * No reference counterpart: the reference keeps every candidate of H.v
+ * (src/functions.cpp:623-639).  With screening on, sfci_hv (and every driver built on it)
+ *   1. emits only connections with |H_ij coeff c_j| > eps_pre (diagonal tags always),
+ *   2. scores each emitted candidate (mode 2: the network's predicted log|c| of its target;
+ *      mode 1: |H_ij coeff c_j| alone),
+ *   3. keeps the candidates whose score reaches the keep-th largest score
+ *      (keep = keep_factor * nd; ties at that score kept) and every diagonal tag,
+ *   4. merges the kept candidates exactly and applies the cut, cap and order of sfci_hv.
+ * Values are FP32 products as in SFCI_PREC_MIXED; sums stay exact.  The diag output is
+ * the exact <kept|y> of the screened y. */
+typedef struct sfci_mlp sfci_mlp;
+#define SFCI_MLP_INPUTS 66     /* +-1 occupation of 64 spin orbitals, excitation rank, plog2|v| */
+#define SFCI_MLP_HIDDEN 256
+/* flat FP32 parameters: W1[256][66] (row = hidden unit), b1[256], W2[256][256],
+ * b2[256], w3[256], b3 */
+#define SFCI_MLP_NPARAM 83201
+
+/* One training sample: a candidate's target determinant, its excitation rank, plog2 of its
+ * value |H_ij coeff c_j| (the FP32 exponent plus the mantissa fraction: log2 piecewise
+ * linear, -150 for zero), and the label log|y_t| (natural log) of its target in the full
+ * product. */
+typedef struct {
+    uint64_t det;
+    float logv;
+    float label;
+    uint32_t cls;
+    uint32_t pad;
+} sfci_screen_sample_t;
+
+/* Network on the context's GPU; weights uniform in +-1/sqrt(fan_in) from `seed`, zero biases. */
+int sfci_mlp_create(sfci_ctx* ctx, uint64_t seed, sfci_mlp** out);
+void sfci_mlp_destroy(sfci_mlp* net);
+/* Master FP32 parameters (SFCI_MLP_NPARAM floats); set resets the Adam state. */
+int sfci_mlp_set_params(sfci_mlp* net, const float* params);
+int sfci_mlp_get_params(const sfci_mlp* net, float* params);
+/* Input and label normalisation: feature 65 = (logv - x_shift) * x_scale (FP32); training target
+ * = (label - y_shift) * y_scale (scales > 0). */
+int sfci_mlp_set_norm(sfci_mlp* net, float x_shift, float x_scale, float y_shift, float y_scale);
+/* y[i] = network(x[i]) for explicit features x[n][66] (rounded to BF16 on the GPU). */
+int sfci_mlp_forward(sfci_mlp* net, const float* x, int64_t n, float* y);
+/* Gradient of the mean squared error mean_i (y_i - t_i)^2 over x[n][66] (n % 256 == 0)
+ * in the flat parameter layout; *loss (optional) receives the error. */
+int sfci_mlp_gradient(sfci_mlp* net, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* One Adam step (bias-corrected) from a given flat gradient. */
+int sfci_mlp_adam_step(sfci_mlp* net, const float* grad, float lr, float beta1, float beta2, float eps);
+/* `steps` Adam steps (lr, 0.9, 0.999, 1e-8) on batches of `batch` samples (batch % 256 == 0)
+ * drawn from samples[n] by a hash of (seed, step, row); loss_curve (optional, `steps` floats)
+ * receives each batch's mean squared error in normalised units. */
+int sfci_mlp_train(sfci_mlp* net, const sfci_screen_sample_t* samples, int64_t n, int steps, int batch, float lr,
+                   uint64_t seed, float* loss_curve);
+/* Network output (normalised units) for samples, and the features it sees (n x 66). */
+int sfci_mlp_predict_samples(sfci_mlp* net, const sfci_screen_sample_t* samples, int64_t n, float* y);
+int sfci_mlp_sample_features(sfci_mlp* net, const sfci_screen_sample_t* samples, int64_t n, float* x);
+
+#define SFCI_SCREEN_OFF 0
+#define SFCI_SCREEN_MAGNITUDE 1   /* rank candidates by |H_ij coeff c_j| */
+#define SFCI_SCREEN_NETWORK 2     /* rank candidates by the network's prediction */
+/* Screening of sfci_hv (see above); net is required for SFCI_SCREEN_NETWORK and must
+ * outlive its use. */
+int sfci_set_screen(sfci_ctx* ctx, int mode, sfci_mlp* net, double keep_factor, double eps_pre);
+/* The current screening setting (any output pointer may be null). */
+int sfci_get_screen(const sfci_ctx* ctx, int* mode, sfci_mlp** net, double* keep_factor, double* eps_pre);
+/* Training samples from the product H v: the candidates of every entry of v that pass eps_pre
+ * (off-diagonal only), a deterministic subsample of at most max_samples of them chosen by
+ * a hash of (seed, candidate), each labelled with log|y_t| of its target in the full,
+ * uncut y = H v (log of 1e-30 for an exact zero).  *n_out receives the count. */
+int sfci_screen_samples(sfci_ctx* ctx, const sfci_vec* v, double eps_pre, int64_t max_samples, uint64_t seed,
+                        sfci_screen_sample_t* out, int64_t* n_out);
 
 #ifdef __cplusplus
 }

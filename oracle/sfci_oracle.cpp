@@ -501,6 +501,95 @@ int oracle_hv(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, in
     return 0;
 }
 
+// ---- screened H.v (no reference counterpart) ---------------------------------------
+// The screened product keeps a subset of the off-diagonal candidates of the mixed-precision
+// product (value (float)H_ij * (float)(coeff c_j)) and every diagonal term, and merges them
+// exactly as oracle_hv(mode 2) does.  Step 1 lists the candidates that pass the prefilter
+// |value| > (float)eps; the caller decides which to keep; step 2 merges the kept ones.
+
+// Step 1: candidates of top_nd(in) in source (key) order and connections_exact order,
+// with their excitation rank (1 single, 2 double).  Returns the count; writes up to cap.
+int64_t oracle_screen_candidates(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, int64_t n_in,
+                                 int64_t nd, double coeff, double eps, uint64_t* t_out, float* v_out,
+                                 uint32_t* cls_out, int64_t cap) {
+    const Ham& H = *(const Ham*)hp;
+    std::vector<KV> src(n_in);
+    for (int64_t i = 0; i < n_in; ++i) src[i] = {in_k[i], in_v[i]};
+    if (n_in > nd) top_cap_sorted(src, (size_t)nd);
+    const float feps = (float)eps;
+    int64_t n = 0;
+    for (const KV& s : src) {
+        const double temp = coeff * s.v;
+        bool first = true;
+        connections_exact(H, s.k, [&](det_t t, double hij) {
+            if (first) {   // the diagonal: always merged, in FP64
+                first = false;
+                return;
+            }
+            const float val = (float)hij * (float)temp;
+            if (!(std::fabs(val) > feps)) return;
+            if (n < cap) {
+                t_out[n] = t;
+                v_out[n] = val;
+                cls_out[n] = (uint32_t)(__builtin_popcountll(t ^ s.k) / 2);
+            }
+            ++n;
+        });
+    }
+    return n;
+}
+
+// Step 2: y_t = sum of the kept candidates' values for t + (E_tt - lambda) coeff c_t for t in
+// top_nd(in), exact; diag = <top_nd(in)|y> (exact, before the cut); out = top_{out_cap} of
+// {|y| > cut}, key order.  Returns 0, or -1 when a term leaves the fixed-point range.
+int oracle_screen_merge(const oracle_ham* hp, const uint64_t* in_k, const double* in_v, int64_t n_in, int64_t nd,
+                        double lambda, double coeff, double cut, int64_t out_cap, const uint64_t* ct, const float* cv,
+                        int64_t nc, uint64_t* out_k, double* out_v, int64_t* out_len, double* diag_out) {
+    const Ham& H = *(const Ham*)hp;
+    std::vector<KV> src(n_in);
+    for (int64_t i = 0; i < n_in; ++i) src[i] = {in_k[i], in_v[i]};
+    if (n_in > nd) top_cap_sorted(src, (size_t)nd);
+    std::vector<Cand> buf;
+    buf.reserve((size_t)nc + src.size());
+    for (int64_t i = 0; i < nc; ++i) {
+        if (!fx_ok((double)cv[i])) return -1;
+        buf.push_back({ct[i], (double)cv[i], 0});
+    }
+    for (const KV& s : src) {
+        const double dv = (diag_exact(H, s.k) - lambda) * (coeff * s.v);
+        if (!fx_ok(dv)) return -1;
+        buf.push_back({s.k, dv, 1});
+    }
+    std::stable_sort(buf.begin(), buf.end(), [](const Cand& a, const Cand& c) { return a.k < c.k; });
+    std::vector<KV> fin;
+    fx_t dacc = 0;
+    size_t i = 0;
+    while (i < buf.size()) {
+        size_t j = i;
+        fx_t acc = 0;
+        bool is_input = false;
+        for (; j < buf.size() && buf[j].k == buf[i].k; ++j) {
+            acc += fx_from(buf[j].v);
+            is_input |= buf[j].tag;
+        }
+        const double y = fx_to(acc);
+        if (is_input) {
+            auto it = std::lower_bound(src.begin(), src.end(), buf[i].k, [](const KV& a, det_t k) { return a.k < k; });
+            dacc += fx_from(y * it->v);
+        }
+        if (std::fabs(y) > cut) fin.push_back({buf[i].k, y});
+        i = j;
+    }
+    top_cap_sorted(fin, (size_t)out_cap);
+    for (size_t q = 0; q < fin.size(); ++q) {
+        out_k[q] = fin[q].k;
+        out_v[q] = fin[q].v;
+    }
+    *out_len = (int64_t)fin.size();
+    *diag_out = fx_to(dacc);
+    return 0;
+}
+
 // ---- sparse-vector operations (src/all.h) ------------------------------------------
 
 // dynVec::dot (src/all.h:317-351), exact accumulation.

@@ -8,13 +8,6 @@
 
 using namespace sfci;
 
-struct sfci_ctx {
-    Ctx c;
-};
-struct sfci_vec {
-    Vec v;
-};
-
 namespace {
 
 uint64_t binom_u64(int n, int k) {
@@ -50,10 +43,6 @@ int read_in_file(const char* path, std::vector<Entry>& out) {
     }
     fclose(fp);
     return SFCI_OK;
-}
-
-void set_err(Ctx* c, const std::string& m) {
-    if (c) c->last_error = m;
 }
 
 int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, sfci_ctx** out, std::string& err) {
@@ -207,24 +196,6 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
     return SFCI_OK;
 }
 
-template <class F>
-int guarded(Ctx* c, F&& f) {
-    try {
-        if (c) HIPCHK(hipSetDevice(c->device));
-        f();
-        return SFCI_OK;
-    } catch (const HipError& e) {
-        set_err(c, e.what());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_err(c, "host allocation failed");
-        return SFCI_E_HIP;
-    } catch (const std::exception& e) {
-        set_err(c, e.what());
-        return SFCI_E_ARG;
-    }
-}
-
 }  // namespace
 
 extern "C" {
@@ -316,6 +287,9 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     s->ms_total = t.ms_total;
     s->n_fallback = t.n_fallback;
     s->hash_bits = t.hash_bits;
+    s->n_kept = t.n_kept;
+    s->screen_eps = t.screen_eps;
+    s->ms_score = t.ms_score;
     return SFCI_OK;
 }
 
@@ -354,7 +328,9 @@ const char* sfci_kernel_name(int id) {
                                        "scan",            "k_bucket_bounds",
                                        "k_bucket_reduce", "k_reduce",
                                        "select",          "vecops",
-                                       "k_energy",        "k_part_scatter_cur<DigitHash>"};
+                                       "k_energy",        "k_part_scatter_cur<DigitHash>",
+                                       "k_mlp_fwd",       "screen_select",
+                                       "mlp_train_step"};
     return (id >= 0 && id < K_NUM) ? names[id] : "";
 }
 
@@ -469,10 +445,48 @@ int sfci_hv(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, int64_t nd, double l
     Ctx* c = &X->c;
     return guarded(c, [&] {
         double d = 0.0;
-        size_t n = dev_hv(c, in->v.k, in->v.v, (size_t)in->v.len, (size_t)nd, lambda, coeff, (size_t)out->v.cap,
-                          out->v.k, out->v.v, &d);
+        size_t n = c->screen_mode
+                       ? dev_hv_screened(c, in->v.k, in->v.v, (size_t)in->v.len, (size_t)nd, lambda, coeff,
+                                         (size_t)out->v.cap, out->v.k, out->v.v, &d)
+                       : dev_hv(c, in->v.k, in->v.v, (size_t)in->v.len, (size_t)nd, lambda, coeff,
+                                (size_t)out->v.cap, out->v.k, out->v.v, &d);
         out->v.len = (int64_t)n;
         if (diag) *diag = d;
+    });
+}
+
+int sfci_set_screen(sfci_ctx* X, int mode, sfci_mlp* net, double keep_factor, double eps_pre) {
+    if (!X || mode < SFCI_SCREEN_OFF || mode > SFCI_SCREEN_NETWORK || (mode == SFCI_SCREEN_NETWORK && !net) ||
+        (mode != SFCI_SCREEN_OFF && (!(keep_factor > 0.0) || !(eps_pre >= 0.0))))
+        return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    c->screen_mode = mode;
+    c->screen_net = mode == SFCI_SCREEN_NETWORK ? net : nullptr;
+    if (mode != SFCI_SCREEN_OFF) {
+        c->screen_keep = keep_factor;
+        c->screen_eps = eps_pre;
+    }
+    return SFCI_OK;
+}
+
+int sfci_get_screen(const sfci_ctx* X, int* mode, sfci_mlp** net, double* keep_factor, double* eps_pre) {
+    if (!X) return SFCI_E_ARG;
+    const Ctx* c = &X->c;
+    if (mode) *mode = c->screen_mode;
+    if (net) *net = const_cast<sfci_mlp*>(c->screen_net);
+    if (keep_factor) *keep_factor = c->screen_keep;
+    if (eps_pre) *eps_pre = c->screen_eps;
+    return SFCI_OK;
+}
+
+int sfci_screen_samples(sfci_ctx* X, const sfci_vec* v, double eps_pre, int64_t max_samples, uint64_t seed,
+                        sfci_screen_sample_t* out, int64_t* n_out) {
+    if (!X || !v || !out || !n_out || max_samples < 0 || !(eps_pre >= 0.0) || v->v.ctx != &X->c) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    *n_out = 0;
+    return guarded(c, [&] {
+        *n_out = (int64_t)dev_screen_samples(c, v->v.k, v->v.v, (size_t)v->v.len, eps_pre, (size_t)max_samples, seed,
+                                             out);
     });
 }
 

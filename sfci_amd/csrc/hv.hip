@@ -442,6 +442,70 @@ __device__ __forceinline__ float cand_val<float>(double hij, double temp) {
     return (float)hij * (float)temp;
 }
 
+// Where generated candidates go.  Plain: every record at its planned position.  Screened
+// (one pass over all target rows): only records with |value| > eps (diagonal tags always),
+// appended to a per-workgroup LDS list that is copied out after each source at an offset
+// reserved with one global atomic; a full LDS list spills single records to the global list
+// directly.  The screened list's order varies between runs; its content does not.
+constexpr uint32_t SCR_LDS = 768;
+struct ScreenOut {
+    RecS* out;
+    unsigned long long* count;
+    uint64_t cap;
+    float eps;
+};
+template <typename T>
+struct PlainSink {
+    typename RecOf<T>::type* out;
+    uint64_t cap;
+    __device__ __forceinline__ bool put(uint64_t idx, uint64_t key, T val, uint32_t) {
+        return put_rec(out, cap, idx, make_rec(key, val));
+    }
+};
+struct ScreenSink {
+    ScreenOut so;
+    RecS* lds;
+    uint32_t* lds_n;
+    __device__ __forceinline__ bool put(uint64_t, uint64_t key, float val, uint32_t cls) {
+        if (!(key & 1ull) && !(fabsf(val) > so.eps)) return true;
+        const RecS r{key, val, cls};
+        const uint32_t i = atomicAdd(lds_n, 1u);
+        if (i < SCR_LDS) {
+            lds[i] = r;
+            return true;
+        }
+        const unsigned long long gi = atomicAdd(so.count, 1ull);
+        if (gi >= so.cap) return false;
+        so.out[gi] = r;
+        return true;
+    }
+    // block-uniform: copy the LDS list out and empty it
+    __device__ __forceinline__ bool flush(uint64_t* s_base) {
+        __syncthreads();
+        const uint32_t nl = min(*lds_n, SCR_LDS);
+        if (threadIdx.x == 0) *s_base = nl ? atomicAdd(so.count, (unsigned long long)nl) : 0ull;
+        __syncthreads();
+        bool ok = true;
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
+            const uint64_t gi = *s_base + i;
+            if (gi < so.cap) so.out[gi] = lds[i];
+            else ok = false;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *lds_n = 0;
+        __syncthreads();
+        return ok;
+    }
+};
+template <typename T, bool SCR>
+struct SinkOf {
+    using type = PlainSink<T>;
+};
+template <>
+struct SinkOf<float, true> {
+    using type = ScreenSink;
+};
+
 constexpr int B2_UNROLL = 4;
 
 // Single-excitation elements of one source, n_items of them, each evaluated by a group of
@@ -450,10 +514,9 @@ constexpr int B2_UNROLL = 4;
 // form, the limbs are combined exactly across the group, and the group's first lane adds
 // h_mn and writes the record.  Same value as single_elem (an exact sum is order-free).
 // item(t, n, m, key) gives item t's annihilated/created spin orbitals and record key.
-template <typename T, typename Item>
+template <typename T, typename Item, typename Sink>
 __device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, uint32_t n_items, int lg, double temp,
-                                             uint64_t pos, Item item, typename RecOf<T>::type* __restrict__ out,
-                                             uint64_t out_cap, bool& range_bad, bool& wbad) {
+                                             uint64_t pos, Item item, Sink& sink, bool& range_bad, bool& wbad) {
     const int L = 1 << lg;
     const int gl = threadIdx.x & (L - 1);
     const uint32_t grp = threadIdx.x >> lg, n_grp = SFCI_BLOCK >> lg;
@@ -485,7 +548,7 @@ __device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, u
             const double hij = ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
             const T val = cand_val<T>(hij, temp);
             range_bad |= !fx_in_range((double)val);
-            wbad |= !put_rec(out, out_cap, pos + t, make_rec(key, val));
+            wbad |= !sink.put(pos + t, key, val, 1u);
         }
     }
 }
@@ -501,16 +564,29 @@ __device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, u
 // element with the O(1) double-excitation element.  Records are written at `pos`, which
 // starts at j * n_conn (single pass, off == nullptr) or off[j] (per-source exclusive scan
 // of the pass counts), so the layout is deterministic.
-template <typename T>
-__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
-                                                         const double* __restrict__ sv, size_t n_src, double lambda,
-                                                         double coeff, uint64_t row_lo, uint64_t row_hi,
-                                                         const uint64_t* __restrict__ off,
-                                                         const uint16_t* __restrict__ sdec_g,
-                                                         typename RecOf<T>::type* __restrict__ out,
-                                                         uint64_t out_cap, double* __restrict__ dterm,
-                                                         uint32_t* flags) {
+// With SCR (float only) the candidates go through ScreenSink (screen.eps prefilter) instead
+// of the planned positions; it then runs as one pass over all target rows.
+template <typename T, bool SCR>
+__device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* __restrict__ sk,
+                                              const double* __restrict__ sv, size_t n_src, double lambda, double coeff,
+                                              uint64_t row_lo, uint64_t row_hi, const uint64_t* __restrict__ off,
+                                              const uint16_t* __restrict__ sdec_g,
+                                              typename RecOf<T>::type* __restrict__ out, uint64_t out_cap,
+                                              double* __restrict__ dterm, uint32_t* flags, const ScreenOut& screen) {
     __shared__ SrcLds s;
+    __shared__ RecS s_scr[SCR ? SCR_LDS : 1];
+    __shared__ uint32_t s_scr_n;
+    __shared__ uint64_t s_scr_base;
+    typename SinkOf<T, SCR>::type sink;
+    if constexpr (SCR) {
+        sink.so = screen;
+        sink.lds = s_scr;
+        sink.lds_n = &s_scr_n;
+        if (threadIdx.x == 0) s_scr_n = 0;
+    } else {
+        sink.out = out;
+        sink.cap = out_cap;
+    }
     __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
     __shared__ uint64_t s_binom[LDS_BINOM];
@@ -533,7 +609,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
             // A0
             if (tid == 0) {
-                wbad |= !put_rec(out, out_cap, pos, make_rec((rowbase + s.rank[1]) << 1 | 1ull, tag_val<T>((uint32_t)j)));
+                wbad |= !sink.put(pos, (rowbase + s.rank[1]) << 1 | 1ull, tag_val<T>((uint32_t)j), 0u);
                 dterm[j] = s.diag_val;
             }
             pos += 1;
@@ -545,7 +621,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                                 m = 2 * s.vir[1][dd >> 8] + 1;
                                 key = (rowbase + s.srank[1][f]) << 1;
                             },
-                            out, out_cap, range_bad, wbad);
+                            sink, range_bad, wbad);
             pos += nsa;
             // A2
             DivIter it(tid, SFCI_BLOCK, nvp);
@@ -556,7 +632,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
                 T val = cand_val<T>(double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                wbad |= !put_rec(out, out_cap, pos + f, make_rec((rowbase + colex_rank(st, H.binom)) << 1, val));
+                wbad |= !sink.put(pos + f, (rowbase + colex_rank(st, H.binom)) << 1, val, 2u);
             }
             pos += ndaa;
         }
@@ -569,7 +645,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                             m = 2 * (int)((aw >> 15) & 31u);
                             key = (s.akey[f] + s.rank[1]) << 1;
                         },
-                        out, out_cap, range_bad, wbad);
+                        sink, range_bad, wbad);
         pos += na;
         // B2: <D''|H|D> = (-1)^p (a i|b j), p = parity of the spin-0 single's sign, of the
         // spin-1 single's sign, and of whether 2i and 2a lie between 2j+1 and 2b+1 (moving
@@ -607,7 +683,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     if (f < nB) {
                         T val = cand_val<T>(neg[u] ? -g[u] : g[u], temp);
                         range_bad |= !fx_in_range((double)val);
-                        wbad |= !put_rec(out, out_cap, pos + f, make_rec((s.akey[qa[u]] + s.srank[1][qb[u]]) << 1, val));
+                        wbad |= !sink.put(pos + f, (s.akey[qa[u]] + s.srank[1][qb[u]]) << 1, val, 2u);
                     }
                 }
             }
@@ -646,7 +722,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                 }
                 it.next();
                 if (all_in) {
-                    if (have) wbad |= !put_rec(out, out_cap, pos + f, make_rec(key, val));
+                    if (have) wbad |= !sink.put(pos + f, key, val, 2u);
                     continue;
                 }
                 uint64_t bal = __ballot(have);
@@ -658,14 +734,40 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                     if (i2 < w) pre += s_wcnt[i2];
                     tot += s_wcnt[i2];
                 }
-                if (have) wbad |= !put_rec(out, out_cap, pos + pre + my, make_rec(key, val));
+                if (have) wbad |= !sink.put(pos + pre + my, key, val, 2u);
                 pos += tot;
                 __syncthreads();
             }
         }
         __syncthreads();
+        if constexpr (SCR) wbad |= !sink.flush(&s_scr_base);
     }
     if (range_bad) atomicOr(flags, FLAG_RANGE);
+    if (wbad) atomicOr(flags, FLAG_GEN_OVERFLOW);
+}
+
+template <typename T>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
+                                                         const double* __restrict__ sv, size_t n_src, double lambda,
+                                                         double coeff, uint64_t row_lo, uint64_t row_hi,
+                                                         const uint64_t* __restrict__ off,
+                                                         const uint16_t* __restrict__ sdec_g,
+                                                         typename RecOf<T>::type* __restrict__ out,
+                                                         uint64_t out_cap, double* __restrict__ dterm,
+                                                         uint32_t* flags) {
+    generate_body<T, false>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, out, out_cap, dterm, flags,
+                            ScreenOut{});
+}
+
+// all target rows, screened (mixed-precision values)
+__global__ __launch_bounds__(SFCI_BLOCK) void k_generate_screened(HamDev Hg, const uint64_t* __restrict__ sk,
+                                                                  const double* __restrict__ sv, size_t n_src,
+                                                                  double lambda, double coeff,
+                                                                  const uint16_t* __restrict__ sdec_g,
+                                                                  double* __restrict__ dterm, uint32_t* flags,
+                                                                  ScreenOut screen) {
+    generate_body<float, true>(Hg, sk, sv, n_src, lambda, coeff, 0, Hg.dim, nullptr, sdec_g, nullptr, 0, dterm, flags,
+                               screen);
 }
 
 // ---- energy readout: calc_energy (src/functions.cpp:1272-1313) ---------------------
@@ -1876,6 +1978,379 @@ void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, doubl
     *all = HvAll{};
     if (c->precision == 1) dev_hv_t<float>(c, ik, iv, n_in, n_in, lambda, coeff, 0, nullptr, nullptr, diag, all);
     else dev_hv_t<double>(c, ik, iv, n_in, n_in, lambda, coeff, 0, nullptr, nullptr, diag, all);
+}
+
+
+// ---------------------------------------------------------------------------------
+// Screened H.v.  One pass over all target rows:
+//   generate  every connection evaluated as in the mixed-precision path, only those with
+//             |H_ij coeff c_j| > eps (and every diagonal tag) written (k_generate<float, true>)
+//   score     the network's prediction of log|y_t| (mode 2) or |value| (mode 1)
+//   select    the candidates whose score reaches the keep-th largest (ties kept), by two
+//             16-bit histogram rounds over an order-preserving integer form of the score
+//   merge     key sort + the exact segmented reduction, then the cut, cap and key order
+// The kept set depends only on the candidates' contents, and the merge is exact, so the
+// result does not depend on the order in which the generation appended them.
+
+// order-preserving map of a float to uint32 (larger float -> larger integer)
+__device__ __forceinline__ uint32_t score_bits(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_score_mag(const RecS* __restrict__ r, size_t n, float* __restrict__ score) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const RecS x = r[i];
+        score[i] = (x.key & 1ull) ? INFINITY : fabsf(x.val);
+    }
+}
+
+// hist[b] += 1 for every score whose high 16 bits are b (hi < 0), or, among scores whose
+// high bits equal hi, whose low 16 bits are b
+__global__ void k_score_hist(const float* __restrict__ score, size_t n, int hi, uint32_t* __restrict__ hist) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t u = score_bits(score[i]);
+        if (hi < 0) atomicAdd(hist + (u >> 16), 1u);
+        else if ((int)(u >> 16) == hi) atomicAdd(hist + (u & 0xffffu), 1u);
+    }
+}
+
+// kept records (score >= threshold) as split key / value arrays, appended with one atomic
+// per wavefront
+__global__ __launch_bounds__(SFCI_BLOCK) void k_keep_compact(const RecS* __restrict__ r, const float* __restrict__ score,
+                                                             size_t n, uint32_t thr, uint64_t* __restrict__ kout,
+                                                             float* __restrict__ vout,
+                                                             unsigned long long* __restrict__ count) {
+    const int lane = threadIdx.x & (SFCI_WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const size_t i = i0 + threadIdx.x;
+        const bool keep = i < n && score_bits(score[i]) >= thr;
+        const uint64_t bal = __ballot(keep);
+        unsigned long long base = 0;
+        if (lane == 0 && bal) base = atomicAdd(count, (unsigned long long)popc64(bal));
+        base = __shfl(base, 0, SFCI_WAVE);
+        if (keep) {
+            const uint64_t at = base + popc64(bal & lt);
+            kout[at] = r[i].key;
+            vout[at] = r[i].val;
+        }
+    }
+}
+
+// all records, split (nothing screened out)
+__global__ void k_recs_split(const RecS* __restrict__ r, size_t n, uint64_t* __restrict__ kout,
+                             float* __restrict__ vout) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        kout[i] = r[i].key;
+        vout[i] = r[i].val;
+    }
+}
+
+namespace {
+
+// Screened generation into c->cand_k[0]; returns the record count.  A list that would not
+// fit the workspace raises eps fourfold and runs again.
+size_t screened_generate(Ctx* c, const uint64_t* sk, const double* sv, size_t ns, double lambda, double coeff,
+                         double* dterm, double eps, RecS** recs_out, double* eps_used) {
+    const HamDev& H = c->ham;
+    const uint64_t total = (uint64_t)ns * H.n_conn;
+    const size_t cap = (size_t)std::max<uint64_t>(1, std::min<uint64_t>(total, c->ws_limit / (2 * sizeof(RecS))));
+    RecS* recs = (RecS*)c->cand_k[0].get<uint8_t>(cap * sizeof(RecS));
+    unsigned long long* cnt = (unsigned long long*)c->d_counter + 2;
+    for (int attempt = 0;; ++attempt) {
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+        int tg = kt_begin(c, K_GENERATE, 16.0 * ns);
+        k_generate_screened<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(
+            H, sk, sv, ns, lambda, coeff, c->d_sdec, dterm, c->d_flags, ScreenOut{recs, cnt, (uint64_t)cap, (float)eps});
+        kt_end(c, tg);
+        HIPCHK(hipGetLastError());
+        unsigned long long m = 0;
+        HIPCHK(hipMemcpyAsync(&m, cnt, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const uint32_t fl = flags_read_clear(c);
+        if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+        if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
+        if (!(fl & FLAG_GEN_OVERFLOW) && m <= cap) {
+            *recs_out = recs;
+            *eps_used = eps;
+            return (size_t)m;
+        }
+        if (attempt >= 8) throw HipError("screened candidates exceed the workspace at every prefilter tried", -4);
+        eps = eps > 0.0 ? eps * 4.0 : 1e-12;
+    }
+}
+
+// the largest threshold u with #(score_bits >= u) >= keep, by two histogram rounds
+uint32_t score_threshold(Ctx* c, const float* score, size_t m, size_t keep) {
+    uint32_t* hist = c->scratch_hist.get<uint32_t>(65536);
+    std::vector<uint32_t> h(65536);
+    const int g = grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16);
+    uint64_t above = 0;
+    uint32_t hi = 0;
+    for (int round = 0; round < 2; ++round) {
+        HIPCHK(hipMemsetAsync(hist, 0, 65536 * sizeof(uint32_t), c->stream));
+        k_score_hist<<<g, SFCI_BLOCK, 0, c->stream>>>(score, m, round == 0 ? -1 : (int)hi, hist);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h.data(), hist, 65536 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int b = 65535;
+        for (; b > 0; --b) {
+            if (above + h[b] >= keep) break;
+            above += h[b];
+        }
+        if (round == 0) hi = (uint32_t)b;
+        else return hi << 16 | (uint32_t)b;
+    }
+    return 0;
+}
+
+}  // namespace
+
+size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda,
+                       double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag) {
+    const HamDev& H = c->ham;
+    auto& st = c->st;
+    st = {};
+    Timer total(c->stream);
+    *diag = 0.0;
+    if (n_in == 0) return 0;
+    if (c->screen_mode == 2 && !c->screen_net) throw HipError("network screening without a network", -1);
+    const uint64_t* sk = ik;
+    const double* sv = iv;
+    size_t ns = n_in;
+    if (n_in > nd) {
+        uint64_t* tk = c->src_k.get<uint64_t>(nd);
+        double* tv = c->src_v.get<double>(nd);
+        ns = select_top(c, ik, iv, n_in, nd, tk, tv);
+        sk = tk;
+        sv = tv;
+    }
+    st.n_src = ns;
+    st.n_passes = 1;
+    double* dterm = c->dterm.get<double>(ns);
+    // 1. screened generation
+    RecS* recs = nullptr;
+    double eps_used = 0.0;
+    size_t m = 0;
+    {
+        Timer t(c->stream);
+        m = screened_generate(c, sk, sv, ns, lambda, coeff, dterm, c->screen_eps, &recs, &eps_used);
+        st.ms_generate += t.stop();
+    }
+    st.n_candidates = m;
+    st.screen_eps = eps_used;
+    // 2. score and select
+    const size_t keep = std::max<size_t>((size_t)(c->screen_keep * (double)nd), ns + 1);
+    // keys + values of the kept records, and their sort scratch, in the second buffer
+    const size_t m_alloc = std::max<size_t>(m, 1);
+    uint8_t* buf = c->cand_k[1].get<uint8_t>(m_alloc * (2 * (sizeof(uint64_t) + sizeof(float))) + m_alloc * sizeof(float));
+    uint64_t* kd = (uint64_t*)buf;
+    uint64_t* kt = kd + m_alloc;
+    float* vd = (float*)(kt + m_alloc);
+    float* vt = vd + m_alloc;
+    float* score = vt + m_alloc;
+    size_t kept = m;
+    if (m > keep) {
+        Timer t(c->stream);
+        if (c->screen_mode == 2) {
+            mlp_score_records(c, c->screen_net, recs, m, score);
+        } else {
+            k_score_mag<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, score);
+        }
+        HIPCHK(hipGetLastError());
+        st.ms_score += t.stop();
+        Timer t2(c->stream);
+        int ts = kt_begin(c, K_SCREEN_SELECT, 20.0 * m + 12.0 * keep);
+        const uint32_t thr = score_threshold(c, score, m, keep);
+        unsigned long long* cnt = (unsigned long long*)c->d_counter + 3;
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+        k_keep_compact<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
+            recs, score, m, thr, kd, vd, cnt);
+        HIPCHK(hipGetLastError());
+        unsigned long long kc = 0;
+        HIPCHK(hipMemcpyAsync(&kc, cnt, sizeof(kc), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        kt_end(c, ts);
+        kept = (size_t)kc;
+        st.ms_select += t2.stop();
+    } else if (m > 0) {
+        k_recs_split<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, kd, vd);
+        HIPCHK(hipGetLastError());
+    }
+    st.n_kept = kept;
+    // 3. exact merge of the kept candidates
+    size_t stage_cap = std::max<size_t>(std::max<size_t>(out_cap + (out_cap >> 2), kept + 1), (size_t)1 << 16);
+    if (c->stage_k.bytes / sizeof(uint64_t) > stage_cap) stage_cap = c->stage_k.bytes / sizeof(uint64_t);
+    uint64_t* stk = c->stage_k.get<uint64_t>(stage_cap);
+    double* stv = c->stage_v.get<double>(stage_cap);
+    const double cut_eff = out_cap ? c->cut : INFINITY;
+    unsigned long long* d_out_count = (unsigned long long*)c->d_counter;
+    unsigned long long* d_uniq = d_out_count + 1;
+    HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
+    uint64_t cnts[2] = {0, 0};
+    if (kept > 0) {
+        Timer ts(c->stream);
+        radix_sort_kv<float>(c, &kd, &vd, kt, vt, kept, bits_for(2 * H.dim * H.dim));
+        st.ms_sort += ts.stop();
+        Timer t(c->stream);
+        acc_clear(c, 1);
+        int grid = (int)std::min<size_t>((kept + RED_TILE - 1) / RED_TILE, 256 * 8);
+        int tr = kt_begin(c, K_REDUCE, 12.0 * kept);
+        k_reduce<float><<<grid, SFCI_BLOCK, 0, c->stream>>>(H, kd, vd, kept, 0, sk, sv, dterm, ns, cut_eff, stk, stv,
+                                                            d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
+        kt_end(c, tr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
+        *diag = acc_read(c, 1);
+        st.ms_reduce += t.stop();
+    }
+    const uint32_t fl = flags_read_clear(c);
+    if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
+    if (fl & FLAG_OUT_OVERFLOW) throw HipError("merged-output staging overflow", -3);
+    if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
+    st.n_unique = cnts[1];
+    // 4. output: determinant keys, top out_cap by |y|, key order
+    const size_t mo = (size_t)cnts[0];
+    if (mo > 0) {
+        k_dense_to_det<<<grid_for(c, (mo + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(H, stk, mo);
+        HIPCHK(hipGetLastError());
+    }
+    size_t n_out = 0;
+    if (out_cap > 0 && mo > 0) {
+        Timer t(c->stream);
+        uint64_t* sel_k = c->tmp_k[0].get<uint64_t>(mo);
+        double* sel_v = c->tmp_v[0].get<double>(mo);
+        n_out = select_top(c, stk, stv, mo, out_cap, sel_k, sel_v);
+        uint64_t* tk = c->tmp_k[1].get<uint64_t>(n_out);
+        double* tv = c->tmp_v[1].get<double>(n_out);
+        uint64_t* kk = sel_k;
+        double* vv = sel_v;
+        radix_sort_pairs(c, &kk, &vv, tk, tv, n_out, c->n_so);
+        dev_copy(c, ok, ov, kk, vv, n_out);
+        st.ms_select += t.stop();
+    }
+    st.n_out = n_out;
+    kt_resolve(c);
+    st.ms_total = total.stop();
+    return n_out;
+}
+
+// ---- training samples ---------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64s(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t rec_hash(const RecS& r, uint64_t seed) {
+    return mix64s(seed ^ mix64s(r.key ^ ((uint64_t)__float_as_uint(r.val) << 32 | r.cls)));
+}
+
+// (hash, record index) of the off-diagonal records whose hash is below thr
+__global__ __launch_bounds__(SFCI_BLOCK) void k_sample_pick(const RecS* __restrict__ r, size_t n, uint64_t seed,
+                                                            uint64_t thr, uint64_t* __restrict__ hk,
+                                                            double* __restrict__ idx,
+                                                            unsigned long long* __restrict__ count) {
+    const int lane = threadIdx.x & (SFCI_WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const size_t i = i0 + threadIdx.x;
+        uint64_t h = 0;
+        bool take = false;
+        if (i < n) {
+            const RecS x = r[i];
+            h = rec_hash(x, seed);
+            take = !(x.key & 1ull) && h < thr;
+        }
+        const uint64_t bal = __ballot(take);
+        unsigned long long base = 0;
+        if (lane == 0 && bal) base = atomicAdd(count, (unsigned long long)popc64(bal));
+        base = __shfl(base, 0, SFCI_WAVE);
+        if (take) {
+            const uint64_t at = base + popc64(bal & lt);
+            hk[at] = h;
+            const uint64_t ii = i;
+            double d;
+            __builtin_memcpy(&d, &ii, 8);
+            idx[at] = d;
+        }
+    }
+}
+
+// samples in hash order, labelled by a binary search of the target in the full product
+__global__ void k_sample_fill(HamDev H, const RecS* __restrict__ r, const double* __restrict__ idx, size_t n,
+                              const uint64_t* __restrict__ yk, const double* __restrict__ yv, size_t ny,
+                              sfci_screen_sample_t* __restrict__ out, uint32_t* flags) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t ii;
+        const double d = idx[i];
+        __builtin_memcpy(&ii, &d, 8);
+        const RecS x = r[ii];
+        const uint64_t det = det_of_dense(x.key >> 1, H);
+        size_t lo = 0, hi = ny;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if (yk[mid] < det) lo = mid + 1;
+            else hi = mid;
+        }
+        double y = 0.0;
+        if (lo < ny && yk[lo] == det) y = yv[lo];
+        else atomicOr(flags, FLAG_BAD_INPUT);
+        sfci_screen_sample_t s;
+        s.det = det;
+        s.logv = plog2(x.val);
+        s.label = (float)log(y != 0.0 ? fabs(y) : 1e-30);
+        s.cls = x.cls;
+        s.pad = 0;
+        out[i] = s;
+    }
+}
+
+size_t dev_screen_samples(Ctx* c, const uint64_t* vk, const double* vv, size_t n, double eps, size_t max_samples,
+                          uint64_t seed, sfci_screen_sample_t* out) {
+    if (n == 0 || max_samples == 0) return 0;
+    // full uncut product: every distinct target, key-sorted (library buffers)
+    HvAll all;
+    double d = 0.0;
+    dev_hv_all(c, vk, vv, n, 0.0, 1.0, &d, &all);
+    // screened candidates of the same product
+    double* dterm = c->dterm.get<double>(n);
+    RecS* recs = nullptr;
+    double eps_used = 0.0;
+    const size_t m = screened_generate(c, vk, vv, n, 0.0, 1.0, dterm, eps, &recs, &eps_used);
+    if (eps_used != eps) throw HipError("sample candidates exceed the workspace at this prefilter", -4);
+    if (m == 0) return 0;
+    const double frac = std::min(1.0, 1.25 * (double)max_samples / (double)m);
+    const uint64_t thr = frac >= 1.0 ? ~0ull : (uint64_t)(frac * 18446744073709551615.0);
+    uint64_t* hk = c->sel_a.get<uint64_t>(2 * m);
+    double* hx = (double*)(hk + m);
+    uint64_t* hk2 = c->sel_b.get<uint64_t>(2 * m);
+    double* hx2 = (double*)(hk2 + m);
+    unsigned long long* cnt = (unsigned long long*)c->d_counter + 3;
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    k_sample_pick<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, seed, thr,
+                                                                                                   hk, hx, cnt);
+    HIPCHK(hipGetLastError());
+    unsigned long long np = 0;
+    HIPCHK(hipMemcpyAsync(&np, cnt, sizeof(np), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (np == 0) return 0;
+    radix_sort_pairs(c, &hk, &hx, hk2, hx2, (size_t)np, 64);
+    const size_t take = std::min<size_t>((size_t)np, max_samples);
+    DevBuf ob;
+    sfci_screen_sample_t* dout = ob.get<sfci_screen_sample_t>(take);
+    k_sample_fill<<<grid_for(c, (take + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(
+        c->ham, recs, hx, take, all.k, all.v, all.n, dout, c->d_flags);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, take * sizeof(sfci_screen_sample_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ob.release();
+    const uint32_t fl = flags_read_clear(c);
+    if (fl & FLAG_BAD_INPUT) throw HipError("a screened candidate's target is missing from the full product", -1);
+    return take;
 }
 
 }  // namespace sfci

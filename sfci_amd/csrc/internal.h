@@ -4,11 +4,15 @@
 
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "common.h"
+#include "../../include/sfci_gpu.h"
+
+struct sfci_mlp;
 
 namespace sfci {
 
@@ -168,6 +172,9 @@ enum KernelId : int {
     K_VECOPS,
     K_ENERGY,
     K_PART_SCATTER1,   // k_part_scatter_cur<DigitHash>: first-level partition with region cursors
+    K_MLP_SCORE,       // k_mlp_fwd on screened candidates (fused three-layer forward, BF16 MFMA)
+    K_SCREEN_SELECT,   // score histograms and the kept-candidate compaction of the screened H.v
+    K_MLP_TRAIN,       // one training step of the screening network (forward, backward, Adam)
     K_NUM
 };
 
@@ -223,6 +230,11 @@ struct Ctx {
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
     int merge_hash_bits = 0;      // distinct targets per candidate, from the last H.v (sizes the hash buckets)
+    // candidate screening of sfci_hv (0 off, 1 by |value|, 2 by the network)
+    int screen_mode = 0;
+    const sfci_mlp* screen_net = nullptr;
+    double screen_keep = 3.0;        // kept candidates per input determinant
+    double screen_eps = 1e-7;        // prefilter |H_ij coeff c_j| > eps
     unsigned long long* d_acc = nullptr;   // 8 x 2 words of fx128 accumulators
     uint32_t* d_flags = nullptr;
     uint64_t* d_counter = nullptr;         // 8 general counters
@@ -233,6 +245,8 @@ struct Ctx {
         uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0, n_fallback = 0;
         int hash_bits = 0;
         double ms_generate = 0, ms_sort = 0, ms_reduce = 0, ms_select = 0, ms_total = 0;
+        uint64_t n_kept = 0;         // screened: candidates kept for the merge
+        double screen_eps = 0.0, ms_score = 0.0;
     } st;
 };
 
@@ -300,6 +314,50 @@ size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_
 // value 0 and dterm[j] = H_jj.  Buffers are the library's candidate buffers.
 void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64** rec, const double** dterm);
 
+// ---- candidate screening (screen.hip / mlp.hip) ------------------------------------
+// A screened candidate: target (dense index << 1 | is_diagonal, as in Rec64), its FP32
+// value H_ij * coeff * c_j (the source index bits for a diagonal tag), and the excitation
+// rank of the connection (0 diagonal, 1 single, 2 double).
+struct alignas(16) RecS {
+    uint64_t key;
+    float val;
+    uint32_t cls;
+};
+
+// Determinant bits of a dense target index rank0 * dim + rank1 (spin-0 string on the even
+// spin orbitals, spin-1 string on the odd ones).
+__device__ __forceinline__ uint64_t det_of_dense(uint64_t idx, const HamDev& H) {
+    const uint64_t r0 = idx / H.dim, r1 = idx - r0 * H.dim;
+    const uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom), s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
+    uint64_t d = 0;
+    for (int i = 0; i < H.n_orb; ++i)
+        d |= (uint64_t)((s0 >> i) & 1u) << (2 * i) | (uint64_t)((s1 >> i) & 1u) << (2 * i + 1);
+    return d;
+}
+// Screening feature of a candidate value: log2|v| piecewise linear in the mantissa,
+// exponent + mantissa fraction of the float (one IEEE rounding; -150 for zero).
+__host__ __device__ __forceinline__ float plog2(float v) {
+    uint32_t b;
+    __builtin_memcpy(&b, &v, 4);
+    b &= 0x7fffffffu;
+    if (b == 0) return -150.0f;
+    return (float)((int)(b >> 23) - 127) + (float)(b & 0x7fffffu) * (1.0f / 8388608.0f);
+}
+// Screened H.v (Ctx::screen_mode != 0) with the semantics of sfci_hv on the kept candidates.
+size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda,
+                       double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag);
+// Training samples from the candidates of v (see sfci_screen_samples); returns the count.
+size_t dev_screen_samples(Ctx* c, const uint64_t* vk, const double* vv, size_t n, double eps, size_t max_samples,
+                          uint64_t seed, sfci_screen_sample_t* out);
+
+// Screening network 66 -> 256 -> 256 -> 1 (SiLU), BF16 operands, FP32 accumulation.
+constexpr int MLP_IN = 66, MLP_INP = 80, MLP_HID = 256;
+constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_HID * MLP_IN, MLP_OFF_W2 = MLP_OFF_B1 + MLP_HID,
+              MLP_OFF_B2 = MLP_OFF_W2 + MLP_HID * MLP_HID, MLP_OFF_W3 = MLP_OFF_B2 + MLP_HID,
+              MLP_OFF_B3 = MLP_OFF_W3 + MLP_HID, MLP_NPARAM = MLP_OFF_B3 + 1;
+// score[i] = network output for record i's features; diagonal tags get +inf.
+void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score);
+
 // kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
 // stream and folds the elapsed times into c->kclock.stat.
 int kt_begin(Ctx* c, int id, double bytes);
@@ -318,6 +376,42 @@ static inline int bits_for(uint64_t max_value_exclusive) {
     int b = 0;
     while (b < 64 && ((uint64_t)1 << b) < max_value_exclusive) ++b;
     return b;
+}
+
+}  // namespace sfci
+
+// Handles of the C-ABI (opaque to callers).
+struct sfci_ctx {
+    sfci::Ctx c;
+};
+struct sfci_vec {
+    sfci::Vec v;
+};
+
+namespace sfci {
+
+inline void set_err(Ctx* c, const std::string& m) {
+    if (c) c->last_error = m;
+}
+
+// Runs f on the context's device and turns exceptions into C-ABI return codes
+// (HipError's code, SFCI_E_HIP (-2) for a failed host allocation, SFCI_E_ARG (-1) otherwise).
+template <class F>
+int guarded(Ctx* c, F&& f) {
+    try {
+        if (c) HIPCHK(hipSetDevice(c->device));
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        set_err(c, e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(c, "host allocation failed");
+        return -2;
+    } catch (const std::exception& e) {
+        set_err(c, e.what());
+        return -1;
+    }
 }
 
 }  // namespace sfci

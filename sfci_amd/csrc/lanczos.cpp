@@ -53,6 +53,25 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
     CK(mk(nd, &v_next));
     double one = 1.0;
     CK(sfci_vec_upload(v, &init_det, &one, 1));
+    // screening: the unscreened energy readout judges every restart (no reference counterpart)
+    int smode = SFCI_SCREEN_OFF;
+    sfci_mlp* snet = nullptr;
+    double skeep = 0.0, seps = 0.0;
+    CK(sfci_get_screen(ctx, &smode, &snet, &skeep, &seps));
+    bool screening = smode != SFCI_SCREEN_OFF;
+    struct ScreenRestore {
+        sfci_ctx* ctx;
+        int mode;
+        sfci_mlp* net;
+        double keep, eps;
+        ~ScreenRestore() {
+            if (mode != SFCI_SCREEN_OFF) sfci_set_screen(ctx, mode, net, keep, eps);
+        }
+    } restore{ctx, smode, snet, skeep, seps};
+    sfci_vec* v_best = nullptr;
+    if (screening) CK(mk(nd, &v_best));
+    double best_e = INFINITY;
+    bool verifying = false;
 
     const double deltaE = p->delta_e;
     double previousE = p->e_prev_init, nearestE = -1000000.0;
@@ -135,6 +154,40 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
         CK(sfci_scale(ctx, v, 1.0 / nrm));
         double et = 0.0;
         CK(sfci_energy(ctx, v, &et));
+        // Unscreened restart: the reference's test on the Ritz value.  Screened restart: judged
+        // by the unscreened readout et; a rise rejects it (best vector back, unscreened from
+        // here on); a drop below deltaE sends the next restart unscreened, which confirms the
+        // energy (stop) or not (continue unscreened).
+        bool stop = previousE - ev[nearest] < deltaE;
+        if (screening) {
+            res->n_screened += 1;
+            stop = false;
+            if (et > best_e + 1e-9) {
+                CK(sfci_vec_copy(v, v_best));
+                et = best_e;
+                CK(sfci_set_screen(ctx, SFCI_SCREEN_OFF, nullptr, 0.0, 0.0));
+                screening = false;
+                res->n_reverts += 1;
+                if (p->verbose) printf("screened restart rejected: energy rose; continuing unscreened\n");
+            } else {
+                const bool stalled = best_e - et < deltaE;
+                best_e = et;
+                CK(sfci_vec_copy(v_best, v));
+                if (stalled) {
+                    CK(sfci_set_screen(ctx, SFCI_SCREEN_OFF, nullptr, 0.0, 0.0));
+                    screening = false;
+                    verifying = true;
+                    if (p->verbose) printf("screened solve settled; verifying with an unscreened restart\n");
+                }
+            }
+        } else if (verifying) {
+            verifying = false;
+            stop = best_e - et < deltaE;
+            if (!stop) {
+                res->n_reverts += 1;
+                if (p->verbose) printf("unscreened restart lowered the energy further; continuing unscreened\n");
+            }
+        }
         ritz[r] = ev[nearest];
         e_trunc[r] = et;
         if (p->verbose) {
@@ -145,7 +198,7 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
         }
         std::fill(KM.begin(), KM.end(), 0.0);
         for (int i = 0; i < nl; ++i) CK(sfci_vec_clear(K[i]));
-        if (previousE - ev[nearest] < deltaE) {
+        if (stop) {
             ++r;
             break;
         }

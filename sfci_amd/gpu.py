@@ -37,7 +37,8 @@ class StatsT(C.Structure):
     _fields_ = [("n_src", C.c_uint64), ("n_candidates", C.c_uint64), ("n_unique", C.c_uint64),
                 ("n_out", C.c_uint64), ("n_passes", C.c_uint64), ("ms_generate", C.c_double),
                 ("ms_sort", C.c_double), ("ms_reduce", C.c_double), ("ms_select", C.c_double),
-                ("ms_total", C.c_double), ("n_fallback", C.c_uint64), ("hash_bits", C.c_int)]
+                ("ms_total", C.c_double), ("n_fallback", C.c_uint64), ("hash_bits", C.c_int),
+                ("n_kept", C.c_uint64), ("screen_eps", C.c_double), ("ms_score", C.c_double)]
 
 
 class KernelStatT(C.Structure):
@@ -55,7 +56,7 @@ class LinearParamsT(C.Structure):
 
 class LanczosResultT(C.Structure):
     _fields_ = [("n_restarts", C.c_int), ("e_final", C.c_double), ("ms_total", C.c_double),
-                ("ms_hv", C.c_double), ("n_hv", C.c_int64)]
+                ("ms_hv", C.c_double), ("n_hv", C.c_int64), ("n_screened", C.c_int), ("n_reverts", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -129,6 +130,20 @@ def lib():
         "sfci_cisd": (ip, [vp, u64, ip, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         "sfci_excited": (ip, [vp, u64, dbl, C.POINTER(LinearParamsT), vp, C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]),
+        "sfci_mlp_create": (ip, [vp, u64, pvp]),
+        "sfci_mlp_destroy": (None, [vp]),
+        "sfci_mlp_set_params": (ip, [vp, vp]),
+        "sfci_mlp_get_params": (ip, [vp, vp]),
+        "sfci_mlp_set_norm": (ip, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "sfci_mlp_forward": (ip, [vp, vp, i64, vp]),
+        "sfci_mlp_gradient": (ip, [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]),
+        "sfci_mlp_adam_step": (ip, [vp, vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "sfci_mlp_train": (ip, [vp, vp, i64, ip, ip, C.c_float, u64, vp]),
+        "sfci_mlp_predict_samples": (ip, [vp, vp, i64, vp]),
+        "sfci_mlp_sample_features": (ip, [vp, vp, i64, vp]),
+        "sfci_set_screen": (ip, [vp, ip, vp, dbl, dbl]),
+        "sfci_screen_samples": (ip, [vp, vp, dbl, i64, u64, vp, C.POINTER(C.c_int64)]),
+        "sfci_get_screen": (ip, [vp, C.POINTER(C.c_int), pvp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -232,7 +247,7 @@ class Context:
         return {f: getattr(s, f) for f, _ in StatsT._fields_}
 
     def kernel_stats(self, reset: bool = False) -> dict:
-        n = 14
+        n = 17
         arr = (KernelStatT * n)()
         self._check(lib().sfci_kernel_stats(self._h, arr, n, 1 if reset else 0))
         return {lib().sfci_kernel_name(i).decode(): dict(ms=arr[i].ms, launches=arr[i].launches,
@@ -296,7 +311,27 @@ class Context:
                                             v_out._h if v_out is not None else None))
         n = r.n_restarts
         return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), e_final=r.e_final,
-                    ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv)
+                    ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv, n_screened=r.n_screened,
+                    n_reverts=r.n_reverts)
+
+    # --- candidate screening ---------------------------------------------------------
+    def set_screen(self, mode: int, net: Optional["Mlp"] = None, keep_factor: float = 3.0, eps_pre: float = 1e-7):
+        """mode 0 off, 1 rank by |H_ij c_j|, 2 rank by the network (SFCI_SCREEN_*)."""
+        self._check(lib().sfci_set_screen(self._h, int(mode), net._h if net is not None else None,
+                                          float(keep_factor), float(eps_pre)))
+        self._screen_net = net   # keep the network alive while the context uses it
+
+    def get_screen(self) -> dict:
+        m, n, k, e = C.c_int(), C.c_void_p(), C.c_double(), C.c_double()
+        self._check(lib().sfci_get_screen(self._h, C.byref(m), C.byref(n), C.byref(k), C.byref(e)))
+        return dict(mode=m.value, net=n.value, keep_factor=k.value, eps_pre=e.value)
+
+    def screen_samples(self, v: Vector, eps_pre: float, max_samples: int, seed: int = 0) -> np.ndarray:
+        out = np.zeros(max(int(max_samples), 1), SAMPLE_DTYPE)
+        n = C.c_int64()
+        self._check(lib().sfci_screen_samples(self._h, v._h, float(eps_pre), int(max_samples), int(seed), _ptr(out),
+                                              C.byref(n)))
+        return out[:n.value].copy()
 
     # --- linear-solver path --------------------------------------------------------
     def shrink_cut(self, dst: Vector, src: Vector, cut: float):
@@ -349,3 +384,77 @@ class Context:
         e, ec = C.c_double(), C.c_double()
         self._check(lib().sfci_excited(self._h, init_det, target_e, C.byref(p), v_out._h, C.byref(e), C.byref(ec)))
         return e.value, ec.value
+
+
+# ---- screening network ---------------------------------------------------------------
+SAMPLE_DTYPE = np.dtype([("det", "<u8"), ("logv", "<f4"), ("label", "<f4"), ("cls", "<u4"), ("pad", "<u4")])
+MLP_IN, MLP_HID = 66, 256
+MLP_NPARAM = MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID + 1
+
+
+class Mlp:
+    """The screening network (sfci_mlp) on a context's GPU."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().sfci_mlp_create(ctx._h, int(seed), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            lib().sfci_mlp_destroy(self._h)
+        self._h = None
+
+    def set_params(self, p: np.ndarray):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == MLP_NPARAM
+        self.ctx._check(lib().sfci_mlp_set_params(self._h, _ptr(a)))
+
+    def get_params(self) -> np.ndarray:
+        a = np.zeros(MLP_NPARAM, np.float32)
+        self.ctx._check(lib().sfci_mlp_get_params(self._h, _ptr(a)))
+        return a
+
+    def set_norm(self, x_shift: float, x_scale: float, y_shift: float, y_scale: float):
+        self.ctx._check(lib().sfci_mlp_set_norm(self._h, x_shift, x_scale, y_shift, y_scale))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.zeros(len(a), np.float32)
+        self.ctx._check(lib().sfci_mlp_forward(self._h, _ptr(a), len(a), _ptr(y)))
+        return y
+
+    def gradient(self, x: np.ndarray, t: np.ndarray):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        tt = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        self.ctx._check(lib().sfci_mlp_gradient(self._h, _ptr(a), _ptr(tt), len(a), _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def adam_step(self, grad: np.ndarray, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        self.ctx._check(lib().sfci_mlp_adam_step(self._h, _ptr(g), lr, beta1, beta2, eps))
+
+    def train(self, samples: np.ndarray, steps: int, batch: int = 65536, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        s = np.ascontiguousarray(samples, dtype=SAMPLE_DTYPE)
+        curve = np.zeros(max(steps, 1), np.float32)
+        self.ctx._check(lib().sfci_mlp_train(self._h, _ptr(s), len(s), int(steps), int(batch), float(lr), int(seed),
+                                             _ptr(curve)))
+        return curve[:steps]
+
+    def predict_samples(self, samples: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(samples, dtype=SAMPLE_DTYPE)
+        y = np.zeros(len(s), np.float32)
+        self.ctx._check(lib().sfci_mlp_predict_samples(self._h, _ptr(s), len(s), _ptr(y)))
+        return y
+
+    def sample_features(self, samples: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(samples, dtype=SAMPLE_DTYPE)
+        x = np.zeros((len(s), MLP_IN), np.float32)
+        self.ctx._check(lib().sfci_mlp_sample_features(self._h, _ptr(s), len(s), _ptr(x)))
+        return x
