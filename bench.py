@@ -131,6 +131,202 @@ def time_to_e(ctx, w, nd, args):
     print(json.dumps(res))
 
 
+def screener(args):
+    """Learned-screener workload (32 spatial / 64 spin orbitals, 16 electrons): training
+    samples from converged plain FP64 solves on seeds 10-19, the network trained on the GPU,
+    then three solves of seed 4 timed from the HF determinant: plain FP64, network-screened
+    mixed precision, and the same screening ranked by |H_ij c_j| alone (no network)."""
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    nd, nl, nd_tr = args.nd or 200_000, args.nl or 30, args.nd_train
+    samples, t_data = [], time.perf_counter()
+    for seed in range(10, 20):
+        w = I.make_workload("c5", data, seed=seed)
+        c = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+        v = c.vector(nd_tr)
+        r = c.ground_state(w["init_hf"], nd_tr, 10, 3, hpsi_cap=args.cap, v_out=v, e_prev_init=float("inf"))
+        s = c.screen_samples(v, args.eps, args.samples_per_seed, seed=seed)
+        samples.append(s)
+        log("training data seed %d: E %.8f after %d restarts, %d samples" % (seed, r["e_final"], r["restarts"], len(s)))
+        c.close()
+    t_data = time.perf_counter() - t_data
+    held = samples[-1]
+    train = np.concatenate(samples[:-1])
+    w = I.make_workload("c5", data)
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    net = gpu.Mlp(ctx, seed=0)
+    xs, ys = float(train["logv"].mean()), float(train["logv"].std())
+    yl, ysd = float(train["label"].mean()), float(train["label"].std())
+    net.set_norm(xs, 1.0 / ys, yl, 1.0 / ysd)
+    ctx.kernel_stats(reset=True)
+    t = time.perf_counter()
+    curve = net.train(train, steps=args.train_steps, batch=args.batch, lr=args.lr, seed=1)
+    t_train = time.perf_counter() - t
+    ks = ctx.kernel_stats()["mlp_train_step"]
+    # held-out seed: how well each score ranks candidates by the label (log|y| of the target)
+    pred = net.predict_samples(held)
+
+    def rank_corr(a, b):
+        ra, rb = np.argsort(np.argsort(a)), np.argsort(np.argsort(b))
+        return float(np.corrcoef(ra, rb)[0, 1])
+
+    def top_recall(score, frac=0.1):
+        k = max(1, int(frac * len(held)))
+        top_true = set(np.argsort(-held["label"])[:k].tolist())
+        return len(top_true & set(np.argsort(-score)[:k].tolist())) / k
+
+    quality = dict(spearman_network=rank_corr(pred, held["label"]), spearman_magnitude=rank_corr(held["logv"], held["label"]),
+                   top10pct_recall_network=top_recall(pred), top10pct_recall_magnitude=top_recall(held["logv"]),
+                   loss_first=float(curve[0]), loss_last=float(np.mean(curve[-20:])))
+    log("network: %s" % quality)
+
+    keeps = [float(k) for k in str(args.keep).split(",")]
+
+    def solve(tag, mode, mixed, keep=keeps[0]):
+        ctx.set_precision(mixed)
+        ctx.set_screen(mode, net if mode == 2 else None, keep, args.eps)
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = ctx.ground_state(w["init_hf"], nd, nl, args.restarts or 8, hpsi_cap=args.cap, e_prev_init=float("inf"),
+                             verbose=True)
+        dt = time.perf_counter() - t0
+        ctx.set_screen(0)
+        ctx.set_precision(False)
+        out = dict(seconds=dt, e_final=r["e_final"], restarts=r["restarts"], n_hv=r["n_hv"], ms_hv=r["ms_hv"],
+                   screened_restarts=r["n_screened"], rejected=r["n_reverts"], e_trunc=list(r["e_trunc"]),
+                   s_restart=[x / 1e3 for x in r["ms_restart"]])
+        log("%s: %.2f s, E %.10f, %d restarts (%d screened, %d rejected)" % (
+            tag, dt, r["e_final"], r["restarts"], r["n_screened"], r["n_reverts"]))
+        return out
+
+    plain = solve("plain fp64", 0, False)
+    # one product of a converged-solve-sized vector in each mode (diagnostics)
+    vin = ctx.vector(nd)
+    ctx.ground_state(w["init_hf"], nd, 4, 1, hpsi_cap=args.cap, v_out=vin, e_prev_init=float("inf"))
+    out = ctx.vector(args.cap)
+    single = {}
+    for tag, mode, mixed in (("plain_fp64", 0, False), ("plain_mixed", 0, True), ("magnitude", 1, True),
+                             ("network", 2, True)):
+        ctx.set_precision(mixed)
+        ctx.set_screen(mode, net if mode == 2 else None, keeps[0], args.eps)
+        ctx.hv(vin, out, nd)
+        ctx.hv(vin, out, nd)
+        st = ctx.stats()
+        ctx.set_screen(0)
+        ctx.set_precision(False)
+        single[tag] = {k: st[k] for k in ("ms_total", "ms_generate", "ms_score", "ms_select", "ms_sort", "ms_reduce",
+                                          "n_candidates", "n_kept", "n_unique", "n_out", "screen_eps")}
+        log("single H.v %s: %s" % (tag, single[tag]))
+    del vin, out
+    def annotate(x):
+        """time to E: the first restart whose (judged) energy is at or below the plain FP64
+        solve's final energy + 1e-5 Ha, against the plain solve's own wall time."""
+        x["energy_error_vs_plain_fp64"] = x["e_final"] - plain["e_final"]
+        x["within_1e-5"] = x["e_final"] <= plain["e_final"] + 1e-5
+        hit = [t for e, t in zip(x["e_trunc"], x["s_restart"]) if e <= plain["e_final"] + 1e-5]
+        x["time_to_plain_energy_s"] = hit[0] if hit else None
+        x["speedup_to_plain_energy"] = plain["seconds"] / hit[0] if hit else None
+
+    runs = {}
+    abl = [float(k) for k in str(args.ablation_keep).split(",") if k]
+    for kf in keeps:
+        runs[kf] = dict(network=solve("network-screened mixed, keep %g Nd" % kf, 2, True, kf))
+        if kf in abl:
+            runs[kf]["magnitude"] = solve("magnitude-screened mixed, keep %g Nd" % kf, 1, True, kf)
+        for x in runs[kf].values():
+            x["keep_factor"] = kf
+            annotate(x)
+    # the prefilter alone: every candidate with |H_ij c_j| > eps merged, nothing ranked
+    pre = solve("prefilter only (eps %g, nothing dropped by rank)" % args.eps, 1, True, 1e12)
+    annotate(pre)
+    nn, mag = runs[keeps[0]]["network"], runs[keeps[0]].get("magnitude")
+    res = dict(metric="time-to-E within 1e-5 Ha of the plain FP64 GPU solve (learned screener)",
+               value=nn["time_to_plain_energy_s"] if nn["time_to_plain_energy_s"] is not None else nn["seconds"],
+               unit="s", higher_is_better=False, dtype="bf16 network / f32 products / f64",
+               data="synthetic",
+               config=dict(workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, Nl=%d (reduced Nd)"
+                           % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, nl),
+                           keep_factor=keeps[0], eps_pre=args.eps, train_seeds="10-18 (19 held out)",
+                           train_nd=nd_tr, train_samples=int(len(train)), train_steps=args.train_steps,
+                           batch=args.batch),
+               plain_fp64=plain, network_screened=nn, magnitude_screened=mag, network_quality=quality,
+               by_keep_factor={str(k): v for k, v in runs.items()}, prefilter_only=pre,
+               single_hv=single,
+               training=dict(seconds=t_train, data_seconds=t_data, ms_per_step=ks["ms"] / max(1, ks["launches"])))
+    print(json.dumps(res))
+    ctx.close()
+
+
+def copy_probe():
+    """A plain copy of known size through the library (k_copy: 2^26 keys and 2^26 values,
+    1 GiB read and 1 GiB written), so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be
+    calibrated against it."""
+    from sfci_amd import gpu, integrals as I
+    import ctypes as C
+
+    w = I.make_workload("c1", os.path.join(tempfile.gettempdir(), "sfci_bench"))
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    n = 1 << 26
+    a, b = ctx.vector(n), ctx.vector(n)
+    ctx._check(gpu.lib().sfci_vec_set_len(a._h, n))
+    b.copy_from(a)
+    ctx.sync()
+    print(json.dumps(dict(copy_bytes_read=16 * n, copy_bytes_written=16 * n)))
+    ctx.close()
+
+
+def screen_hv(args):
+    """One screened H.v of a grown input on the learned-screener workload (seed 4), in each
+    scoring mode, after a short training of the network on samples of the same product:
+    reports the network kernel's MFMA throughput (165 kFLOP per scored candidate)."""
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    w = I.make_workload("c5", data)
+    nd = args.nd or 200_000
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    v = build_input(ctx, w, w["init_hf"], nd, args.cap)
+    smp = ctx.screen_samples(v, args.eps, 400_000, seed=1)
+    net = gpu.Mlp(ctx, seed=0)
+    net.set_norm(float(smp["logv"].mean()), 1.0 / float(smp["logv"].std()), float(smp["label"].mean()),
+                 1.0 / float(smp["label"].std()))
+    net.train(smp, steps=200, batch=args.batch, lr=args.lr, seed=1)
+    out = ctx.vector(args.cap)
+    res = {}
+    for tag, mode in (("plain_mixed", 0), ("magnitude", 1), ("network", 2)):
+        ctx.set_precision(True)
+        ctx.set_screen(mode, net if mode == 2 else None, float(str(args.keep).split(",")[0]), args.eps)
+        ctx.hv(v, out, nd)
+        ctx.sync()
+        ctx.kernel_stats(reset=True)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.hv(v, out, nd)
+        ctx.sync()
+        dt = (time.perf_counter() - t) / args.steps
+        st = ctx.stats()
+        ks = ctx.kernel_stats()
+        ctx.set_screen(0)
+        ctx.set_precision(False)
+        r = dict(ms=dt * 1e3, emitted=st["n_candidates"], kept=st["n_kept"], out=st["n_out"],
+                 ms_generate=st["ms_generate"], ms_score=st["ms_score"], ms_select=st["ms_select"])
+        if mode == 2:
+            k = ks["k_mlp_fwd"]
+            per = k["ms"] / max(1, k["launches"])
+            flops = 2.0 * (80 * 256 + 256 * 256) + 2.0 * 256
+            r["k_mlp_fwd"] = dict(avg_launch_ms=per, rows=st["n_candidates"],
+                                  tflops=flops * st["n_candidates"] / (per * 1e-3) / 1e12,
+                                  frac_of_bf16_peak=flops * st["n_candidates"] / (per * 1e-3) / 2.5e15)
+        res[tag] = r
+        log("%s: %s" % (tag, r))
+    print(json.dumps(dict(metric="screened H.v (learned-screener workload)", unit="ms", config=dict(
+        workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d" % (w["n_orb"], w["n_so"], w["n_elec"],
+                                                                               w["seed"], nd),
+        eps_pre=args.eps, keep_factor=args.keep), modes=res)))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,11 +343,36 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="with --time-to-e: print the solver's per-step lines")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="H.v candidate precision: fp64, or mixed (FP32 products, exact sums, FP64 vectors)")
+    ap.add_argument("--screener", action="store_true",
+                    help="learned-screener workload: train on seeds 10-19, time plain / network / magnitude solves")
+    ap.add_argument("--screen-hv", action="store_true",
+                    help="time one screened H.v per scoring mode on the learned-screener workload")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one plain device copy of 1 GiB read + 1 GiB written (counter calibration)")
+    ap.add_argument("--nl", type=int, default=None)
+    ap.add_argument("--restarts", type=int, default=None)
+    ap.add_argument("--nd-train", type=int, default=50_000)
+    ap.add_argument("--samples-per-seed", type=int, default=400_000)
+    ap.add_argument("--eps", type=float, default=1e-6, help="screening prefilter |H_ij c_j| > eps")
+    ap.add_argument("--keep", default="3", help="screening keeps keep * Nd candidates (comma list: one solve each)")
+    ap.add_argument("--ablation-keep", default="3", help="keep factors also solved with |H_ij c_j| ranking")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: multi-GPU sharding of H.v is described in DESIGN.md and not built",
               file=sys.stderr)
         sys.exit(2)
+    if args.screener:
+        screener(args)
+        return
+    if args.screen_hv:
+        screen_hv(args)
+        return
+    if args.copy_probe:
+        copy_probe()
+        return
     from sfci_amd import gpu, integrals as I
 
     data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")

@@ -173,6 +173,8 @@ typedef struct {
     double e_prev_init;  /* "previous" Ritz value before the first restart.  The reference uses 0.0
                             (src/sparse_solver.cpp:270), which ends the solve after the first restart
                             whenever the energy is positive; +INFINITY always compares two restarts. */
+    double* ms_restart;  /* optional (capacity num_restart): wall time from the start of the solve
+                            to the end of each restart (no reference counterpart) */
 } sfci_lanczos_params_t;
 
 typedef struct {

@@ -447,7 +447,8 @@ __device__ __forceinline__ float cand_val<float>(double hij, double temp) {
 // appended to a per-workgroup LDS list that is copied out after each source at an offset
 // reserved with one global atomic; a full LDS list spills single records to the global list
 // directly.  The screened list's order varies between runs; its content does not.
-constexpr uint32_t SCR_LDS = 768;
+constexpr uint32_t SCR_LDS = 256;
+constexpr uint32_t SCR_FLUSH_AT = 128;   // checkpoints copy the list out once it holds this many
 struct ScreenOut {
     RecS* out;
     unsigned long long* count;
@@ -458,7 +459,7 @@ template <typename T>
 struct PlainSink {
     typename RecOf<T>::type* out;
     uint64_t cap;
-    __device__ __forceinline__ bool put(uint64_t idx, uint64_t key, T val, uint32_t) {
+    __device__ __forceinline__ bool put(uint64_t idx, uint64_t key, T val, uint32_t, uint64_t) {
         return put_rec(out, cap, idx, make_rec(key, val));
     }
 };
@@ -466,18 +467,42 @@ struct ScreenSink {
     ScreenOut so;
     RecS* lds;
     uint32_t* lds_n;
-    __device__ __forceinline__ bool put(uint64_t, uint64_t key, float val, uint32_t cls) {
-        if (!(key & 1ull) && !(fabsf(val) > so.eps)) return true;
-        const RecS r{key, val, cls};
-        const uint32_t i = atomicAdd(lds_n, 1u);
-        if (i < SCR_LDS) {
+    // one LDS atomic per wavefront call: the first passing lane reserves the wavefront's
+    // slots (ballot over the active lanes; divergent callers are fine)
+    __device__ __forceinline__ bool put(uint64_t, uint64_t key, float val, uint32_t cls, uint64_t det) {
+        const bool pass = (key & 1ull) || fabsf(val) > so.eps;
+        const uint64_t bal = __ballot(pass);
+        if (!pass) return true;
+        const int lane = threadIdx.x & (SFCI_WAVE - 1);
+        const int leader = __ffsll((long long)bal) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(lds_n, (uint32_t)popc64(bal));
+        base = __shfl(base, leader, SFCI_WAVE);
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t i = base + (uint32_t)popc64(bal & lt);
+        const RecS r{key, det, val, cls};
+        // a full list sends the rest straight to the global list, again one atomic per wavefront
+        const bool spill = i >= SCR_LDS;
+        const uint64_t sbal = __ballot(spill);
+        if (!spill) {
             lds[i] = r;
             return true;
         }
-        const unsigned long long gi = atomicAdd(so.count, 1ull);
+        const int sl = __ffsll((long long)sbal) - 1;
+        unsigned long long gbase = 0;
+        if (lane == sl) gbase = atomicAdd(so.count, (unsigned long long)popc64(sbal));
+        gbase = __shfl(gbase, sl, SFCI_WAVE);
+        const unsigned long long gi = gbase + popc64(sbal & lt);
         if (gi >= so.cap) return false;
         so.out[gi] = r;
         return true;
+    }
+    // block-uniform: copy the list out when it holds SCR_FLUSH_AT or more
+    __device__ __forceinline__ bool checkpoint(uint64_t* s_base) {
+        __syncthreads();
+        const bool full = *lds_n >= SCR_FLUSH_AT;
+        __syncthreads();
+        return full ? flush(s_base) : true;
     }
     // block-uniform: copy the LDS list out and empty it
     __device__ __forceinline__ bool flush(uint64_t* s_base) {
@@ -548,7 +573,7 @@ __device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, u
             const double hij = ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
             const T val = cand_val<T>(hij, temp);
             range_bad |= !fx_in_range((double)val);
-            wbad |= !sink.put(pos + t, key, val, 1u);
+            wbad |= !sink.put(pos + t, key, val, 1u, s.key ^ (1ull << n) ^ (1ull << m));
         }
     }
 }
@@ -609,7 +634,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
             // A0
             if (tid == 0) {
-                wbad |= !sink.put(pos, (rowbase + s.rank[1]) << 1 | 1ull, tag_val<T>((uint32_t)j), 0u);
+                wbad |= !sink.put(pos, (rowbase + s.rank[1]) << 1 | 1ull, tag_val<T>((uint32_t)j), 0u, d);
                 dterm[j] = s.diag_val;
             }
             pos += 1;
@@ -632,9 +657,12 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                 uint32_t st = s.str[1] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
                 T val = cand_val<T>(double_elem(H, d, 2 * i + 1, 2 * jo + 1, 2 * a + 1, 2 * b + 1), temp);
                 range_bad |= !fx_in_range((double)val);
-                wbad |= !sink.put(pos + f, (rowbase + colex_rank(st, H.binom)) << 1, val, 2u);
+                wbad |= !sink.put(pos + f, (rowbase + colex_rank(st, H.binom)) << 1, val, 2u,
+                                  d ^ (1ull << (2 * i + 1)) ^ (1ull << (2 * jo + 1)) ^ (1ull << (2 * a + 1)) ^
+                                      (1ull << (2 * b + 1)));
             }
             pos += ndaa;
+            if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
         }
         // B1
         const uint32_t na = (uint32_t)s.n_alist;
@@ -647,6 +675,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                         },
                         sink, range_bad, wbad);
         pos += na;
+        if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
         // B2: <D''|H|D> = (-1)^p (a i|b j), p = parity of the spin-0 single's sign, of the
         // spin-1 single's sign, and of whether 2i and 2a lie between 2j+1 and 2b+1 (moving
         // the spin-0 electron first changes the occupation between the spin-1 pair by
@@ -656,7 +685,11 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             const uint32_t nB = na * nsa;
             const uint32_t N2 = (uint32_t)H.n_orb * (uint32_t)H.n_orb, gts = (uint32_t)H.gt_stride;
             DivIter it(tid, SFCI_BLOCK, nsa ? nsa : 1);
-            for (uint32_t f0 = tid; f0 < nB; f0 += B2_UNROLL * SFCI_BLOCK) {
+            // screened: every thread runs every round (block-uniform), so the list can be
+            // copied out between rounds; plain: each thread stops at its last item
+            for (uint32_t f00 = 0; f00 < nB; f00 += B2_UNROLL * SFCI_BLOCK) {
+                const uint32_t f0 = f00 + tid;
+                if (!SCR && f0 >= nB) break;
                 double g[B2_UNROLL];
                 uint32_t qa[B2_UNROLL], qb[B2_UNROLL];
                 bool neg[B2_UNROLL];
@@ -683,9 +716,18 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                     if (f < nB) {
                         T val = cand_val<T>(neg[u] ? -g[u] : g[u], temp);
                         range_bad |= !fx_in_range((double)val);
-                        wbad |= !sink.put(pos + f, (s.akey[qa[u]] + s.srank[1][qb[u]]) << 1, val, 2u);
+                        uint64_t tdet = 0;
+                        if constexpr (SCR) {
+                            const uint32_t aw = s.aw[qa[u]], bw = s.bw[qb[u]];
+                            const uint32_t i = (aw >> 10) & 31u, a = (aw >> 15) & 31u, jj = (bw >> 20) & 31u;
+                            const uint32_t b = ((bw >> 10) & 31u) + ((bw >> 15) & 31u) - jj;
+                            tdet = d ^ (1ull << (2 * i)) ^ (1ull << (2 * a)) ^ (1ull << (2 * jj + 1)) ^
+                                   (1ull << (2 * b + 1));
+                        }
+                        wbad |= !sink.put(pos + f, (s.akey[qa[u]] + s.srank[1][qb[u]]) << 1, val, 2u, tdet);
                     }
                 }
+                if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
             }
             pos += nB;
         }
@@ -696,13 +738,15 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             for (uint32_t f0 = 0; f0 < ndaa; f0 += SFCI_BLOCK) {
                 uint32_t f = f0 + tid;
                 bool have = false;
-                uint64_t key = 0;
+                uint64_t key = 0, tdet = 0;
                 T val = (T)0;
                 if (f < ndaa) {
                     uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
                     int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
                     int a = s.vir[0][vp & 0xff], b = s.vir[0][vp >> 8];
                     uint32_t st = s.str[0] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                    if constexpr (SCR)
+                        tdet = d ^ (1ull << (2 * i)) ^ (1ull << (2 * jo)) ^ (1ull << (2 * a)) ^ (1ull << (2 * b));
                     // the ranks of k-strings whose highest orbital is m fill [C(m,k), C(m+1,k)):
                     // targets whose class misses the pass range skip the full rank
                     const int top = max(31 - (int)__clz(st), 0);
@@ -722,7 +766,8 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                 }
                 it.next();
                 if (all_in) {
-                    if (have) wbad |= !sink.put(pos + f, key, val, 2u);
+                    if (have) wbad |= !sink.put(pos + f, key, val, 2u, tdet);
+                    if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
                     continue;
                 }
                 uint64_t bal = __ballot(have);
@@ -734,7 +779,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                     if (i2 < w) pre += s_wcnt[i2];
                     tot += s_wcnt[i2];
                 }
-                if (have) wbad |= !sink.put(pos + pre + my, key, val, 2u);
+                if (have) wbad |= !sink.put(pos + pre + my, key, val, 2u, tdet);
                 pos += tot;
                 __syncthreads();
             }
@@ -2005,14 +2050,23 @@ __global__ void k_score_mag(const RecS* __restrict__ r, size_t n, float* __restr
     }
 }
 
-// hist[b] += 1 for every score whose high 16 bits are b (hi < 0), or, among scores whose
-// high bits equal hi, whose low 16 bits are b
-__global__ void k_score_hist(const float* __restrict__ score, size_t n, int hi, uint32_t* __restrict__ hist) {
+// Histogram of bits [shift, shift + width) of score_bits over the scores whose bits above
+// shift + width equal `prefix` (all of them when shift + width == 32), privatized in LDS.
+constexpr int SEL_BITS = 11;
+__global__ __launch_bounds__(SFCI_BLOCK) void k_score_hist(const float* __restrict__ score, size_t n, int shift,
+                                                           int width, uint32_t prefix, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[1 << SEL_BITS];
+    const uint32_t nb = 1u << width;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int top = shift + width;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const uint32_t u = score_bits(score[i]);
-        if (hi < 0) atomicAdd(hist + (u >> 16), 1u);
-        else if ((int)(u >> 16) == hi) atomicAdd(hist + (u & 0xffffu), 1u);
+        if (top >= 32 || (u >> top) == prefix) atomicAdd(&h[(u >> shift) & (nb - 1)], 1u);
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (h[i]) atomicAdd(hist + i, h[i]);
 }
 
 // kept records (score >= threshold) as split key / value arrays, appended with one atomic
@@ -2082,28 +2136,30 @@ size_t screened_generate(Ctx* c, const uint64_t* sk, const double* sv, size_t ns
     }
 }
 
-// the largest threshold u with #(score_bits >= u) >= keep, by two histogram rounds
+// the largest threshold u with #(score_bits >= u) >= keep, by three histogram rounds
+// (bits 31..21, 20..10, 9..0)
 uint32_t score_threshold(Ctx* c, const float* score, size_t m, size_t keep) {
-    uint32_t* hist = c->scratch_hist.get<uint32_t>(65536);
-    std::vector<uint32_t> h(65536);
-    const int g = grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16);
+    uint32_t* hist = c->scratch_hist.get<uint32_t>(1 << SEL_BITS);
+    std::vector<uint32_t> h(1 << SEL_BITS);
+    const int g = grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 8);
     uint64_t above = 0;
-    uint32_t hi = 0;
-    for (int round = 0; round < 2; ++round) {
-        HIPCHK(hipMemsetAsync(hist, 0, 65536 * sizeof(uint32_t), c->stream));
-        k_score_hist<<<g, SFCI_BLOCK, 0, c->stream>>>(score, m, round == 0 ? -1 : (int)hi, hist);
+    uint32_t prefix = 0;
+    const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+    for (int round = 0; round < 3; ++round) {
+        const int nb = 1 << widths[round];
+        HIPCHK(hipMemsetAsync(hist, 0, nb * sizeof(uint32_t), c->stream));
+        k_score_hist<<<g, SFCI_BLOCK, 0, c->stream>>>(score, m, shifts[round], widths[round], prefix, hist);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(h.data(), hist, 65536 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h.data(), hist, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        int b = 65535;
+        int b = nb - 1;
         for (; b > 0; --b) {
             if (above + h[b] >= keep) break;
             above += h[b];
         }
-        if (round == 0) hi = (uint32_t)b;
-        else return hi << 16 | (uint32_t)b;
+        prefix = prefix << widths[round] | (uint32_t)b;
     }
-    return 0;
+    return prefix;
 }
 
 }  // namespace
@@ -2289,7 +2345,7 @@ __global__ void k_sample_fill(HamDev H, const RecS* __restrict__ r, const double
         const double d = idx[i];
         __builtin_memcpy(&ii, &d, 8);
         const RecS x = r[ii];
-        const uint64_t det = det_of_dense(x.key >> 1, H);
+        const uint64_t det = x.det;
         size_t lo = 0, hi = ny;
         while (lo < hi) {
             const size_t mid = (lo + hi) >> 1;

@@ -315,25 +315,16 @@ size_t dev_cisd(Ctx* c, uint64_t ref_det, int state_idx, size_t out_cap, uint64_
 void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64** rec, const double** dterm);
 
 // ---- candidate screening (screen.hip / mlp.hip) ------------------------------------
-// A screened candidate: target (dense index << 1 | is_diagonal, as in Rec64), its FP32
-// value H_ij * coeff * c_j (the source index bits for a diagonal tag), and the excitation
-// rank of the connection (0 diagonal, 1 single, 2 double).
-struct alignas(16) RecS {
+// A screened candidate: target (dense index << 1 | is_diagonal, as in Rec64), the target
+// determinant's bits, its FP32 value H_ij * coeff * c_j (the source index bits for a
+// diagonal tag), and the excitation rank of the connection (0 diagonal, 1 single, 2 double).
+struct alignas(8) RecS {
     uint64_t key;
+    uint64_t det;
     float val;
     uint32_t cls;
 };
 
-// Determinant bits of a dense target index rank0 * dim + rank1 (spin-0 string on the even
-// spin orbitals, spin-1 string on the odd ones).
-__device__ __forceinline__ uint64_t det_of_dense(uint64_t idx, const HamDev& H) {
-    const uint64_t r0 = idx / H.dim, r1 = idx - r0 * H.dim;
-    const uint32_t s0 = colex_unrank(r0, H.k, H.n_orb, H.binom), s1 = colex_unrank(r1, H.k, H.n_orb, H.binom);
-    uint64_t d = 0;
-    for (int i = 0; i < H.n_orb; ++i)
-        d |= (uint64_t)((s0 >> i) & 1u) << (2 * i) | (uint64_t)((s1 >> i) & 1u) << (2 * i + 1);
-    return d;
-}
 // Screening feature of a candidate value: log2|v| piecewise linear in the mantissa,
 // exponent + mantissa fraction of the float (one IEEE rounding; -150 for zero).
 __host__ __device__ __forceinline__ float plog2(float v) {

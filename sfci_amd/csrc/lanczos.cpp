@@ -72,6 +72,7 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
     if (screening) CK(mk(nd, &v_best));
     double best_e = INFINITY;
     bool verifying = false;
+    const bool judged = screening;
 
     const double deltaE = p->delta_e;
     double previousE = p->e_prev_init, nearestE = -1000000.0;
@@ -154,42 +155,51 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
         CK(sfci_scale(ctx, v, 1.0 / nrm));
         double et = 0.0;
         CK(sfci_energy(ctx, v, &et));
-        // Unscreened restart: the reference's test on the Ritz value.  Screened restart: judged
-        // by the unscreened readout et; a rise rejects it (best vector back, unscreened from
-        // here on); a drop below deltaE sends the next restart unscreened, which confirms the
-        // energy (stop) or not (continue unscreened).
+        // Unscreened restart: the reference's test on the Ritz value.  With screening, every
+        // restart is judged by the unscreened readout et and the lowest-energy vector is kept:
+        // a screened restart whose energy rises is rejected (best vector back, unscreened from
+        // here on); one that settles (drop < deltaE) hands over to an unscreened restart, which
+        // stops the solve unless it lowers the energy by deltaE or more.  The Ritz values of
+        // screened restarts are not compared with unscreened ones.
         bool stop = previousE - ev[nearest] < deltaE;
-        if (screening) {
-            res->n_screened += 1;
-            stop = false;
-            if (et > best_e + 1e-9) {
-                CK(sfci_vec_copy(v, v_best));
-                et = best_e;
-                CK(sfci_set_screen(ctx, SFCI_SCREEN_OFF, nullptr, 0.0, 0.0));
-                screening = false;
-                res->n_reverts += 1;
-                if (p->verbose) printf("screened restart rejected: energy rose; continuing unscreened\n");
-            } else {
-                const bool stalled = best_e - et < deltaE;
+        bool reset_prev = false;
+        if (judged) {
+            const double prev_best = best_e;
+            if (et < best_e) {
                 best_e = et;
                 CK(sfci_vec_copy(v_best, v));
-                if (stalled) {
+            }
+            if (screening) {
+                res->n_screened += 1;
+                stop = false;
+                if (et > prev_best + 1e-9) {
+                    CK(sfci_vec_copy(v, v_best));
+                    et = best_e;
+                    res->n_reverts += 1;
+                    if (p->verbose) printf("screened restart rejected: energy rose; continuing unscreened\n");
+                } else if (prev_best - et < deltaE) {
+                    verifying = true;
+                    if (p->verbose) printf("screened solve settled; confirming with an unscreened restart\n");
+                }
+                if (res->n_reverts > 0 || verifying) {
                     CK(sfci_set_screen(ctx, SFCI_SCREEN_OFF, nullptr, 0.0, 0.0));
                     screening = false;
-                    verifying = true;
-                    if (p->verbose) printf("screened solve settled; verifying with an unscreened restart\n");
+                    reset_prev = true;   // the first unscreened restart compares with +inf
+                    nearestE = -1000000.0;   // and takes the lowest Ritz value, not the one nearest a
+                                             // screened restart's
                 }
-            }
-        } else if (verifying) {
-            verifying = false;
-            stop = best_e - et < deltaE;
-            if (!stop) {
-                res->n_reverts += 1;
-                if (p->verbose) printf("unscreened restart lowered the energy further; continuing unscreened\n");
+            } else if (verifying) {
+                verifying = false;
+                stop = prev_best - et < deltaE;
+                if (!stop) {
+                    res->n_reverts += 1;
+                    if (p->verbose) printf("unscreened restart lowered the energy further; continuing unscreened\n");
+                }
             }
         }
         ritz[r] = ev[nearest];
         e_trunc[r] = et;
+        if (p->ms_restart) p->ms_restart[r] = ms_since(t_start);
         if (p->verbose) {
             printf("energy Psi_res after truncation:%.12f\n", et);
             printf("gsaes energy:%.12f\n", ev[nearest]);
@@ -202,8 +212,9 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
             ++r;
             break;
         }
-        previousE = ev[nearest];
+        previousE = reset_prev ? INFINITY : ev[nearest];
     }
+    if (judged) CK(sfci_vec_copy(v, v_best));   // the lowest-energy vector of the solve
     double ef = 0.0;
     CK(sfci_energy(ctx, v, &ef));
     if (p->verbose) printf("final e:%.12f\n", ef);

@@ -88,18 +88,33 @@ __device__ __forceinline__ float feature(int col, uint64_t det, int n_so, uint32
 
 enum { IN_X = 0, IN_REC = 1, IN_SAMPLE = 2 };
 
-// Fused three-layer forward over tiles of FW_TM rows: one workgroup of four wavefronts per
-// tile; wavefront w computes hidden columns [64 w, 64 w + 64) of both layers for all 64
-// rows (2 x 2 accumulator tiles), the first hidden layer passes through LDS, the output
-// dot is reduced across lanes by shuffles and across wavefronts in LDS, in fixed orders.
+// Fused three-layer forward, persistent over tiles of FW_TM rows, one workgroup of four
+// wavefronts.  Wavefront w owns hidden columns [64 w, 64 w + 64) of both layers and keeps
+// their weight fragments (B operands) in registers for the whole launch, so the weights are
+// read from memory once per workgroup.  Per tile: features -> LDS; layer 1 (A from LDS)
+// -> bias, SiLU, BF16 -> LDS; layer 2 (A from LDS) -> bias, SiLU, BF16 -> LDS; the output
+// dot over each row is summed by four threads over 64 columns each and the four partials
+// are added in a fixed order.
 // Inputs: IN_X, BF16 features [n][INP]; IN_REC, screened records; IN_SAMPLE, samples.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, HamDev H, int n_so, const void* __restrict__ in,
-                                                 size_t n, float* __restrict__ y, FwdSave sv) {
+__global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void* __restrict__ in, size_t n,
+                                                 float* __restrict__ y, FwdSave sv) {
     __shared__ __attribute__((aligned(16))) bf16_t sx[FW_TM * FW_XS];
     __shared__ __attribute__((aligned(16))) bf16_t sh[FW_TM * FW_HS];
+    __shared__ float sw3[HID];
     __shared__ float ypart[4][FW_TM];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 31, g = lane >> 5;
+    sw3[tid] = P.w3r[tid];
+    bf16x8 w1f[2][INP / 16], w2f[2][HID / 16];
+    float b1c[2], b2c[2];
+    for (int nt = 0; nt < 2; ++nt) {
+        const int col = w * 64 + nt * 32 + li;
+        for (int ks = 0; ks < INP / 16; ++ks) w1f[nt][ks] = ld8(P.w1t + (size_t)col * INP + ks * 16 + g * 8);
+        for (int ks = 0; ks < HID / 16; ++ks) w2f[nt][ks] = ld8(P.w2t + (size_t)col * HID + ks * 16 + g * 8);
+        b1c[nt] = P.b1[col];
+        b2c[nt] = P.b2[col];
+    }
+    const float b3 = P.b3[0];
     const size_t n_tiles = (n + FW_TM - 1) / FW_TM;
     for (size_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const size_t row0 = tile * FW_TM;
@@ -119,7 +134,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, HamDev H, int n_so, c
                 float lv;
                 if (MODE == IN_REC) {
                     const RecS rr = ((const RecS*)in)[gr];
-                    det = det_of_dense(rr.key >> 1, H);
+                    det = rr.det;
                     cls = rr.cls;
                     lv = plog2(rr.val);
                 } else {
@@ -139,19 +154,17 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, HamDev H, int n_so, c
                 for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 #pragma unroll
         for (int ks = 0; ks < INP / 16; ++ks) {
-            bf16x8 av[2], bv[2];
+            bf16x8 av[2];
             for (int mt = 0; mt < 2; ++mt) av[mt] = ld8(sx + (mt * 32 + li) * FW_XS + ks * 16 + g * 8);
-            for (int nt = 0; nt < 2; ++nt) bv[nt] = ld8(P.w1t + (size_t)(w * 64 + nt * 32 + li) * INP + ks * 16 + g * 8);
             for (int mt = 0; mt < 2; ++mt)
-                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], bv[nt], acc[mt][nt]);
+                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], w1f[nt][ks], acc[mt][nt]);
         }
         for (int mt = 0; mt < 2; ++mt)
             for (int nt = 0; nt < 2; ++nt) {
                 const int col = w * 64 + nt * 32 + li;
-                const float bias = P.b1[col];
                 for (int r = 0; r < 16; ++r) {
                     const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-                    const float z = acc[mt][nt][r] + bias;
+                    const float z = acc[mt][nt][r] + b1c[nt];
                     const bf16_t h = f2bf(silu(z));
                     sh[row * FW_HS + col] = h;
                     if (sv.z1 && row0 + row < n) {
@@ -165,46 +178,46 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, HamDev H, int n_so, c
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b)
                 for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-#pragma unroll 4
+#pragma unroll
         for (int ks = 0; ks < HID / 16; ++ks) {
-            bf16x8 av[2], bv[2];
+            bf16x8 av[2];
             for (int mt = 0; mt < 2; ++mt) av[mt] = ld8(sh + (mt * 32 + li) * FW_HS + ks * 16 + g * 8);
-            for (int nt = 0; nt < 2; ++nt) bv[nt] = ld8(P.w2t + (size_t)(w * 64 + nt * 32 + li) * HID + ks * 16 + g * 8);
             for (int mt = 0; mt < 2; ++mt)
-                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], bv[nt], acc[mt][nt]);
+                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], w2f[nt][ks], acc[mt][nt]);
         }
-        // layer 3: per-lane partial dots over this wavefront's columns
-        float part[2][16];
+        __syncthreads();   // every wavefront has read the first hidden layer
         for (int mt = 0; mt < 2; ++mt)
-            for (int r = 0; r < 16; ++r) part[mt][r] = 0.0f;
-        for (int nt = 0; nt < 2; ++nt) {
-            const int col = w * 64 + nt * 32 + li;
-            const float bias = P.b2[col], w3 = P.w3r[col];
-            for (int mt = 0; mt < 2; ++mt)
+            for (int nt = 0; nt < 2; ++nt) {
+                const int col = w * 64 + nt * 32 + li;
                 for (int r = 0; r < 16; ++r) {
                     const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-                    const float z = acc[mt][nt][r] + bias;
+                    const float z = acc[mt][nt][r] + b2c[nt];
                     const bf16_t h = f2bf(silu(z));
+                    sh[row * FW_HS + col] = h;
                     if (sv.z2 && row0 + row < n) {
                         sv.z2[(row0 + row) * HID + col] = z;
                         sv.h2[(row0 + row) * HID + col] = h;
                     }
-                    part[mt][r] += bf2f(h) * w3;
                 }
-        }
-        for (int mt = 0; mt < 2; ++mt)
-            for (int r = 0; r < 16; ++r) {
-                float v = part[mt][r];
-                for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (li == 0) ypart[w][mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3)] = v;
             }
         __syncthreads();
+        // layer 3: thread (row, quarter) sums 64 columns in order
+        {
+            const int r = tid >> 2, q = tid & 3;
+            const bf16_t* hr = sh + r * FW_HS + q * 64;
+            float v = 0.0f;
+            for (int c8 = 0; c8 < 64; c8 += 8) {
+                const bf16x8 hv = ld8(hr + c8);
+                for (int e = 0; e < 8; ++e) v += (float)hv[e] * sw3[q * 64 + c8 + e];
+            }
+            ypart[q][r] = v;
+        }
+        __syncthreads();
         if (tid < FW_TM && row0 + tid < n) {
-            float v = (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]) + P.b3[0];
+            float v = (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]) + b3;
             if (MODE == IN_REC && (((const RecS*)in)[row0 + tid].key & 1ull)) v = INFINITY;
             y[row0 + tid] = v;
         }
-        __syncthreads();
     }
 }
 
@@ -426,14 +439,14 @@ void fwd_launch(sfci_mlp* net, int mode, const void* in, size_t n, float* y, Fwd
     Ctx* c = net->c;
     if (n == 0) return;
     const size_t tiles = (n + FW_TM - 1) / FW_TM;
-    const int grid = (int)std::min<size_t>(tiles, 256 * 3 * 4);
+    const int grid = (int)std::min<size_t>(tiles, 256);   // one resident workgroup per CU
     const MlpDev P = net->dev();
     if (mode == IN_X)
-        k_mlp_fwd<IN_X><<<grid, 256, 0, c->stream>>>(P, c->ham, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_X><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     else if (mode == IN_REC)
-        k_mlp_fwd<IN_REC><<<grid, 256, 0, c->stream>>>(P, c->ham, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_REC><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     else
-        k_mlp_fwd<IN_SAMPLE><<<grid, 256, 0, c->stream>>>(P, c->ham, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_SAMPLE><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     HIPCHK(hipGetLastError());
 }
 

@@ -47,7 +47,8 @@ class KernelStatT(C.Structure):
 
 class LanczosParamsT(C.Structure):
     _fields_ = [("nd", C.c_int64), ("nl", C.c_int), ("num_restart", C.c_int), ("hpsi_cap", C.c_int64),
-                ("delta_e", C.c_double), ("verbose", C.c_int), ("e_prev_init", C.c_double)]
+                ("delta_e", C.c_double), ("verbose", C.c_int), ("e_prev_init", C.c_double),
+                ("ms_restart", C.c_void_p)]
 
 
 class LinearParamsT(C.Structure):
@@ -304,13 +305,15 @@ class Context:
     def ground_state(self, init_det: int, nd: int, nl: int, num_restart: int, hpsi_cap: int = 60_000_000,
                      delta_e: float = 1e-5, verbose: bool = False, v_out: Optional[Vector] = None,
                      e_prev_init: float = 0.0) -> dict:
-        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0, e_prev_init)
-        ritz, et = np.zeros(num_restart), np.zeros(num_restart)
+        ritz, et, ms = np.zeros(num_restart), np.zeros(num_restart), np.zeros(num_restart)
+        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0, e_prev_init,
+                           ms.ctypes.data)
         r = LanczosResultT()
         self._check(lib().sfci_ground_state(self._h, init_det, C.byref(p), _ptr(ritz), _ptr(et), C.byref(r),
                                             v_out._h if v_out is not None else None))
         n = r.n_restarts
-        return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), e_final=r.e_final,
+        return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), ms_restart=ms[:n].copy(),
+                    e_final=r.e_final,
                     ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv, n_screened=r.n_screened,
                     n_reverts=r.n_reverts)
 
