@@ -191,10 +191,13 @@ typedef struct {
  * one value per restart (capacity num_restart): the selected Ritz value and <v|H|v>
  * after truncation.  v_out (optional, capacity >= nd) receives the final vector.
  * With screening on (sfci_set_screen), the unscreened energy readout <v|H|v> judges every
- * restart: a restart whose energy rises by more than 1e-9 Ha over the best so far is
- * rejected, the best vector is restored and the rest of the solve runs unscreened; the
- * solve stops when the judged energy drops by less than delta_e.  The context's screening
- * setting is the same on return. */
+ * restart and the lowest-energy vector is kept: a screened restart whose energy rises by
+ * more than 1e-9 Ha over the best so far is rejected, the best vector is restored and the
+ * rest of the solve runs unscreened; a screened solve whose energy drops by less than
+ * delta_e hands over to one unscreened restart, which ends the solve unless it lowers the
+ * energy by delta_e or more.  The first unscreened restart takes the lowest Ritz value.
+ * The solve returns the lowest-energy vector (e_final is its energy).  The context's
+ * screening setting is the same on return. */
 int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_lanczos_params_t* params, double* ritz,
                       double* e_trunc, sfci_lanczos_result_t* result, sfci_vec* v_out);
 

@@ -48,9 +48,20 @@ __host__ __device__ __forceinline__ float bf2f(bf16_t h) {
     __builtin_memcpy(&f, &u, 4);
     return f;
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// BF16 rounding (nearest even) in one instruction on gfx950 (v_cvt_pk_bf16_f32); the same
+// result as f2bf for every finite value
+__device__ __forceinline__ bf16_t f2bf_hw(float f) {
+    const __bf16 h = (__bf16)f;
+    bf16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+// sigmoid with the hardware exp2 and reciprocal (about 1 ulp; the activations are then
+// rounded to BF16, 2^-9 relative)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ bf16x8 ld8(const bf16_t* p) { return *(const bf16x8*)p; }
@@ -143,7 +154,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void*
                     cls = s.cls;
                     lv = s.logv;
                 }
-                for (int c2 = 0; c2 < 20; ++c2) dst[c2] = f2bf(feature(part * 20 + c2, det, n_so, cls, lv, P));
+                for (int c2 = 0; c2 < 20; ++c2) dst[c2] = f2bf_hw(feature(part * 20 + c2, det, n_so, cls, lv, P));
             }
         }
         __syncthreads();
@@ -165,7 +176,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void*
                 for (int r = 0; r < 16; ++r) {
                     const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
                     const float z = acc[mt][nt][r] + b1c[nt];
-                    const bf16_t h = f2bf(silu(z));
+                    const bf16_t h = f2bf_hw(silu(z));
                     sh[row * FW_HS + col] = h;
                     if (sv.z1 && row0 + row < n) {
                         sv.z1[(row0 + row) * HID + col] = z;
@@ -192,7 +203,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void*
                 for (int r = 0; r < 16; ++r) {
                     const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
                     const float z = acc[mt][nt][r] + b2c[nt];
-                    const bf16_t h = f2bf(silu(z));
+                    const bf16_t h = f2bf_hw(silu(z));
                     sh[row * FW_HS + col] = h;
                     if (sv.z2 && row0 + row < n) {
                         sv.z2[(row0 + row) * HID + col] = z;
@@ -301,7 +312,7 @@ __global__ void k_bwd_out(const float* __restrict__ dy, const float* __restrict_
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * HID; i += (size_t)gridDim.x * blockDim.x) {
         const size_t r = i / HID;
         const int col = (int)(i - r * HID);
-        dz2[i] = f2bf(dy[r] * w3r[col] * silu_grad(z2[i]));
+        dz2[i] = f2bf_hw(dy[r] * w3r[col] * silu_grad(z2[i]));
     }
 }
 
@@ -309,7 +320,7 @@ __global__ void k_bwd_out(const float* __restrict__ dy, const float* __restrict_
 __global__ void k_bwd_hidden(const float* __restrict__ dh1, const float* __restrict__ z1, size_t n,
                              bf16_t* __restrict__ dz1) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * HID; i += (size_t)gridDim.x * blockDim.x)
-        dz1[i] = f2bf(dh1[i] * silu_grad(z1[i]));
+        dz1[i] = f2bf_hw(dh1[i] * silu_grad(z1[i]));
 }
 
 // Flat gradient from the pieces: dW1 [HID][XT_ROWS] (columns < IN used), db1, dW2, db2,
