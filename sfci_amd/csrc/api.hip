@@ -234,7 +234,7 @@ void sfci_destroy(sfci_ctx* X) {
                       &c.src_k, &c.src_v, &c.dterm, &c.bucket_bounds, &c.lin_k[0], &c.lin_k[1], &c.lin_v[0],
                       &c.lin_v[1], &c.part_meta[0], &c.part_meta[1], &c.part_start[0], &c.part_start[1],
                       &c.part_len[0], &c.part_len[1], &c.part_hbase[0], &c.part_hbase[1], &c.part_hstride[0],
-                      &c.part_hstride[1], &c.part_region[0], &c.part_region[1], &c.part_base, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
+                      &c.part_hstride[1], &c.part_region[0], &c.part_region[1], &c.part_base, &c.part_dig, &c.stage_k, &c.stage_v, &c.tmp_k[0], &c.tmp_k[1],
                       &c.tmp_v[0], &c.tmp_v[1], &c.sel_a, &c.sel_b, &c.sel_c, &c.scratch_hist,
                       &c.scratch_off, &c.scratch_small, &c.rowhist, &c.plan_k[0], &c.plan_k[1], &c.plan_v[0], &c.plan_v[1],
                       &c.plan_fl, &c.plan_pos, &c.plan_first, &c.plan_str, &c.plan_cnt};

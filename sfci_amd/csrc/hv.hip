@@ -1615,7 +1615,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     // 2. plan passes over target rows
     const uint64_t n_rows = H.dim;
     const size_t rec = sizeof(uint64_t) + sizeof(double);
-    size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec), (size_t)1 << 20);
+    // two record buffers per pass and one digit byte per first-level slot
+    size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
     if (c->merge_mode == 0) {
         // keep the distinct targets of a pass within 2^16 LDS-table buckets at a load of
         // at most 0.8, using the distinct/candidate ratio of the previous call

@@ -218,6 +218,7 @@ struct Ctx {
     DevBuf lin_k[2], lin_v[2];       // uncut H.v results of the linear-solver products
     DevBuf part_meta[2], part_start[2], part_len[2], part_hbase[2], part_hstride[2], part_region[2];   // partition chunks
     DevBuf part_base;
+    DevBuf part_dig;                 // second-level digit byte of each first-level partition slot
     DevBuf tmp_k[2], tmp_v[2];
     DevBuf sel_a, sel_b, sel_c;      // selection / merge scratch
     DevBuf scratch_hist, scratch_off, scratch_small;

@@ -288,7 +288,7 @@ constexpr int PART_ITEMS = PART_TILE / PART_THREADS;   // 8
 constexpr int HIST_CHUNKS = 16;                        // chunks per histogram workgroup
 
 size_t part_capacity(size_t n) {
-    size_t cap1 = (size_t)((double)n / RADIX * 1.02) + 65536;
+    size_t cap1 = ((size_t)((double)n / RADIX * 1.02) + 65536 + 63) & ~(size_t)63;   // chunk starts 64-aligned
     return (size_t)RADIX * cap1;
 }
 
@@ -415,12 +415,15 @@ __global__ __launch_bounds__(PART_THREADS) void k_part_scatter(const R* __restri
 
 // pass 1: tile t = records [t * PART_TILE, ...) of rin; digit d's run is reserved on
 // cursor[d] and written only when it ends at or before rend[d]
+// dout (optional) receives, at each record's output position, the record's digit under
+// dg2: the digit of the next partition pass, so that its histogram reads one byte per record
 template <class Digit, typename R>
 __global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cur(const R* __restrict__ rin, size_t n,
                                                                     R* __restrict__ rout, Digit dg,
                                                                     unsigned long long* __restrict__ cursor,
                                                                     const uint64_t* __restrict__ rend,
-                                                                    uint32_t* __restrict__ flags) {
+                                                                    uint32_t* __restrict__ flags,
+                                                                    uint8_t* __restrict__ dout, Digit dg2) {
     __shared__ R s_rec[PART_TILE];
     __shared__ uint8_t s_dig[PART_TILE];
     __shared__ uint64_t s_gbase[RADIX];
@@ -477,7 +480,12 @@ __global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cur(const R* __re
     __syncthreads();
     for (uint32_t p = tid; p < len; p += PART_THREADS) {
         const uint32_t d = s_dig[p];
-        if (s_ok[d]) rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
+        if (s_ok[d]) {
+            const uint64_t at = s_gbase[d] + (p - s_lstart[d]);
+            const R x = s_rec[p];
+            rout[at] = x;
+            if (dout) dout[at] = (uint8_t)dg2(rec_key(x));
+        }
     }
 }
 
@@ -544,6 +552,34 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist_cm(const R* __restrict
         const uint64_t st = ch.start[c];
         const uint32_t len = ch.len[c];
         for (uint32_t i = tid; i < len; i += SFCI_BLOCK) atomicAdd(&h[cc][dg(rec_key(rin[st + i]))], 1u);
+    }
+    __syncthreads();
+    for (int e = tid; e < HIST_CHUNKS * RADIX; e += SFCI_BLOCK) {
+        const size_t c = c0 + e / RADIX;
+        if (c < ch.n) cnt[c * RADIX + (e % RADIX)] = (&h[0][0])[e];
+    }
+}
+
+// the same counters from the digit bytes written by the previous pass (chunk starts are
+// 64-aligned: four digits per 32-bit load)
+__global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist_bytes(const uint8_t* __restrict__ din, Chunks ch,
+                                                                uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t h[HIST_CHUNKS][RADIX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < HIST_CHUNKS * RADIX; i += SFCI_BLOCK) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const size_t c0 = (size_t)blockIdx.x * HIST_CHUNKS;
+    for (int cc = 0; cc < HIST_CHUNKS; ++cc) {
+        const size_t c = c0 + cc;
+        if (c >= ch.n) break;
+        const uint64_t st = ch.start[c];
+        const uint32_t len = ch.len[c];
+        const uint32_t* w = (const uint32_t*)(din + st);
+        for (uint32_t i = tid; 4 * i < len; i += SFCI_BLOCK) {
+            const uint32_t x = w[i];
+            const uint32_t m = min(4u, len - 4 * i);
+            for (uint32_t e = 0; e < m; ++e) atomicAdd(&h[cc][(x >> (8 * e)) & 255u], 1u);
+        }
     }
     __syncthreads();
     for (int e = tid; e < HIST_CHUNKS * RADIX; e += SFCI_BLOCK) {
@@ -624,7 +660,7 @@ __global__ __launch_bounds__(PART_THREADS) void k_part_scatter_cm(const R* __res
 // regions given by (rstart, rsize) with dense output starts dense_start: rin -> rout
 template <typename R, class Digit>
 static void part_pass_regions(Ctx* c, const R* rin, R* rout, const std::vector<uint64_t>& rstart,
-                              const std::vector<uint64_t>& rsize, Digit dg, size_t n) {
+                              const std::vector<uint64_t>& rsize, Digit dg, size_t n, const uint8_t* din = nullptr) {
     const int nr = (int)rstart.size();
     Chunks ch = make_chunks(c, 0, rstart, rsize);
     // meta of slot 0 holds cbase at [2 nr, 3 nr]; dense starts go to slot 1
@@ -639,8 +675,13 @@ static void part_pass_regions(Ctx* c, const R* rin, R* rout, const std::vector<u
     HIPCHK(hipMemcpyAsync(dds, ds.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
     uint32_t* cnt = c->scratch_hist.get<uint32_t>((size_t)RADIX * std::max<size_t>(ch.n, 1));
     unsigned long long* cursor = (unsigned long long*)c->part_base.get<uint64_t>((size_t)RADIX * nr);
-    int t0 = kt_begin(c, K_PART_HIST, (double)sizeof(R) * n + 4.0 * RADIX * ch.n);
-    k_part_hist_cm<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(rin, ch, dg, cnt);
+    int t0 = kt_begin(c, K_PART_HIST, (din ? 1.0 : (double)sizeof(R)) * n + 4.0 * RADIX * ch.n);
+    if (din)
+        k_part_hist_bytes<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(din, ch,
+                                                                                                           cnt);
+    else
+        k_part_hist_cm<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(rin, ch, dg,
+                                                                                                        cnt);
     kt_end(c, t0);
     int t1 = kt_begin(c, K_SCAN, 4.0 * RADIX * ch.n);
     k_region_colscan<<<nr, RADIX, 0, c->stream>>>(cnt, cbase, dds, cursor);
@@ -670,9 +711,10 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
     uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX);
     HIPCHK(hipMemcpyAsync(dcur, cur.data(), 2 * RADIX * 8, hipMemcpyHostToDevice, c->stream));
     const size_t n_tiles = (n + PART_TILE - 1) / PART_TILE;
-    int t1 = kt_begin(c, K_PART_SCATTER1, 2.0 * sizeof(R) * n);
+    uint8_t* dig = c->part_dig.get<uint8_t>(part_capacity(n));
+    int t1 = kt_begin(c, K_PART_SCATTER1, (2.0 * sizeof(R) + 1.0) * n);
     k_part_scatter_cur<<<(unsigned)n_tiles, PART_THREADS, 0, c->stream>>>(
-        A, n, B, DigitHash{hb - 8, hb}, (unsigned long long*)dcur, dcur + RADIX, c->d_flags);
+        A, n, B, DigitHash{hb - 8, hb}, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig, DigitHash{0, hb});
     kt_end(c, t1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(cur.data(), dcur, RADIX * 8, hipMemcpyDeviceToHost, c->stream));
@@ -687,7 +729,7 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
         rstart[d] = (uint64_t)d * cap1;
         rsize[d] = cur[d] - rstart[d];
     }
-    part_pass_regions<R>(c, B, A, rstart, rsize, DigitHash{0, hb}, n);
+    part_pass_regions<R>(c, B, A, rstart, rsize, DigitHash{0, hb}, n, dig);
     return A;
 }
 
