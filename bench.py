@@ -35,11 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # copy of known size (summary in profiles/); other workloads and precisions report null
 PMC_TRAFFIC = {
     ("c2", "fp64"): {
-        "k_generate": 8.561e10,
-        "k_part_scatter_cur<DigitHash>": 1.643e11,
-        "k_part_scatter<DigitHash>": 1.645e11,
-        "k_part_hist<DigitHash>": 8.139e10,
-        "k_bucket_reduce": 8.147e10,
+        "k_generate": 8.559e10,
+        "k_part_scatter_cur<DigitHash>": 1.733e11,
+        "k_part_scatter<DigitHash>": 1.641e11,
+        "k_part_hist<DigitHash>": 5.672e9,
+        "k_bucket_reduce": 8.148e10,
     },
 }
 

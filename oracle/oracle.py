@@ -113,6 +113,14 @@ def lib():
         L.oracle_minres.argtypes = [C.c_void_p, _u64p, _f64p, C.c_int64, _u64p, _f64p, C.c_int64, C.c_int64,
                                     C.c_int, C.c_double, C.c_double, C.c_double, _u64p, _f64p,
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.oracle_connections_wide.restype = C.c_int64
+        L.oracle_connections_wide.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, _u64p, _u64p, _f64p, C.c_int64]
+        L.oracle_diag_wide.restype = C.c_double
+        L.oracle_diag_wide.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        L.oracle_hv_wide.restype = C.c_int
+        L.oracle_hv_wide.argtypes = [C.c_void_p, _u64p, _u64p, _f64p, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                     C.c_double, C.c_int64, _u64p, _u64p, _f64p, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_double), C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -193,6 +201,34 @@ class Oracle:
             raise RuntimeError("oracle_hv output bound too small")
         m = olen.value
         return ok[:m].copy(), ov[:m].copy(), diag.value, ncand.value
+
+    # --- 128-bit keys (lo = spin orbitals 0-63, hi = 64-127) ---------------------------
+    def connections_wide(self, d: int):
+        lo, hi = d & 0xFFFFFFFFFFFFFFFF, d >> 64
+        z = np.zeros(1, np.uint64)
+        n = lib().oracle_connections_wide(self._h, lo, hi, z, z.copy(), np.zeros(1), 0)
+        kl, kh, v = np.zeros(n, np.uint64), np.zeros(n, np.uint64), np.zeros(n)
+        lib().oracle_connections_wide(self._h, lo, hi, kl, kh, v, n)
+        return kl, kh, v
+
+    def diag_wide(self, d: int) -> float:
+        return lib().oracle_diag_wide(self._h, d & 0xFFFFFFFFFFFFFFFF, d >> 64)
+
+    def hv_wide(self, lo, hi, vals, nd=None, lam=0.0, coeff=1.0, cut=1e-5, out_cap=60_000_000,
+                nthreads=0, mode=0):
+        """H.v on 128-bit keys given as (lo, hi) word arrays, ascending by hi:lo."""
+        lo, hi, vals = _u64(lo), _u64(hi), _f64(vals)
+        n = len(lo)
+        nd = n if nd is None else nd
+        bound = min(out_cap, max(1, min(n, nd)) * int(self.n_conn()))
+        ol, oh, ov = np.zeros(bound, np.uint64), np.zeros(bound, np.uint64), np.zeros(bound)
+        olen, diag = C.c_int64(), C.c_double()
+        rc = lib().oracle_hv_wide(self._h, lo, hi, vals, n, nd, lam, coeff, cut, bound, ol, oh, ov,
+                                  C.byref(olen), C.byref(diag), nthreads, mode)
+        if rc:
+            raise RuntimeError("oracle_hv_wide failed (%d)" % rc)
+        m = olen.value
+        return ol[:m].copy(), oh[:m].copy(), ov[:m].copy(), diag.value
 
     def energy(self, keys, vals, nthreads=0) -> float:
         keys, vals = _u64(keys), _f64(vals)

@@ -83,10 +83,15 @@ static uint64_t binom(int n, int k) {
 }
 
 // signCal_bit (src/bitstr.cpp:133-160): (-1)^(#occupied strictly between p and q), p < q.
-static inline double sgn_between(det_t d, int p, int q) {
+// D is det_t (64-bit keys) or u128 (128-bit keys, more than 64 spin orbitals).
+typedef unsigned __int128 u128;
+static inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+static inline int popc(u128 x) { return __builtin_popcountll((uint64_t)x) + __builtin_popcountll((uint64_t)(x >> 64)); }
+template <class D>
+static inline double sgn_between(D d, int p, int q) {
     if (p > q) std::swap(p, q);
-    det_t mask = (q - p > 1) ? ((((det_t)1 << (q - p - 1)) - 1) << (p + 1)) : 0;
-    return (__builtin_popcountll(d & mask) & 1) ? -1.0 : 1.0;
+    D mask = (q - p > 1) ? ((((D)1 << (q - p - 1)) - 1) << (p + 1)) : 0;
+    return (popc(d & mask) & 1) ? -1.0 : 1.0;
 }
 
 // indexCal (src/bitstr.cpp:74-131): colex rank of the even-bit string.
@@ -103,7 +108,8 @@ static inline uint64_t alpha_rank(const Ham& H, det_t d) {
     return r;
 }
 
-static int occ_list(const Ham& H, det_t d, int* occ) {
+template <class D>
+static int occ_list(const Ham& H, D d, int* occ) {
     int k = 0;
     for (int i = 0; i < H.n_so; ++i)
         if (d >> i & 1) occ[k++] = i;
@@ -114,7 +120,8 @@ static int occ_list(const Ham& H, det_t d, int* occ) {
 
 // DiagCal_bit restated with exact accumulation: E_core + sum_occ h_mm +
 // sum_{m != p occ} 1/2 [(mm|pp) - (mp|pm)].
-static double diag_exact(const Ham& H, det_t d) {
+template <class D>
+static double diag_exact(const Ham& H, D d) {
     int occ[128];
     int k = occ_list(H, d, occ);
     fx_t acc = fx_from(H.constant);
@@ -130,7 +137,8 @@ static double diag_exact(const Ham& H, det_t d) {
 }
 
 // <D'|H|D> for D' = D with n -> m (m empty, n occupied).
-static double single_exact(const Ham& H, det_t d, int n, int m) {
+template <class D>
+static double single_exact(const Ham& H, D d, int n, int m) {
     int occ[128];
     int k = occ_list(H, d, occ);
     fx_t acc = fx_from(H.h[m * H.n_so + n]);
@@ -144,25 +152,26 @@ static double single_exact(const Ham& H, det_t d, int n, int m) {
 }
 
 // <D''|H|D> for D'' = D - {n,q} + {x,y}, n < q occupied, x,y empty.
-static double double_exact(const Ham& H, det_t d, int n, int q, int x, int y) {
-    det_t one = 1;
-    det_t d1 = d - (one << q) + (one << y);
+template <class D>
+static double double_exact(const Ham& H, D d, int n, int q, int x, int y) {
+    D one = 1;
+    D d1 = d - (one << q) + (one << y);
     double s1 = sgn_between(d, q, y) * sgn_between(d1, n, x);
-    det_t d2 = d - (one << q) + (one << x);
+    D d2 = d - (one << q) + (one << x);
     double s2 = sgn_between(d, q, x) * sgn_between(d2, n, y);
     return s1 * H.g(x, n, y, q) + s2 * H.g(y, n, x, q);
 }
 
 // Unique connections of d in a fixed class order (diagonal first, then singles and
 // doubles by ascending spin-orbital indices).  Calls f(key, H_ij) for each.
-template <class F>
-static void connections_exact(const Ham& H, det_t d, F&& f) {
+template <class D, class F>
+static void connections_exact(const Ham& H, D d, F&& f) {
     int occ[128], vir[128];
     int k = 0, v = 0;
     for (int i = 0; i < H.n_so; ++i) {
         if (d >> i & 1) occ[k++] = i; else vir[v++] = i;
     }
-    det_t one = 1;
+    D one = 1;
     f(d, diag_exact(H, d));
     for (int a = 0; a < k; ++a)
         for (int b = 0; b < v; ++b) {
@@ -179,7 +188,7 @@ static void connections_exact(const Ham& H, det_t d, F&& f) {
                     // spin conservation: {x,y} has the spins of {n,q}
                     int sn = (n & 1) + (q & 1), sx = (x & 1) + (y & 1);
                     if (sn != sx) continue;
-                    det_t t = d - (one << n) - (one << q) + (one << x) + (one << y);
+                    D t = d - (one << n) - (one << q) + (one << x) + (one << y);
                     f(t, double_exact(H, d, n, q, x, y));
                 }
         }
@@ -702,6 +711,109 @@ double oracle_energy(const oracle_ham* h, const uint64_t* k, const double* v, in
     fx_t tot = 0;
     for (auto p : part) tot += p;
     return fx_to(tot);
+}
+
+// ---- 128-bit determinant keys (more than 64 spin orbitals) -------------------------
+// No reference counterpart: the reference stores determinants in one 64-bit word
+// (src/all.h:150-194, src/main.cpp:84-85).  These functions extend the exact mode (0)
+// and the mixed mode (2) of oracle_hv to keys of two words, lo = spin orbitals 0-63,
+// hi = spin orbitals 64-127, ordered as the 128-bit integer hi:lo.  The Slater-Condon
+// helpers are the same templates the 64-bit functions use.
+
+// Unique connections of one determinant (diagonal first).  Returns the count; writes up
+// to cap.
+int64_t oracle_connections_wide(const oracle_ham* h, uint64_t lo, uint64_t hi, uint64_t* out_lo,
+                                uint64_t* out_hi, double* vals, int64_t cap) {
+    const u128 d = ((u128)hi << 64) | lo;
+    int64_t n = 0;
+    connections_exact(*(const Ham*)h, d, [&](u128 k, double v) {
+        if (n < cap) { out_lo[n] = (uint64_t)k; out_hi[n] = (uint64_t)(k >> 64); vals[n] = v; }
+        ++n;
+    });
+    return n;
+}
+
+double oracle_diag_wide(const oracle_ham* h, uint64_t lo, uint64_t hi) {
+    return diag_exact(*(const Ham*)h, ((u128)hi << 64) | lo);
+}
+
+// oracle_hv (modes 0 and 2) on 128-bit keys; `in` key-sorted.  Returns 0, -1 when a term
+// leaves the fixed-point range, -2 for an unsupported mode.
+int oracle_hv_wide(const oracle_ham* hp, const uint64_t* in_lo, const uint64_t* in_hi, const double* in_v,
+                   int64_t n_in, int64_t nd, double lambda, double coeff, double cut, int64_t out_cap,
+                   uint64_t* out_lo, uint64_t* out_hi, double* out_v, int64_t* out_len, double* diag_out,
+                   int nthreads, int mode) {
+    const Ham& H = *(const Ham*)hp;
+    if (mode != 0 && mode != 2) return -2;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    struct KW { u128 k; double v; };
+    auto rank_above = [](const KW& a, const KW& b) {
+        double fa = std::fabs(a.v), fb = std::fabs(b.v);
+        if (fa != fb) return fa > fb;
+        return a.k < b.k;
+    };
+    auto by_key = [](const KW& a, const KW& b) { return a.k < b.k; };
+    auto top_cap = [&](std::vector<KW>& x, size_t cap) {
+        if (x.size() > cap) {
+            std::nth_element(x.begin(), x.begin() + cap, x.end(), rank_above);
+            x.resize(cap);
+        }
+        std::sort(x.begin(), x.end(), by_key);
+    };
+    std::vector<KW> src(n_in);
+    for (int64_t i = 0; i < n_in; ++i) src[i] = {((u128)in_hi[i] << 64) | in_lo[i], in_v[i]};
+    if (n_in > nd) top_cap(src, (size_t)nd);
+    const int64_t ns = (int64_t)src.size();
+    struct CW { u128 k; double v; uint8_t tag; };
+    std::vector<std::vector<CW>> res(nthreads);
+    int bad = 0;
+#pragma omp parallel num_threads(nthreads)
+    {
+        auto& mine = res[omp_get_thread_num()];
+#pragma omp for schedule(static, 25)
+        for (int64_t i = 0; i < ns; ++i) {
+            const double temp = coeff * src[i].v;
+            bool first = true;
+            connections_exact(H, src[i].k, [&](u128 t, double hij) {
+                double val;
+                uint8_t tag = 0;
+                if (first) { val = (hij - lambda) * temp; tag = 1; first = false; }
+                else if (mode == 2) val = (double)((float)hij * (float)temp);
+                else val = hij * temp;
+                if (!fx_ok(val)) bad = 1;
+                mine.push_back({t, val, tag});
+            });
+        }
+    }
+    if (bad) return -1;
+    std::vector<CW> all;
+    for (auto& r : res) { all.insert(all.end(), r.begin(), r.end()); std::vector<CW>().swap(r); }
+    std::sort(all.begin(), all.end(), [](const CW& a, const CW& b) { return a.k < b.k; });
+    std::vector<KW> fin;
+    fx_t dacc = 0;
+    size_t i = 0;
+    while (i < all.size()) {
+        size_t j = i;
+        fx_t acc = 0;
+        bool is_input = false;
+        for (; j < all.size() && all[j].k == all[i].k; ++j) { acc += fx_from(all[j].v); is_input |= all[j].tag; }
+        const double y = fx_to(acc);
+        if (is_input) {
+            auto it = std::lower_bound(src.begin(), src.end(), all[i].k, [](const KW& a, u128 k) { return a.k < k; });
+            dacc += fx_from(y * it->v);
+        }
+        if (std::fabs(y) > cut) fin.push_back({all[i].k, y});
+        i = j;
+    }
+    top_cap(fin, (size_t)out_cap);
+    for (size_t q = 0; q < fin.size(); ++q) {
+        out_lo[q] = (uint64_t)fin[q].k;
+        out_hi[q] = (uint64_t)(fin[q].k >> 64);
+        out_v[q] = fin[q].v;
+    }
+    *out_len = (int64_t)fin.size();
+    *diag_out = fx_to(dacc);
+    return 0;
 }
 
 }  // extern "C"
