@@ -327,6 +327,75 @@ def screen_hv(args):
     ctx.close()
 
 
+def wide_hv(args):
+    """H.v on 128-bit keys: the strong-correlation workload (50 spatial / 100 spin orbitals,
+    20 electrons, seed 3) at a reduced Nd (default 1e4: its full Nd of 1e7 would be 2.3e12
+    connections per H.v).  The input is grown from HF by H.v with output capacity Nd,
+    normalized on the host between rounds (not timed)."""
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    t0 = time.perf_counter()
+    w = I.make_workload("c4", data)
+    nd = args.nd or 10_000
+    log("integrals ready (%.1f s): %s" % (time.perf_counter() - t0, w["in_path"]))
+    t0 = time.perf_counter()
+    ctx = gpu.WideContext(w["in_path"], w["n_so"], w["n_elec"])
+    log("context ready (%.1f s)" % (time.perf_counter() - t0))
+    n_conn = ctx.info()["n_conn"]
+    mixed = args.precision == "mixed"
+    hf = w["init_hf"]
+    lo, hi = gpu.split_keys([hf])
+    v = ctx.vector(nd, lo, hi, np.array([1.0]))
+    out = ctx.vector(nd)
+    for it in range(6):
+        if v.len >= nd:
+            break
+        ctx.hv(v, out, nd)
+        ol, oh, ov = out.download()
+        v = ctx.vector(nd, ol, oh, ov / np.sqrt(np.dot(ov, ov)))
+        log("grow %d: %d determinants" % (it, v.len))
+    ctx.set_precision(mixed)
+    big = ctx.vector(args.cap)
+    for i in range(args.warmup):
+        ctx.hv(v, big, nd)
+        log("warmup %d: %s" % (i, ctx.stats()))
+    t = time.perf_counter()
+    sts = []
+    for i in range(args.steps):
+        ctx.hv(v, big, nd)
+        sts.append(ctx.stats())
+        log("step %d: %s" % (i, sts[-1]))
+    dt = (time.perf_counter() - t) / args.steps
+    n_in = v.len
+    conn = n_in * n_conn
+    avg = {k: sum(x[k] for x in sts) / len(sts) for k in ("ms_generate", "ms_sort", "ms_reduce", "ms_select",
+                                                          "ms_total", "n_passes", "n_unique", "n_out")}
+    res = dict(metric="H.v connections/s, 128-bit keys", value=conn / dt, unit="connections/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=dt * 1e3, higher_is_better=True,
+               dtype="f32/f64" if mixed else "f64", data="synthetic",
+               config=dict(workload="H.v, %d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d (reduced), "
+                           "cap=%d" % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, args.cap),
+                           n_in=n_in, n_conn=n_conn, candidates=conn),
+               phases_ms=avg,
+               generate_write_gbs=24.0 * conn / (avg["ms_generate"] * 1e-3) / 1e9 if avg["ms_generate"] else None)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
+        vl, vh, vv = v.download()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        m = min(len(vl), 200)
+        t = time.perf_counter()
+        orc.hv_wide(vl[:m], vh[:m], vv[:m], nthreads=threads)
+        tc = time.perf_counter() - t
+        res["cpu_baseline"] = dict(value=m * n_conn / tc, unit="connections/s", cores=threads, kind="port",
+                                   sample="first %d of the %d input determinants (key order), oracle H.v on "
+                                          "128-bit keys with %d OpenMP threads: %.2f s" % (m, n_in, threads, tc))
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    print(json.dumps(res))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,6 +416,8 @@ def main():
                     help="learned-screener workload: train on seeds 10-19, time plain / network / magnitude solves")
     ap.add_argument("--screen-hv", action="store_true",
                     help="time one screened H.v per scoring mode on the learned-screener workload")
+    ap.add_argument("--wide", action="store_true",
+                    help="H.v on 128-bit keys (100 spin orbitals, 20 electrons, seed 3; reduced Nd)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one plain device copy of 1 GiB read + 1 GiB written (counter calibration)")
     ap.add_argument("--nl", type=int, default=None)
@@ -372,6 +443,9 @@ def main():
         return
     if args.copy_probe:
         copy_probe()
+        return
+    if args.wide:
+        wide_hv(args)
         return
     from sfci_amd import gpu, integrals as I
 

@@ -338,6 +338,38 @@ int sfci_get_screen(const sfci_ctx* ctx, int* mode, sfci_mlp** net, double* keep
 int sfci_screen_samples(sfci_ctx* ctx, const sfci_vec* v, double eps_pre, int64_t max_samples, uint64_t seed,
                         sfci_screen_sample_t* out, int64_t* n_out);
 
+/* ---- 128-bit determinant keys: more than 64 spin orbitals (up to 128) -------------
+ * No reference counterpart: the reference stores a determinant in one 64-bit word
+ * (dynVec<u64,double>, src/all.h:150-194) and stops at 64 spin orbitals
+ * (src/main.cpp:84-85).  These entry points extend SpMVMtply_bucket
+ * (src/functions.cpp:511-804) to keys of two words.  A key is stored as two uint64:
+ * word 0 = spin orbitals 0-63, word 1 = spin orbitals 64-127; vectors are sorted
+ * ascending by the 128-bit value word1:word0.  The context takes the same _IN files as
+ * sfci_create; every other convention (return codes, exact sums, tie rule, cut) is that
+ * of the 64-bit entry points. */
+typedef struct sfci_wctx sfci_wctx;
+typedef struct sfci_wvec sfci_wvec;
+
+/* n_so even <= 128, n_elec even <= 64, n_elec / 2 <= n_so / 2. */
+int sfci_wide_create(const char* in_path, int n_so, int n_elec, int device, sfci_wctx** out);
+void sfci_wide_destroy(sfci_wctx* ctx);
+const char* sfci_wide_last_error(const sfci_wctx* ctx);
+int sfci_wide_info(const sfci_wctx* ctx, sfci_info_t* info);
+int sfci_wide_set_cut(sfci_wctx* ctx, double cut);
+int sfci_wide_set_precision(sfci_wctx* ctx, int precision);
+/* Device bytes the candidate buffers of one pass may use (104 bytes per candidate). */
+int sfci_wide_set_workspace(sfci_wctx* ctx, uint64_t bytes);
+int sfci_wide_hv_stats(const sfci_wctx* ctx, sfci_hv_stats_t* st);
+int sfci_wide_vec_create(sfci_wctx* ctx, int64_t cap, sfci_wvec** out);
+void sfci_wide_vec_destroy(sfci_wvec* v);
+int64_t sfci_wide_vec_len(const sfci_wvec* v);
+/* keys: 2 * n words (entry i = keys[2i], keys[2i+1]), strictly ascending. */
+int sfci_wide_vec_upload(sfci_wvec* v, const uint64_t* keys, const double* vals, int64_t n);
+int sfci_wide_vec_download(const sfci_wvec* v, uint64_t* keys, double* vals, int64_t max_n, int64_t* n_out);
+/* sfci_hv on 128-bit keys. */
+int sfci_wide_hv(sfci_wctx* ctx, const sfci_wvec* in, sfci_wvec* out, int64_t nd, double lambda, double coeff,
+                 double* diag);
+
 #ifdef __cplusplus
 }
 #endif

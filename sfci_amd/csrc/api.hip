@@ -45,10 +45,13 @@ int read_in_file(const char* path, std::vector<Entry>& out) {
     return SFCI_OK;
 }
 
-int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, sfci_ctx** out, std::string& err) {
-    if (n_so < 2 || n_so > 64 || (n_so & 1) || n_elec < 2 || (n_elec & 1) || n_elec / 2 > n_so / 2 ||
+// max_so: 64 for the 64-bit-key entry points, 128 for the 128-bit-key (sfci_wide_*) ones.
+int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, sfci_ctx** out, std::string& err,
+              int max_so = 64) {
+    if (n_so < 2 || n_so > max_so || (n_so & 1) || n_elec < 2 || (n_elec & 1) || n_elec / 2 > n_so / 2 ||
         n_elec / 2 > SFCI_MAX_K) {
-        err = "unsupported n_so / n_elec (n_so even <= 64, n_elec even)";
+        err = max_so == 64 ? "unsupported n_so / n_elec (n_so even <= 64, n_elec even)"
+                           : "unsupported n_so / n_elec (n_so even <= 128, n_elec even, n_elec <= 64)";
         return SFCI_E_ARG;
     }
     const int N = n_so / 2;
@@ -684,6 +687,125 @@ int sfci_add_three(sfci_ctx* X, sfci_vec* out, const sfci_vec* x, const sfci_vec
         size_t f = dev_filter_above(c, k0, v0, m, 1e-14, k1, v1);
         dev_copy(c, out->v.k, out->v.v, k1, v1, f);
         out->v.len = (int64_t)f;
+    });
+}
+
+// ---- 128-bit keys (more than 64 spin orbitals) -------------------------------------
+
+int sfci_wide_create(const char* in_path, int n_so, int n_elec, int device, sfci_wctx** out) {
+    if (!in_path || !out) return SFCI_E_ARG;
+    *out = nullptr;
+    std::vector<Entry> ent;
+    int rc = read_in_file(in_path, ent);
+    if (rc) return rc;
+    std::string err;
+    sfci_ctx* X = nullptr;
+    rc = build_ctx(n_so, n_elec, ent, device, &X, err, 128);
+    if (rc) {
+        fprintf(stderr, "sfci_wide_create: %s\n", err.c_str());
+        return rc;
+    }
+    *out = new sfci_wctx{X};
+    return SFCI_OK;
+}
+
+void sfci_wide_destroy(sfci_wctx* W) {
+    if (!W) return;
+    sfci_destroy(W->x);
+    delete W;
+}
+
+const char* sfci_wide_last_error(const sfci_wctx* W) { return W ? sfci_last_error(W->x) : "null context"; }
+
+int sfci_wide_info(const sfci_wctx* W, sfci_info_t* info) { return W ? sfci_info(W->x, info) : SFCI_E_ARG; }
+
+int sfci_wide_set_cut(sfci_wctx* W, double cut) { return W ? sfci_set_cut(W->x, cut) : SFCI_E_ARG; }
+
+int sfci_wide_set_precision(sfci_wctx* W, int precision) {
+    return W ? sfci_set_precision(W->x, precision) : SFCI_E_ARG;
+}
+
+int sfci_wide_set_workspace(sfci_wctx* W, uint64_t bytes) { return W ? sfci_set_workspace(W->x, bytes) : SFCI_E_ARG; }
+
+int sfci_wide_hv_stats(const sfci_wctx* W, sfci_hv_stats_t* s) { return W ? sfci_hv_stats(W->x, s) : SFCI_E_ARG; }
+
+int sfci_wide_vec_create(sfci_wctx* W, int64_t cap, sfci_wvec** out) {
+    if (!W || !out || cap < 1) return SFCI_E_ARG;
+    *out = nullptr;
+    Ctx* c = &W->x->c;
+    sfci_wvec* V = new sfci_wvec();
+    int rc = guarded(c, [&] {
+        V->ctx = c;
+        V->cap = cap;
+        HIPCHK(hipMalloc(&V->k, (size_t)cap * 16));
+        HIPCHK(hipMalloc(&V->v, (size_t)cap * 8));
+    });
+    if (rc) {
+        if (V->k) (void)hipFree(V->k);
+        if (V->v) (void)hipFree(V->v);
+        delete V;
+        return rc;
+    }
+    *out = V;
+    return SFCI_OK;
+}
+
+void sfci_wide_vec_destroy(sfci_wvec* V) {
+    if (!V) return;
+    (void)hipSetDevice(V->ctx->device);
+    (void)hipStreamSynchronize(V->ctx->stream);
+    if (V->k) (void)hipFree(V->k);
+    if (V->v) (void)hipFree(V->v);
+    delete V;
+}
+
+int64_t sfci_wide_vec_len(const sfci_wvec* V) { return V ? V->len : -1; }
+
+int sfci_wide_vec_upload(sfci_wvec* V, const uint64_t* keys, const double* vals, int64_t n) {
+    if (!V || n < 0 || n > V->cap || (n && (!keys || !vals))) return SFCI_E_ARG;
+    for (int64_t i = 1; i < n; ++i) {
+        const uint64_t alo = keys[2 * i - 2], ahi = keys[2 * i - 1], blo = keys[2 * i], bhi = keys[2 * i + 1];
+        if (!(ahi < bhi || (ahi == bhi && alo < blo))) {
+            set_err(V->ctx, "sfci_wide_vec_upload: keys must be strictly ascending (hi:lo)");
+            return SFCI_E_ARG;
+        }
+    }
+    Ctx* c = V->ctx;
+    return guarded(c, [&] {
+        if (n) {
+            HIPCHK(hipMemcpyAsync(V->k, keys, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(V->v, vals, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        V->len = n;
+    });
+}
+
+int sfci_wide_vec_download(const sfci_wvec* V, uint64_t* keys, double* vals, int64_t max_n, int64_t* n_out) {
+    if (!V || max_n < 0) return SFCI_E_ARG;
+    const int64_t n = std::min(V->len, max_n);
+    if (n && (!keys || !vals)) return SFCI_E_ARG;
+    Ctx* c = V->ctx;
+    return guarded(c, [&] {
+        if (n) {
+            HIPCHK(hipMemcpyAsync(keys, V->k, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(vals, V->v, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (n_out) *n_out = n;
+    });
+}
+
+int sfci_wide_hv(sfci_wctx* W, const sfci_wvec* in, sfci_wvec* out, int64_t nd, double lambda, double coeff,
+                 double* diag) {
+    if (!W || !in || !out || nd < 0 || in == out || in->ctx != &W->x->c || out->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] {
+        double d = 0.0;
+        const size_t n = dev_hv_wide(c, c->n_so, in->k, in->v, (size_t)in->len, (size_t)nd, lambda, coeff,
+                                     (size_t)out->cap, out->k, out->v, &d);
+        out->len = (int64_t)n;
+        if (diag) *diag = d;
     });
 }
 

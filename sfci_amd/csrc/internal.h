@@ -304,6 +304,10 @@ struct HvAll {
 };
 void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, double lambda, double coeff, double* diag,
                 HvAll* all);
+// H.v on 128-bit keys (wide.hip): keys are two words per entry (lo = spin orbitals 0-63,
+// hi = 64-127), ascending as hi:lo; same semantics and outputs as dev_hv.
+size_t dev_hv_wide(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t n_in, size_t nd, double lambda,
+                   double coeff, size_t out_cap, uint64_t* ok2, double* ov, double* diag);
 // <v|H|v> with calc_energy's per-source pull (no candidate list); v key-sorted.
 double dev_energy(Ctx* c, const uint64_t* k, const double* v, size_t n);
 // cisd.hip: the CISD block eigenvector (state_idx, ascending) of ref_det into (ok, ov),
@@ -378,6 +382,18 @@ struct sfci_ctx {
 };
 struct sfci_vec {
     sfci::Vec v;
+};
+// A context for more than 64 spin orbitals (128-bit keys) and its vectors: keys hold two
+// words per entry.
+struct sfci_wctx {
+    sfci_ctx* x;
+};
+struct sfci_wvec {
+    sfci::Ctx* ctx = nullptr;
+    uint64_t* k = nullptr;   // 2 * cap words
+    double* v = nullptr;
+    int64_t len = 0;
+    int64_t cap = 0;
 };
 
 namespace sfci {

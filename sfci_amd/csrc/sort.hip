@@ -738,6 +738,7 @@ template Rec32* hash_partition<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 
 template void radix_sort_kv<double>(Ctx*, uint64_t**, double**, uint64_t*, double*, size_t, int);
 template void radix_sort_kv<float>(Ctx*, uint64_t**, float**, uint64_t*, float*, size_t, int);
+template void radix_sort_kv<uint64_t>(Ctx*, uint64_t**, uint64_t**, uint64_t*, uint64_t*, size_t, int);
 template void radix_partition_hash<double>(Ctx*, uint64_t**, double**, uint64_t*, double*, size_t, int);
 template void radix_partition_hash<float>(Ctx*, uint64_t**, float**, uint64_t*, float*, size_t, int);
 

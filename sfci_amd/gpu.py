@@ -145,6 +145,20 @@ def lib():
         "sfci_set_screen": (ip, [vp, ip, vp, dbl, dbl]),
         "sfci_screen_samples": (ip, [vp, vp, dbl, i64, u64, vp, C.POINTER(C.c_int64)]),
         "sfci_get_screen": (ip, [vp, C.POINTER(C.c_int), pvp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "sfci_wide_create": (ip, [C.c_char_p, ip, ip, ip, pvp]),
+        "sfci_wide_destroy": (None, [vp]),
+        "sfci_wide_last_error": (C.c_char_p, [vp]),
+        "sfci_wide_info": (ip, [vp, C.POINTER(InfoT)]),
+        "sfci_wide_set_cut": (ip, [vp, dbl]),
+        "sfci_wide_set_precision": (ip, [vp, ip]),
+        "sfci_wide_set_workspace": (ip, [vp, u64]),
+        "sfci_wide_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
+        "sfci_wide_vec_create": (ip, [vp, i64, pvp]),
+        "sfci_wide_vec_destroy": (None, [vp]),
+        "sfci_wide_vec_len": (i64, [vp]),
+        "sfci_wide_vec_upload": (ip, [vp, vp, vp, i64]),
+        "sfci_wide_vec_download": (ip, [vp, vp, vp, i64, C.POINTER(C.c_int64)]),
+        "sfci_wide_hv": (ip, [vp, vp, vp, i64, dbl, dbl, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -461,3 +475,108 @@ class Mlp:
         x = np.zeros((len(s), MLP_IN), np.float32)
         self.ctx._check(lib().sfci_mlp_sample_features(self._h, _ptr(s), len(s), _ptr(x)))
         return x
+
+
+# ---- 128-bit keys (more than 64 spin orbitals) ----------------------------------------
+def split_keys(keys) -> tuple:
+    """Python ints -> (lo, hi) uint64 arrays (lo = spin orbitals 0-63)."""
+    lo = np.array([k & 0xFFFFFFFFFFFFFFFF for k in keys], np.uint64)
+    hi = np.array([k >> 64 for k in keys], np.uint64)
+    return lo, hi
+
+
+def join_keys(lo, hi) -> list:
+    return [(int(h) << 64) | int(l) for l, h in zip(lo, hi)]
+
+
+class WideVector:
+    """Device sparse vector with 128-bit keys (two words per entry, ascending as hi:lo)."""
+
+    def __init__(self, ctx: "WideContext", cap: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().sfci_wide_vec_create(ctx._h, int(cap), C.byref(h)))
+        self._h = h
+        self.cap = int(cap)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            lib().sfci_wide_vec_destroy(self._h)
+            self._h = None
+
+    @property
+    def len(self) -> int:
+        return lib().sfci_wide_vec_len(self._h)
+
+    def upload(self, lo, hi, vals) -> "WideVector":
+        lo = np.ascontiguousarray(lo, dtype=np.uint64)
+        hi = np.ascontiguousarray(hi, dtype=np.uint64)
+        k = np.empty(2 * len(lo), np.uint64)
+        k[0::2], k[1::2] = lo, hi
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        self.ctx._check(lib().sfci_wide_vec_upload(self._h, _ptr(k), _ptr(v), len(lo)))
+        return self
+
+    def download(self):
+        """-> (lo, hi, vals)"""
+        n = self.len
+        k, v = np.zeros(2 * max(n, 1), np.uint64), np.zeros(max(n, 1))
+        got = C.c_int64()
+        self.ctx._check(lib().sfci_wide_vec_download(self._h, _ptr(k), _ptr(v), n, C.byref(got)))
+        m = got.value
+        return k[0:2 * m:2].copy(), k[1:2 * m:2].copy(), v[:m].copy()
+
+
+class WideContext:
+    """One loaded Hamiltonian with more than 64 spin orbitals (128-bit keys) on one GPU."""
+
+    def __init__(self, in_path: str, n_so: int, n_elec: int, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().sfci_wide_create(in_path.encode(), n_so, n_elec, device, C.byref(h))
+        if rc:
+            raise SfciError(rc, "sfci_wide_create(%s)" % in_path)
+        self._h = h
+        self.n_so, self.n_elec = n_so, n_elec
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sfci_wide_destroy(self._h)
+            self._h = None
+
+    def _check(self, rc: int):
+        if rc != SFCI_OK:
+            msg = lib().sfci_wide_last_error(self._h) if self._h else b""
+            raise SfciError(rc, (msg or b"").decode())
+
+    def info(self) -> dict:
+        i = InfoT()
+        self._check(lib().sfci_wide_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in InfoT._fields_}
+
+    def set_cut(self, cut: float):
+        self._check(lib().sfci_wide_set_cut(self._h, cut))
+
+    def set_precision(self, mixed: bool):
+        self._check(lib().sfci_wide_set_precision(self._h, 1 if mixed else 0))
+
+    def set_workspace(self, nbytes: int):
+        self._check(lib().sfci_wide_set_workspace(self._h, int(nbytes)))
+
+    def stats(self) -> dict:
+        s = StatsT()
+        self._check(lib().sfci_wide_hv_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in StatsT._fields_}
+
+    def vector(self, cap: int, lo=None, hi=None, vals=None) -> WideVector:
+        v = WideVector(self, cap)
+        if lo is not None:
+            v.upload(lo, hi, vals)
+        return v
+
+    def hv(self, vin: WideVector, vout: WideVector, nd: int, lam: float = 0.0, coeff: float = 1.0) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_wide_hv(self._h, vin._h, vout._h, int(nd), lam, coeff, C.byref(d)))
+        return d.value
