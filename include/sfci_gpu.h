@@ -360,6 +360,10 @@ int sfci_wide_set_precision(sfci_wctx* ctx, int precision);
 /* Device bytes the candidate buffers of one pass may use (104 bytes per candidate). */
 int sfci_wide_set_workspace(sfci_wctx* ctx, uint64_t bytes);
 int sfci_wide_hv_stats(const sfci_wctx* ctx, sfci_hv_stats_t* st);
+/* How many bits of the key hash group candidate records before the exact merge (8..64,
+ * default 48; a tuning and test knob: fewer bits make distinct keys share groups, which
+ * are then told apart by their full keys).  Results are identical for every value. */
+int sfci_wide_set_group_bits(sfci_wctx* ctx, int bits);
 int sfci_wide_vec_create(sfci_wctx* ctx, int64_t cap, sfci_wvec** out);
 void sfci_wide_vec_destroy(sfci_wvec* v);
 int64_t sfci_wide_vec_len(const sfci_wvec* v);

@@ -727,6 +727,12 @@ int sfci_wide_set_precision(sfci_wctx* W, int precision) {
 
 int sfci_wide_set_workspace(sfci_wctx* W, uint64_t bytes) { return W ? sfci_set_workspace(W->x, bytes) : SFCI_E_ARG; }
 
+int sfci_wide_set_group_bits(sfci_wctx* W, int bits) {
+    if (!W || bits < 8 || bits > 64) return SFCI_E_ARG;
+    W->x->c.wide_group_bits = bits;
+    return SFCI_OK;
+}
+
 int sfci_wide_hv_stats(const sfci_wctx* W, sfci_hv_stats_t* s) { return W ? sfci_hv_stats(W->x, s) : SFCI_E_ARG; }
 
 int sfci_wide_vec_create(sfci_wctx* W, int64_t cap, sfci_wvec** out) {

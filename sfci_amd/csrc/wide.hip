@@ -19,10 +19,13 @@
 //               Slater-Condon element (exact limb sums); a connection is kept when its
 //               hash class is the pass, appended as (lo, hi, value) with one atomic per
 //               wavefront
-//      sort     stable LSD radix sort on lo, then on hi (index payload)
+//      group    stable LSD radix sort on a 48-bit hash of the key (index payload), so
+//               that the records of one key are adjacent (keys that share a hash form
+//               one run and are told apart inside it)
 //      reduce   exact fixed-point sum per distinct key; diag from the sources whose own
 //               key falls in the pass; |y| > cut appended to the staging list
-//   4. output : staging list sorted by key (when there were several passes), top out_cap.
+//   4. output : staging list sorted by the 128-bit key (LSD on lo, then on hi), top
+//               out_cap.
 #include "internal.h"
 
 namespace sfci {
@@ -111,21 +114,36 @@ __device__ __forceinline__ double wide_val(double hij, double temp) {
     return hij * temp;
 }
 
-// Appends the records of the lanes with `want` set: one atomic per wavefront.  Called by
-// every lane of a wavefront that is still in the enclosing loop.
-__device__ __forceinline__ void wide_append(bool want, uint64_t lo, uint64_t hi, double val, uint64_t* rlo,
-                                            uint64_t* rhi, double* rv, uint64_t cap,
-                                            unsigned long long* count, uint32_t* flags) {
-    const uint64_t bal = __ballot(want);
-    if (bal == 0) return;
-    const int lane = threadIdx.x & (SFCI_WAVE - 1);
-    const int leader = __ffsll((long long)bal) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(bal));
-    base = __shfl(base, leader, SFCI_WAVE);
-    if (want) {
-        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        const uint64_t i = base + (uint64_t)__popcll(bal & lt);
+// Where a source's records go.  Single pass: record `idx` of source j at j * n_conn + idx
+// (every connection is kept; no atomics).  Several passes: the source's in-pass records
+// are counted first and reserved with one global atomic (base); each wavefront then takes
+// its slots from a workgroup cursor in LDS with one LDS atomic.
+struct WEmit {
+    uint64_t* rlo;
+    uint64_t* rhi;
+    double* rv;
+    uint64_t cap;
+    uint64_t base;
+    unsigned* cursor;   // LDS; nullptr in single-pass mode
+    uint32_t* flags;
+    // called by every lane of a wavefront that is still in the enclosing loop
+    __device__ __forceinline__ void put(bool want, uint32_t idx, uint64_t lo, uint64_t hi, double val) const {
+        uint64_t i;
+        if (!cursor) {
+            if (!want) return;
+            i = base + idx;
+        } else {
+            const uint64_t bal = __ballot(want);
+            if (bal == 0) return;
+            const int lane = threadIdx.x & (SFCI_WAVE - 1);
+            const int leader = __ffsll((long long)bal) - 1;
+            unsigned off = 0;
+            if (lane == leader) off = atomicAdd(cursor, (unsigned)__popcll(bal));
+            off = __shfl(off, leader, SFCI_WAVE);
+            if (!want) return;
+            const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            i = base + off + (uint64_t)__popcll(bal & lt);
+        }
         if (i < cap) {
             rlo[i] = lo;
             rhi[i] = hi;
@@ -134,6 +152,43 @@ __device__ __forceinline__ void wide_append(bool want, uint64_t lo, uint64_t hi,
             atomicOr(flags, FLAG_GEN_OVERFLOW);
         }
     }
+};
+
+// Target of connection e of the staged source (e >= 1; 0 is the diagonal), in the order
+// singles (spin 0, spin 1), same-spin doubles (spin 0, spin 1), opposite-spin doubles.
+__device__ __forceinline__ void conn_target(const HamDev& H, const WSrc& s, uint32_t e, uint64_t& tl,
+                                            uint64_t& th) {
+    const uint32_t kv = (uint32_t)H.n_sa, dss = (uint32_t)H.n_daa, v = (uint32_t)H.v;
+    tl = s.lo;
+    th = s.hi;
+    uint32_t x = e - 1;
+    if (x < 2 * kv) {
+        const int sp = x >= kv ? 1 : 0;
+        x -= (uint32_t)sp * kv;
+        const uint32_t i = x / v, a = x - i * v;
+        flip(tl, th, 2 * s.occ[sp][i] + sp);
+        flip(tl, th, 2 * s.vir[sp][a] + sp);
+        return;
+    }
+    x -= 2 * kv;
+    if (x < 2 * dss) {
+        const int sp = x >= dss ? 1 : 0;
+        x -= (uint32_t)sp * dss;
+        const uint32_t op = x / H.n_vpair, vp = x - op * H.n_vpair;
+        const uint16_t po = H.occ_pairs[op], pv = H.vir_pairs[vp];
+        flip(tl, th, 2 * s.occ[sp][po & 0xff] + sp);
+        flip(tl, th, 2 * s.occ[sp][po >> 8] + sp);
+        flip(tl, th, 2 * s.vir[sp][pv & 0xff] + sp);
+        flip(tl, th, 2 * s.vir[sp][pv >> 8] + sp);
+        return;
+    }
+    x -= 2 * dss;
+    const uint32_t s0 = x / kv, s1 = x - s0 * kv;
+    const uint32_t i = s0 / v, a = s0 - i * v, jj = s1 / v, b = s1 - jj * v;
+    flip(tl, th, 2 * s.occ[0][i]);
+    flip(tl, th, 2 * s.vir[0][a]);
+    flip(tl, th, 2 * s.occ[1][jj] + 1);
+    flip(tl, th, 2 * s.vir[1][b] + 1);
 }
 
 // Connections of each source whose hash class is `pass`: (key, coeff (H - lambda)_tj c_j)
@@ -147,6 +202,8 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                                                            double* __restrict__ rv, uint64_t cap,
                                                            unsigned long long* count, uint32_t* flags) {
     __shared__ WSrc s;
+    __shared__ unsigned s_cnt, s_cur;
+    __shared__ unsigned long long s_base;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), wv = tid / SFCI_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int N = H.n_orb, k = H.k, v = H.v;
@@ -190,7 +247,30 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
         const uint64_t lo = s.lo, hi = s.hi;
         const double temp = s.temp;
         const int n_occ = s.n_occ_so;
+        const uint32_t n_conn = (uint32_t)H.n_conn;
         bool bad = false;
+        WEmit em{rlo, rhi, rv, cap, (uint64_t)j * n_conn, nullptr, flags};
+        if (n_pass > 1) {
+            // count this source's in-pass connections, reserve them with one global atomic
+            unsigned cnt = 0;
+            for (uint32_t e = tid; e < n_conn; e += W_BLOCK) {
+                uint64_t tl = lo, th = hi;
+                if (e) conn_target(H, s, e, tl, th);
+                cnt += wide_class(tl, th, n_pass) == pass ? 1u : 0u;
+            }
+            for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, SFCI_WAVE);
+            if (tid == 0) {
+                s_cnt = 0;
+                s_cur = 0;
+            }
+            __syncthreads();
+            if (lane == 0) atomicAdd(&s_cnt, cnt);
+            __syncthreads();
+            if (tid == 0) s_base = atomicAdd(count, (unsigned long long)s_cnt);
+            __syncthreads();
+            em.base = s_base;
+            em.cursor = &s_cur;
+        }
         // diagonal: E_core + sum_occ h_mm + sum_{m != p} 1/2 [(mm|pp) - (mp|pm)], exact
         if (wv == 0) {
             const bool mine = wide_class(lo, hi, n_pass) == pass;   // wave-uniform
@@ -222,7 +302,7 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                     dv = (e - lambda) * temp;
                     if (!dok || !fx_in_range(dv)) bad = true;
                 }
-                wide_append(lane == 0, lo, hi, dv, rlo, rhi, rv, cap, count, flags);
+                em.put(lane == 0, 0, lo, hi, dv);
             }
         }
         // singles, both spins: spin orbital n -> m
@@ -249,7 +329,7 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                 val = wide_val<MIXED>(hij, temp);
                 bad |= !fx_in_range(val);
             }
-            wide_append(want, tl, th, val, rlo, rhi, rv, cap, count, flags);
+            em.put(want, 1 + e, tl, th, val);
         }
         // same-spin doubles, both spins: occupied pair (i < j) -> empty pair (a < b)
         const uint32_t dss = (uint32_t)H.n_daa;
@@ -271,7 +351,7 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                 val = wide_val<MIXED>(double_elem_w(H, lo, hi, n, q, xx, y), temp);
                 bad |= !fx_in_range(val);
             }
-            wide_append(want, tl, th, val, rlo, rhi, rv, cap, count, flags);
+            em.put(want, 1 + 2 * kv + e, tl, th, val);
         }
         // opposite-spin doubles: spin-0 i -> a with spin-1 j -> b
         const uint32_t dos = kv * kv;
@@ -293,7 +373,7 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                 val = wide_val<MIXED>(double_elem_w(H, lo, hi, n, q, xx, y), temp);
                 bad |= !fx_in_range(val);
             }
-            wide_append(want, tl, th, val, rlo, rhi, rv, cap, count, flags);
+            em.put(want, 1 + 2 * kv + 2 * dss + e, tl, th, val);
         }
         if (bad) atomicOr(flags, FLAG_RANGE);
     }
@@ -346,12 +426,6 @@ __global__ void k_wide_mark(const uint64_t* __restrict__ idx, size_t cap, uint32
         fl[idx[i]] = 1u;
 }
 
-__global__ void k_wide_heads(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, size_t n,
-                             uint32_t* __restrict__ head) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0 || lo[i] != lo[i - 1] || hi[i] != hi[i - 1]) ? 1u : 0u;
-}
-
 // start[pos[i]] = i for every head; start[n_seg] = n
 __global__ void k_wide_seg_start(const uint32_t* __restrict__ head, const uint64_t* __restrict__ pos, size_t n,
                                  uint64_t* __restrict__ start) {
@@ -361,47 +435,8 @@ __global__ void k_wide_seg_start(const uint32_t* __restrict__ head, const uint64
     }
 }
 
-// exact sum of each run of equal keys
-__global__ void k_wide_seg_sum(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
-                               const double* __restrict__ v, const uint64_t* __restrict__ start, size_t n_seg,
-                               uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy) {
-    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_seg; s += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t b = start[s], e = start[s + 1];
-        fx128 acc = fx_zero();
-        for (uint64_t j = b; j < e; ++j) acc = fx_add(acc, fx_from(v[j]));
-        olo[s] = lo[b];
-        ohi[s] = hi[b];
-        oy[s] = fx_to(acc);
-    }
-}
-
 __device__ __forceinline__ bool key_less(uint64_t alo, uint64_t ahi, uint64_t blo, uint64_t bhi) {
     return ahi < bhi || (ahi == bhi && alo < blo);
-}
-
-// diag += y_t c_t for the sources t of this pass (each finds its own key among the merged
-// targets: every source emits its diagonal record)
-__global__ __launch_bounds__(W_BLOCK) void k_wide_diag(const uint64_t* __restrict__ slo,
-                                                       const uint64_t* __restrict__ shi,
-                                                       const double* __restrict__ sv, size_t ns, uint32_t n_pass,
-                                                       uint32_t pass, const uint64_t* __restrict__ ylo,
-                                                       const uint64_t* __restrict__ yhi,
-                                                       const double* __restrict__ y, size_t n_y,
-                                                       unsigned long long* acc, uint32_t* flags) {
-    fx128 mine = fx_zero();
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t lo = slo[i], hi = shi[i];
-        if (wide_class(lo, hi, n_pass) != pass) continue;
-        size_t a = 0, b = n_y;
-        while (a < b) {
-            const size_t m = (a + b) >> 1;
-            if (key_less(ylo[m], yhi[m], lo, hi)) a = m + 1;
-            else b = m;
-        }
-        if (a < n_y && ylo[a] == lo && yhi[a] == hi) mine = fx_add(mine, fx_from(y[a] * sv[i]));
-        else atomicOr(flags, FLAG_BAD_INPUT);
-    }
-    fx_block_reduce_atomic<W_BLOCK>(mine, acc);
 }
 
 __global__ void k_wide_flag_above(const double* __restrict__ y, size_t n, double cut, uint32_t* __restrict__ fl) {
@@ -426,6 +461,102 @@ __global__ void k_wide_check_sorted(const uint64_t* __restrict__ lo, const uint6
                                     uint32_t* flags) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < n; i += (size_t)gridDim.x * blockDim.x)
         if (!key_less(lo[i - 1], hi[i - 1], lo[i], hi[i])) atomicOr(flags, FLAG_BAD_INPUT);
+}
+
+// grouping hash of a key: its top `bits` bits (records are grouped by it before the
+// exact merge; 48 by default)
+__host__ __device__ __forceinline__ uint64_t group_hash(uint64_t lo, uint64_t hi, int bits) {
+    return wide_hash(lo, hi) >> (64 - bits);
+}
+
+__global__ void k_wide_group_key(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, size_t n,
+                                 int bits, uint64_t* __restrict__ key) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        key[i] = group_hash(lo[i], hi[i], bits);
+}
+
+__global__ void k_wide_run_heads(const uint64_t* __restrict__ h, size_t n, uint32_t* __restrict__ head) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || h[i] != h[i - 1]) ? 1u : 0u;
+}
+
+// Distinct keys of each run of equal grouping hash: 1 unless two keys share the hash.
+__global__ void k_wide_run_count(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                                 const uint64_t* __restrict__ start, size_t n_run, uint32_t* __restrict__ cnt) {
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_run; r += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = start[r], e = start[r + 1];
+        bool same = true;
+        for (uint64_t i = b + 1; i < e && same; ++i) same = lo[i] == lo[b] && hi[i] == hi[b];
+        uint32_t c = 1;
+        if (!same) {
+            c = 0;
+            for (uint64_t i = b; i < e; ++i) {
+                bool first = true;
+                for (uint64_t q = b; q < i && first; ++q) first = !(lo[q] == lo[i] && hi[q] == hi[i]);
+                c += first ? 1u : 0u;
+            }
+        }
+        cnt[r] = c;
+    }
+}
+
+// Exact sum of every distinct key of each run, written from opos[r] on (first occurrence
+// order inside a run); the grouping hash goes along for the diag search.
+__global__ void k_wide_run_sum(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                               const double* __restrict__ v, const uint64_t* __restrict__ h,
+                               const uint64_t* __restrict__ start, const uint64_t* __restrict__ opos, size_t n_run,
+                               uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy,
+                               uint64_t* __restrict__ oh) {
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_run; r += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = start[r], e = start[r + 1];
+        uint64_t o = opos[r];
+        for (uint64_t i = b; i < e; ++i) {
+            bool first = true;
+            for (uint64_t q = b; q < i && first; ++q) first = !(lo[q] == lo[i] && hi[q] == hi[i]);
+            if (!first) continue;
+            fx128 acc = fx_zero();
+            for (uint64_t q = i; q < e; ++q)
+                if (lo[q] == lo[i] && hi[q] == hi[i]) acc = fx_add(acc, fx_from(v[q]));
+            olo[o] = lo[i];
+            ohi[o] = hi[i];
+            oy[o] = fx_to(acc);
+            oh[o] = h[b];
+            ++o;
+        }
+    }
+}
+
+// diag += y_t c_t for the sources t of this pass; the merged list is ordered by grouping
+// hash (every source emits its diagonal record, so each finds its own key)
+__global__ __launch_bounds__(W_BLOCK) void k_wide_diag_h(const uint64_t* __restrict__ slo,
+                                                         const uint64_t* __restrict__ shi,
+                                                         const double* __restrict__ sv, size_t ns, uint32_t n_pass,
+                                                         uint32_t pass, const uint64_t* __restrict__ ylo,
+                                                         const uint64_t* __restrict__ yhi,
+                                                         const double* __restrict__ y,
+                                                         const uint64_t* __restrict__ yh, size_t n_y, int bits,
+                                                         unsigned long long* acc, uint32_t* flags) {
+    fx128 mine = fx_zero();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = slo[i], hi = shi[i];
+        if (wide_class(lo, hi, n_pass) != pass) continue;
+        const uint64_t h = group_hash(lo, hi, bits);
+        size_t a = 0, b = n_y;
+        while (a < b) {
+            const size_t m = (a + b) >> 1;
+            if (yh[m] < h) a = m + 1;
+            else b = m;
+        }
+        bool found = false;
+        for (; a < n_y && yh[a] == h; ++a)
+            if (ylo[a] == lo && yhi[a] == hi) {
+                mine = fx_add(mine, fx_from(y[a] * sv[i]));
+                found = true;
+                break;
+            }
+        if (!found) atomicOr(flags, FLAG_BAD_INPUT);
+    }
+    fx_block_reduce_atomic<W_BLOCK>(mine, acc);
 }
 
 int grid_of_w(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 256 * 16)); }
@@ -459,7 +590,7 @@ uint64_t read_u64(Ctx* c, const uint64_t* d) {
     return h;
 }
 
-// Scratch of the wide path: eleven 8-byte arrays of n entries and two of n + 1.
+// Scratch of the wide path: thirteen 8-byte arrays of n + 1 entries.
 struct WBufs {
     uint64_t* a[11];
     uint64_t* pos;
@@ -470,7 +601,7 @@ WBufs wide_bufs(Ctx* c, size_t n) {
     WBufs w;
     DevBuf* bufs[11] = {&c->cand_k[0], &c->cand_k[1], &c->cand_v[0], &c->cand_v[1], &c->tmp_k[0], &c->tmp_k[1],
                         &c->tmp_v[0],  &c->tmp_v[1],  &c->plan_first, &c->plan_str, &c->plan_cnt};
-    for (int i = 0; i < 11; ++i) w.a[i] = bufs[i]->get<uint64_t>(std::max<size_t>(n, 1));
+    for (int i = 0; i < 11; ++i) w.a[i] = bufs[i]->get<uint64_t>(n + 1);
     w.pos = c->src_off.get<uint64_t>(n + 1);
     w.start = c->part_base.get<uint64_t>(n + 1);
     return w;
@@ -627,7 +758,8 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
                                                                  rlo, rhi, rv, cap,
                                                                  (unsigned long long*)c->d_counter, c->d_flags);
             HIPCHK(hipGetLastError());
-            const uint64_t n = read_u64(c, c->d_counter);
+            const uint64_t n = n_pass == 1 ? total_cand : read_u64(c, c->d_counter);
+            HIPCHK(hipStreamSynchronize(c->stream));
             st.ms_generate += tm.stop();
             const uint32_t f = flags_read_clear(c);
             if (f & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", SFCI_E_RANGE);
@@ -640,33 +772,50 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             }
             st.n_candidates += n;
             if (n == 0) continue;
-            // sort by key
+            // group by the key hash: stable LSD sort of (hash, index), then gathers
+            const int g = grid_of_w(n);
+            const int gbits = c->wide_group_bits;
+            k_wide_group_key<<<g, 256, 0, c->stream>>>(rlo, rhi, n, gbits, w.a[3]);
+            k_wide_iota<<<g, 256, 0, c->stream>>>(w.a[4], n);
+            uint64_t* K = w.a[3];
+            uint64_t* I = w.a[4];
+            radix_sort_kv<uint64_t>(c, &K, &I, w.a[5], w.a[6], n, gbits);
             uint64_t *olo = w.a[7], *ohi = w.a[8];
             double* ovv = (double*)w.a[9];
-            sort128(c, rlo, rhi, rv, n, n_so, w.a[3], w.a[4], w.a[5], w.a[6], olo, ohi, ovv);
+            k_wide_gather<uint64_t><<<g, 256, 0, c->stream>>>(rlo, I, n, olo);
+            k_wide_gather<uint64_t><<<g, 256, 0, c->stream>>>(rhi, I, n, ohi);
+            k_wide_gather<double><<<g, 256, 0, c->stream>>>(rv, I, n, ovv);
+            HIPCHK(hipGetLastError());
             st.ms_sort += tm.stop();
-            // reduce: run heads, run starts, exact sums (into the record arrays)
-            uint32_t* head = (uint32_t*)w.a[3];
-            const int g = grid_of_w(n);
-            k_wide_heads<<<g, 256, 0, c->stream>>>(olo, ohi, n, head);
+            // runs of equal hash, distinct keys per run, exact sums (into the record arrays)
+            uint64_t* Kfree = (K == w.a[3]) ? w.a[5] : w.a[3];
+            uint64_t* Ifree = (I == w.a[4]) ? w.a[6] : w.a[4];
+            uint32_t* head = (uint32_t*)Kfree;
+            k_wide_run_heads<<<g, 256, 0, c->stream>>>(K, n, head);
             exclusive_scan_u32(c, head, n, w.pos, w.pos + n);
-            const uint64_t n_seg = read_u64(c, w.pos + n);
+            const uint64_t n_run = read_u64(c, w.pos + n);
             k_wide_seg_start<<<g, 256, 0, c->stream>>>(head, w.pos, n, w.start);
+            uint32_t* cnt = (uint32_t*)Ifree;
+            k_wide_run_count<<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, w.start, n_run, cnt);
+            exclusive_scan_u32(c, cnt, n_run, w.a[10], w.a[10] + n_run);
+            const uint64_t n_seg = read_u64(c, w.a[10] + n_run);
             uint64_t *ylo = rlo, *yhi = rhi;
             double* y = rv;
-            k_wide_seg_sum<<<grid_of_w(n_seg), 256, 0, c->stream>>>(olo, ohi, ovv, w.start, n_seg, ylo, yhi, y);
+            uint64_t* yh = I;   // the sort's index array is free after the gathers
+            k_wide_run_sum<<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, w.a[10], n_run, ylo,
+                                                                    yhi, y, yh);
             HIPCHK(hipGetLastError());
             st.n_unique += n_seg;
             // diag over the sources of this pass
             acc_clear(c, 0);
-            k_wide_diag<<<grid_of_w(ns), W_BLOCK, 0, c->stream>>>(slo, shi, sv, ns, n_pass, p, ylo, yhi, y, n_seg,
-                                                                 c->d_acc, c->d_flags);
+            k_wide_diag_h<<<grid_of_w(ns), W_BLOCK, 0, c->stream>>>(slo, shi, sv, ns, n_pass, p, ylo, yhi, y, yh,
+                                                                   n_seg, gbits, c->d_acc, c->d_flags);
             HIPCHK(hipGetLastError());
             dsum = fx_add(dsum, acc_read_raw(c, 0));
             if (flags_read_clear(c) & FLAG_BAD_INPUT)
                 throw HipError("a source's diagonal target was not found after the merge", SFCI_E_OVERFLOW);
             // cut, append to staging
-            uint32_t* fl = (uint32_t*)w.a[3];
+            uint32_t* fl = (uint32_t*)Kfree;
             k_wide_flag_above<<<grid_of_w(n_seg), 256, 0, c->stream>>>(y, n_seg, c->cut, fl);
             exclusive_scan_u32(c, fl, n_seg, w.pos, w.pos + n_seg);
             const uint64_t n_keep = read_u64(c, w.pos + n_seg);
@@ -691,13 +840,11 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
         const uint64_t* flo = stg.lo;
         const uint64_t* fhi = stg.hi;
         const double* fy = stg.y;
-        if (n_pass > 1) {
-            sort128(c, stg.lo, stg.hi, stg.y, m, n_so, w2.a[0], w2.a[1], w2.a[2], w2.a[3], w2.a[4], w2.a[5],
-                    (double*)w2.a[6]);
-            flo = w2.a[4];
-            fhi = w2.a[5];
-            fy = (double*)w2.a[6];
-        }
+        sort128(c, stg.lo, stg.hi, stg.y, m, n_so, w2.a[0], w2.a[1], w2.a[2], w2.a[3], w2.a[4], w2.a[5],
+                (double*)w2.a[6]);
+        flo = w2.a[4];
+        fhi = w2.a[5];
+        fy = (double*)w2.a[6];
         uint64_t *glo = w2.a[7], *ghi = w2.a[8];
         double* gy = (double*)w2.a[9];
         n_out = select128(c, flo, fhi, fy, m, out_cap, w2.a[0], w2.a[1], w2.a[2], w2.a[3], w2.pos, glo, ghi, gy);

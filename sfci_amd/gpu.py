@@ -153,6 +153,7 @@ def lib():
         "sfci_wide_set_precision": (ip, [vp, ip]),
         "sfci_wide_set_workspace": (ip, [vp, u64]),
         "sfci_wide_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
+        "sfci_wide_set_group_bits": (ip, [vp, ip]),
         "sfci_wide_vec_create": (ip, [vp, i64, pvp]),
         "sfci_wide_vec_destroy": (None, [vp]),
         "sfci_wide_vec_len": (i64, [vp]),
@@ -564,6 +565,9 @@ class WideContext:
 
     def set_workspace(self, nbytes: int):
         self._check(lib().sfci_wide_set_workspace(self._h, int(nbytes)))
+
+    def set_group_bits(self, bits: int):
+        self._check(lib().sfci_wide_set_group_bits(self._h, int(bits)))
 
     def stats(self) -> dict:
         s = StatsT()

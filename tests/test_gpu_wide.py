@@ -98,6 +98,21 @@ def test_wide_hv_multipass(wctx, worc, winput):
     assert bits([gd])[0] == bits([od])[0]
 
 
+def test_wide_hv_shared_group_hashes(wctx, worc, winput):
+    """With 16 grouping bits many distinct keys share a group: they must still merge exactly."""
+    lo, hi, v = winput
+    wctx.set_group_bits(16)
+    try:
+        gl, gh, gv, gd = gpu_wide_hv(wctx, lo, hi, v, 300, 0.1, 1.0, 60_000_000)
+        st = wctx.stats()
+    finally:
+        wctx.set_group_bits(48)
+    ol, oh, ov, od = worc.hv_wide(lo, hi, v, nd=300, lam=0.1)
+    assert st["n_unique"] > 4 * 65536     # far more distinct targets than groups
+    assert_same(gl, gh, gv, ol, oh, ov)
+    assert bits([gd])[0] == bits([od])[0]
+
+
 def test_wide_path_matches_64bit_path(make_system):
     """Below 65 spin orbitals the 128-bit path must reproduce the 64-bit sfci_hv."""
     from sfci_amd import gpu
