@@ -480,53 +480,34 @@ __global__ void k_wide_run_heads(const uint64_t* __restrict__ h, size_t n, uint3
         head[i] = (i == 0 || h[i] != h[i - 1]) ? 1u : 0u;
 }
 
-// Runs of equal grouping hash hold one key, or a few keys that share the hash.  A run is
-// walked key by key: the next key is the first record that differs from every key already
-// done (up to RUN_KEYS of them kept in registers; beyond that, the record's earlier
-// occurrences are searched), so a run of L records with d keys costs O(L d).
-constexpr int RUN_KEYS = 8;
-
-__device__ __forceinline__ bool seen_before(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
-                                            uint64_t b, uint64_t i, const uint64_t* klo, const uint64_t* khi,
-                                            int nk) {
-    if (nk <= RUN_KEYS) {
-        for (int t = 0; t < nk; ++t)
-            if (klo[t] == lo[i] && khi[t] == hi[i]) return true;
-        return false;
-    }
-    for (uint64_t q = b; q < i; ++q)
-        if (lo[q] == lo[i] && hi[q] == hi[i]) return true;
-    return false;
-}
-
-// Distinct keys of each run (count[r]); with out != nullptr also their exact sums, written
-// from opos[r] on in first-occurrence order, with the run's hash for the diag search.
+// Runs of equal grouping hash hold one key, or a few keys that share the hash.  A run of
+// one key is summed in one scan.  A shared run is walked key by key: the first record not
+// yet taken starts a key, and one scan takes (marks in `taken`, zero on entry) and sums
+// every record of that key, so a run of L records with d keys costs O(L d).
+// Distinct keys of each run go to cnt[r]; with SUM also their exact sums, written from
+// opos[r] on in first-occurrence order, with the run's hash for the diag search.
 template <bool SUM>
 __global__ void k_wide_runs(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
                             const double* __restrict__ v, const uint64_t* __restrict__ h,
-                            const uint64_t* __restrict__ start, size_t n_run, uint32_t* __restrict__ cnt,
-                            const uint64_t* __restrict__ opos, uint64_t* __restrict__ olo,
-                            uint64_t* __restrict__ ohi, double* __restrict__ oy, uint64_t* __restrict__ oh) {
+                            const uint64_t* __restrict__ start, size_t n_run, uint32_t* __restrict__ taken,
+                            uint32_t* __restrict__ cnt, const uint64_t* __restrict__ opos,
+                            uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy,
+                            uint64_t* __restrict__ oh) {
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_run; r += (size_t)gridDim.x * blockDim.x) {
         const uint64_t b = start[r], e = start[r + 1];
-        uint64_t klo[RUN_KEYS], khi[RUN_KEYS];
-        int nk = 0;
+        bool same = true;
+        for (uint64_t i = b + 1; i < e && same; ++i) same = lo[i] == lo[b] && hi[i] == hi[b];
         uint64_t o = SUM ? opos[r] : 0;
-        uint64_t i = b;
-        while (i < e) {
+        uint32_t nk = 0;
+        for (uint64_t i = b; i < e; ++i) {
+            if (!same && taken[i]) continue;
             const uint64_t cl = lo[i], ch = hi[i];
             fx128 acc = fx_zero();
-            uint64_t next = e;   // first record after i with a key not done yet
             for (uint64_t q = i; q < e; ++q) {
                 if (lo[q] == cl && hi[q] == ch) {
+                    if (!same) taken[q] = 1u;
                     if (SUM) acc = fx_add(acc, fx_from(v[q]));
-                } else if (next == e && !(q > i && seen_before(lo, hi, b, q, klo, khi, nk))) {
-                    next = q;
                 }
-            }
-            if (nk < RUN_KEYS) {
-                klo[nk] = cl;
-                khi[nk] = ch;
             }
             ++nk;
             if (SUM) {
@@ -536,9 +517,9 @@ __global__ void k_wide_runs(const uint64_t* __restrict__ lo, const uint64_t* __r
                 oh[o] = h[b];
                 ++o;
             }
-            i = next;
+            if (same) break;
         }
-        if (!SUM) cnt[r] = (uint32_t)nk;
+        if (!SUM) cnt[r] = nk;
     }
 }
 
@@ -812,15 +793,19 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             const uint64_t n_run = read_u64(c, w.pos + n);
             k_wide_seg_start<<<g, 256, 0, c->stream>>>(head, w.pos, n, w.start);
             uint32_t* cnt = (uint32_t*)Ifree;
-            k_wide_runs<false><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, cnt,
-                                                                       nullptr, nullptr, nullptr, nullptr, nullptr);
+            uint32_t* taken = head;   // the head flags are free once the run starts are known
+            HIPCHK(hipMemsetAsync(taken, 0, n * 4, c->stream));
+            k_wide_runs<false><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, taken,
+                                                                       cnt, nullptr, nullptr, nullptr, nullptr,
+                                                                       nullptr);
             exclusive_scan_u32(c, cnt, n_run, w.a[10], w.a[10] + n_run);
             const uint64_t n_seg = read_u64(c, w.a[10] + n_run);
             uint64_t *ylo = rlo, *yhi = rhi;
             double* y = rv;
             uint64_t* yh = I;   // the sort's index array is free after the gathers
-            k_wide_runs<true><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, nullptr,
-                                                                      w.a[10], ylo, yhi, y, yh);
+            HIPCHK(hipMemsetAsync(taken, 0, n * 4, c->stream));
+            k_wide_runs<true><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, taken,
+                                                                      nullptr, w.a[10], ylo, yhi, y, yh);
             HIPCHK(hipGetLastError());
             st.n_unique += n_seg;
             // diag over the sources of this pass
