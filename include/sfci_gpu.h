@@ -361,7 +361,7 @@ int sfci_wide_set_precision(sfci_wctx* ctx, int precision);
 int sfci_wide_set_workspace(sfci_wctx* ctx, uint64_t bytes);
 int sfci_wide_hv_stats(const sfci_wctx* ctx, sfci_hv_stats_t* st);
 /* How many bits of the key hash group candidate records before the exact merge (8..64,
- * default 32; a tuning and test knob: fewer bits make distinct keys share groups, which
+ * default 48; a tuning and test knob: fewer bits make distinct keys share groups, which
  * are then told apart by their full keys).  Results are identical for every value. */
 int sfci_wide_set_group_bits(sfci_wctx* ctx, int bits);
 int sfci_wide_vec_create(sfci_wctx* ctx, int64_t cap, sfci_wvec** out);

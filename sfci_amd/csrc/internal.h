@@ -230,8 +230,8 @@ struct Ctx {
     bool g_one_block = false;        // the four two-electron spin blocks share one table
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
-    int merge_hash_bits = 0;
-    int wide_group_bits = 32;        // 128-bit-key H.v: bits of the key hash that group records      // distinct targets per candidate, from the last H.v (sizes the hash buckets)
+    int merge_hash_bits = 0;         // hash bits of merge_mode 2
+    int wide_group_bits = 48;        // 128-bit-key H.v: bits of the key hash that group records
     // candidate screening of sfci_hv (0 off, 1 by |value|, 2 by the network)
     int screen_mode = 0;
     const sfci_mlp* screen_net = nullptr;

@@ -19,7 +19,7 @@
 //               Slater-Condon element (exact limb sums); a connection is kept when its
 //               hash class is the pass, appended as (lo, hi, value) with one atomic per
 //               wavefront
-//      group    stable LSD radix sort on a 32-bit hash of the key (index payload), so
+//      group    stable LSD radix sort on a 48-bit hash of the key (index payload), so
 //               that the records of one key are adjacent (keys that share a hash form
 //               one run and are told apart inside it)
 //      reduce   exact fixed-point sum per distinct key; diag from the sources whose own
@@ -464,7 +464,7 @@ __global__ void k_wide_check_sorted(const uint64_t* __restrict__ lo, const uint6
 }
 
 // grouping hash of a key: its top `bits` bits (records are grouped by it before the
-// exact merge; 32 by default)
+// exact merge; 48 by default)
 __host__ __device__ __forceinline__ uint64_t group_hash(uint64_t lo, uint64_t hi, int bits) {
     return wide_hash(lo, hi) >> (64 - bits);
 }
@@ -480,46 +480,49 @@ __global__ void k_wide_run_heads(const uint64_t* __restrict__ h, size_t n, uint3
         head[i] = (i == 0 || h[i] != h[i - 1]) ? 1u : 0u;
 }
 
-// Runs of equal grouping hash hold one key, or a few keys that share the hash.  A run of
-// one key is summed in one scan.  A shared run is walked key by key: the first record not
-// yet taken starts a key, and one scan takes (marks in `taken`, zero on entry) and sums
-// every record of that key, so a run of L records with d keys costs O(L d).
-// Distinct keys of each run go to cnt[r]; with SUM also their exact sums, written from
-// opos[r] on in first-occurrence order, with the run's hash for the diag search.
-template <bool SUM>
-__global__ void k_wide_runs(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
-                            const double* __restrict__ v, const uint64_t* __restrict__ h,
-                            const uint64_t* __restrict__ start, size_t n_run, uint32_t* __restrict__ taken,
-                            uint32_t* __restrict__ cnt, const uint64_t* __restrict__ opos,
-                            uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy,
-                            uint64_t* __restrict__ oh) {
+// Distinct keys of each run of equal grouping hash: 1 unless two keys share the hash.
+__global__ void k_wide_run_count(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                                 const uint64_t* __restrict__ start, size_t n_run, uint32_t* __restrict__ cnt) {
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_run; r += (size_t)gridDim.x * blockDim.x) {
         const uint64_t b = start[r], e = start[r + 1];
         bool same = true;
         for (uint64_t i = b + 1; i < e && same; ++i) same = lo[i] == lo[b] && hi[i] == hi[b];
-        uint64_t o = SUM ? opos[r] : 0;
-        uint32_t nk = 0;
-        for (uint64_t i = b; i < e; ++i) {
-            if (!same && taken[i]) continue;
-            const uint64_t cl = lo[i], ch = hi[i];
-            fx128 acc = fx_zero();
-            for (uint64_t q = i; q < e; ++q) {
-                if (lo[q] == cl && hi[q] == ch) {
-                    if (!same) taken[q] = 1u;
-                    if (SUM) acc = fx_add(acc, fx_from(v[q]));
-                }
+        uint32_t c = 1;
+        if (!same) {
+            c = 0;
+            for (uint64_t i = b; i < e; ++i) {
+                bool first = true;
+                for (uint64_t q = b; q < i && first; ++q) first = !(lo[q] == lo[i] && hi[q] == hi[i]);
+                c += first ? 1u : 0u;
             }
-            ++nk;
-            if (SUM) {
-                olo[o] = cl;
-                ohi[o] = ch;
-                oy[o] = fx_to(acc);
-                oh[o] = h[b];
-                ++o;
-            }
-            if (same) break;
         }
-        if (!SUM) cnt[r] = nk;
+        cnt[r] = c;
+    }
+}
+
+// Exact sum of every distinct key of each run, written from opos[r] on (first occurrence
+// order inside a run); the grouping hash goes along for the diag search.
+__global__ void k_wide_run_sum(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                               const double* __restrict__ v, const uint64_t* __restrict__ h,
+                               const uint64_t* __restrict__ start, const uint64_t* __restrict__ opos, size_t n_run,
+                               uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy,
+                               uint64_t* __restrict__ oh) {
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_run; r += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = start[r], e = start[r + 1];
+        uint64_t o = opos[r];
+        for (uint64_t i = b; i < e; ++i) {
+            bool first = true;
+            for (uint64_t q = b; q < i && first; ++q) first = !(lo[q] == lo[i] && hi[q] == hi[i]);
+            if (!first) continue;
+            fx128 acc = fx_zero();
+            for (uint64_t q = i; q < e; ++q)
+                if (lo[q] == lo[i] && hi[q] == hi[i]) acc = fx_add(acc, fx_from(v[q]));
+            olo[o] = lo[i];
+            ohi[o] = hi[i];
+            oy[o] = fx_to(acc);
+            oh[o] = h[b];
+            ++o;
+        }
     }
 }
 
@@ -793,19 +796,14 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             const uint64_t n_run = read_u64(c, w.pos + n);
             k_wide_seg_start<<<g, 256, 0, c->stream>>>(head, w.pos, n, w.start);
             uint32_t* cnt = (uint32_t*)Ifree;
-            uint32_t* taken = head;   // the head flags are free once the run starts are known
-            HIPCHK(hipMemsetAsync(taken, 0, n * 4, c->stream));
-            k_wide_runs<false><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, taken,
-                                                                       cnt, nullptr, nullptr, nullptr, nullptr,
-                                                                       nullptr);
+            k_wide_run_count<<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, w.start, n_run, cnt);
             exclusive_scan_u32(c, cnt, n_run, w.a[10], w.a[10] + n_run);
             const uint64_t n_seg = read_u64(c, w.a[10] + n_run);
             uint64_t *ylo = rlo, *yhi = rhi;
             double* y = rv;
             uint64_t* yh = I;   // the sort's index array is free after the gathers
-            HIPCHK(hipMemsetAsync(taken, 0, n * 4, c->stream));
-            k_wide_runs<true><<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, n_run, taken,
-                                                                      nullptr, w.a[10], ylo, yhi, y, yh);
+            k_wide_run_sum<<<grid_of_w(n_run), 256, 0, c->stream>>>(olo, ohi, ovv, K, w.start, w.a[10], n_run, ylo,
+                                                                    yhi, y, yh);
             HIPCHK(hipGetLastError());
             st.n_unique += n_seg;
             // diag over the sources of this pass

@@ -106,7 +106,7 @@ def test_wide_hv_shared_group_hashes(wctx, worc, winput):
         gl, gh, gv, gd = gpu_wide_hv(wctx, lo, hi, v, 300, 0.1, 1.0, 60_000_000)
         st = wctx.stats()
     finally:
-        wctx.set_group_bits(32)
+        wctx.set_group_bits(48)
     ol, oh, ov, od = worc.hv_wide(lo, hi, v, nd=300, lam=0.1)
     assert st["n_unique"] > 4 * 65536     # far more distinct targets than groups
     assert_same(gl, gh, gv, ol, oh, ov)
