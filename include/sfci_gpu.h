@@ -373,6 +373,28 @@ int sfci_wide_vec_download(const sfci_wvec* v, uint64_t* keys, double* vals, int
 /* sfci_hv on 128-bit keys. */
 int sfci_wide_hv(sfci_wctx* ctx, const sfci_wvec* in, sfci_wvec* out, int64_t nd, double lambda, double coeff,
                  double* diag);
+/* The vector operations of the ground-state driver on 128-bit keys, with the semantics of
+ * sfci_vec_copy, sfci_vec_clear, sfci_dot, sfci_norm, sfci_scale, sfci_add_two,
+ * sfci_shrink_from and sfci_lincomb (dynVec, src/all.h:249-480,610-623; restart
+ * recombination, src/sparse_solver.cpp:492-516). */
+int sfci_wide_vec_copy(sfci_wvec* dst, const sfci_wvec* src);
+int sfci_wide_vec_clear(sfci_wvec* v);
+int sfci_wide_dot(sfci_wctx* ctx, const sfci_wvec* a, const sfci_wvec* b, double* out);
+int sfci_wide_norm(sfci_wctx* ctx, const sfci_wvec* a, double* out);
+int sfci_wide_scale(sfci_wctx* ctx, sfci_wvec* a, double s);
+int sfci_wide_add_two(sfci_wctx* ctx, sfci_wvec* v, const sfci_wvec* w, double a);
+int sfci_wide_shrink_from(sfci_wctx* ctx, sfci_wvec* dst, const sfci_wvec* src);
+int sfci_wide_lincomb(sfci_wctx* ctx, const sfci_wvec* const* vecs, const double* c, int n, sfci_wvec* out);
+/* <v|H|v> (calc_energy, src/functions.cpp:1272-1313) as the exact diag of H.v over all of v:
+ * sums exact, rounded per target value and once for the total (the 64-bit sfci_energy
+ * rounds per source instead; the two agree to the last few bits). */
+int sfci_wide_energy(sfci_wctx* ctx, const sfci_wvec* v, double* e);
+int sfci_wide_sync(sfci_wctx* ctx);
+/* sparse_krylov_solve_eigenval (src/sparse_solver.cpp:246-548) on 128-bit keys: the
+ * driver of sfci_ground_state without candidate screening.  init_det is two words
+ * (init_det[0] = spin orbitals 0-63). */
+int sfci_wide_ground_state(sfci_wctx* ctx, const uint64_t* init_det, const sfci_lanczos_params_t* params,
+                           double* ritz, double* e_trunc, sfci_lanczos_result_t* result, sfci_wvec* v_out);
 
 #ifdef __cplusplus
 }

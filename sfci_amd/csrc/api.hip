@@ -815,4 +815,76 @@ int sfci_wide_hv(sfci_wctx* W, const sfci_wvec* in, sfci_wvec* out, int64_t nd, 
     });
 }
 
+int sfci_wide_vec_copy(sfci_wvec* D, const sfci_wvec* S) {
+    if (!D || !S || D->ctx != S->ctx || S->len > D->cap) return SFCI_E_ARG;
+    if (D == S) return SFCI_OK;
+    Ctx* c = D->ctx;
+    return guarded(c, [&] {
+        if (S->len) {
+            HIPCHK(hipMemcpyAsync(D->k, S->k, (size_t)S->len * 16, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(D->v, S->v, (size_t)S->len * 8, hipMemcpyDeviceToDevice, c->stream));
+        }
+        D->len = S->len;
+    });
+}
+
+int sfci_wide_vec_clear(sfci_wvec* V) {
+    if (!V) return SFCI_E_ARG;
+    V->len = 0;
+    return SFCI_OK;
+}
+
+int sfci_wide_dot(sfci_wctx* W, const sfci_wvec* a, const sfci_wvec* b, double* out) {
+    if (!W || !a || !b || !out || a->ctx != &W->x->c || b->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] { *out = dev_wide_dot(c, a->k, a->v, (size_t)a->len, b->k, b->v, (size_t)b->len); });
+}
+
+int sfci_wide_norm(sfci_wctx* W, const sfci_wvec* a, double* out) {
+    if (!W || !a || !out || a->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] { *out = dev_norm(c, a->v, (size_t)a->len); });
+}
+
+int sfci_wide_scale(sfci_wctx* W, sfci_wvec* a, double s) {
+    if (!W || !a || a->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] { dev_scale(c, a->v, (size_t)a->len, s); });
+}
+
+int sfci_wide_add_two(sfci_wctx* W, sfci_wvec* v, const sfci_wvec* w, double a) {
+    if (!W || !v || !w || v == w || v->ctx != &W->x->c || w->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] {
+        v->len = (int64_t)dev_wide_add_two(c, v->k, v->v, (size_t)v->len, w->k, w->v, (size_t)w->len, a,
+                                           (size_t)v->cap, v->k, v->v);
+    });
+}
+
+int sfci_wide_shrink_from(sfci_wctx* W, sfci_wvec* dst, const sfci_wvec* src) {
+    if (!W || !dst || !src || dst->ctx != &W->x->c || src->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] {
+        dst->len = (int64_t)dev_wide_shrink(c, src->k, src->v, (size_t)src->len, (size_t)dst->cap, dst->k, dst->v);
+    });
+}
+
+int sfci_wide_lincomb(sfci_wctx* W, const sfci_wvec* const* vecs, const double* coef, int n, sfci_wvec* out) {
+    if (!W || !vecs || !coef || n < 1 || !out || out->ctx != &W->x->c) return SFCI_E_ARG;
+    for (int i = 0; i < n; ++i)
+        if (!vecs[i] || vecs[i]->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] {
+        out->len = (int64_t)dev_wide_lincomb(c, vecs, coef, n, (size_t)out->cap, out->k, out->v);
+    });
+}
+
+int sfci_wide_energy(sfci_wctx* W, const sfci_wvec* v, double* e) {
+    if (!W || !v || !e || v->ctx != &W->x->c) return SFCI_E_ARG;
+    Ctx* c = &W->x->c;
+    return guarded(c, [&] { *e = dev_wide_energy(c, v->k, v->v, (size_t)v->len); });
+}
+
+int sfci_wide_sync(sfci_wctx* W) { return W ? sfci_sync(W->x) : SFCI_E_ARG; }
+
 }  // extern "C"

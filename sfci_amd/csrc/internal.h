@@ -309,6 +309,14 @@ void dev_hv_all(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, doubl
 // hi = 64-127), ascending as hi:lo; same semantics and outputs as dev_hv.
 size_t dev_hv_wide(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t n_in, size_t nd, double lambda,
                    double coeff, size_t out_cap, uint64_t* ok2, double* ov, double* diag);
+// Vector operations on two-word keys (wide.hip), same semantics as their 64-bit forms.
+double dev_wide_dot(Ctx* c, const uint64_t* ak2, const double* av, size_t na, const uint64_t* bk2, const double* bv,
+                    size_t nb);
+size_t dev_wide_add_two(Ctx* c, const uint64_t* vk2, const double* vv, size_t nv, const uint64_t* wk2,
+                        const double* wv, size_t nw, double a, size_t cap, uint64_t* ok2, double* ov);
+size_t dev_wide_shrink(Ctx* c, const uint64_t* sk2, const double* sv, size_t n, size_t cap, uint64_t* ok2,
+                       double* ov);
+double dev_wide_energy(Ctx* c, const uint64_t* k2, const double* v, size_t n);
 // <v|H|v> with calc_energy's per-source pull (no candidate list); v key-sorted.
 double dev_energy(Ctx* c, const uint64_t* k, const double* v, size_t n);
 // cisd.hip: the CISD block eigenvector (state_idx, ascending) of ref_det into (ok, ov),
@@ -396,6 +404,10 @@ struct sfci_wvec {
     int64_t len = 0;
     int64_t cap = 0;
 };
+namespace sfci {
+size_t dev_wide_lincomb(Ctx* c, const sfci_wvec* const* vecs, const double* coef, int nvec, size_t cap,
+                        uint64_t* ok2, double* ov);
+}  // namespace sfci
 
 namespace sfci {
 

@@ -559,6 +559,77 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_diag_h(const uint64_t* __restr
     fx_block_reduce_atomic<W_BLOCK>(mine, acc);
 }
 
+// ---- vector operations on two-word keys ----------------------------------------------
+__global__ void k_wide_keyheads(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, size_t n,
+                                uint32_t* __restrict__ head) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || lo[i] != lo[i - 1] || hi[i] != hi[i - 1]) ? 1u : 0u;
+}
+
+// One value per distinct key of a key-sorted list: the exact fixed-point sum of its entries
+// (EXACT), or the plain double sum of its (at most two) entries.
+template <bool EXACT>
+__global__ void k_wide_keymerge(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                                const double* __restrict__ v, const uint64_t* __restrict__ start, size_t n_seg,
+                                uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ ov,
+                                uint32_t* flags) {
+    bool bad = false;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_seg; s += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = start[s], e = start[s + 1];
+        double y;
+        if (EXACT) {
+            fx128 acc = fx_zero();
+            for (uint64_t j = b; j < e; ++j) {
+                bad |= !fx_in_range(v[j]);
+                acc = fx_add(acc, fx_from(v[j]));
+            }
+            y = fx_to(acc);
+        } else {
+            y = v[b];
+            for (uint64_t j = b + 1; j < e; ++j) y = y + v[j];
+        }
+        olo[s] = lo[b];
+        ohi[s] = hi[b];
+        ov[s] = y;
+    }
+    if (bad) atomicOr(flags, FLAG_RANGE);
+}
+
+// dst[off + i] = (key i of a two-word-key vector, a * value i)
+__global__ void k_wide_scaled_split(const uint64_t* __restrict__ k2, const double* __restrict__ v, size_t n, double a,
+                                    uint64_t* __restrict__ lo, uint64_t* __restrict__ hi, double* __restrict__ vo) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        lo[i] = k2[2 * i];
+        hi[i] = k2[2 * i + 1];
+        vo[i] = a * v[i];
+    }
+}
+
+// exact sum of a_i b_t over the keys the two (key-sorted) vectors share
+__global__ __launch_bounds__(W_BLOCK) void k_wide_dot(const uint64_t* __restrict__ ak2, const double* __restrict__ av,
+                                                      size_t na, const uint64_t* __restrict__ bk2,
+                                                      const double* __restrict__ bv, size_t nb,
+                                                      unsigned long long* acc, uint32_t* flags) {
+    fx128 mine = fx_zero();
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < na; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = ak2[2 * i], hi = ak2[2 * i + 1];
+        size_t a = 0, b = nb;
+        while (a < b) {
+            const size_t m = (a + b) >> 1;
+            if (key_less(bk2[2 * m], bk2[2 * m + 1], lo, hi)) a = m + 1;
+            else b = m;
+        }
+        if (a < nb && bk2[2 * a] == lo && bk2[2 * a + 1] == hi) {
+            const double p = av[i] * bv[a];
+            bad |= !fx_in_range(p);
+            mine = fx_add(mine, fx_from(p));
+        }
+    }
+    if (bad) atomicOr(flags, FLAG_RANGE);
+    fx_block_reduce_atomic<W_BLOCK>(mine, acc);
+}
+
 int grid_of_w(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 256 * 16)); }
 
 struct WTimer {
@@ -865,6 +936,107 @@ size_t dev_hv_wide(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size
                    double coeff, size_t out_cap, uint64_t* ok2, double* ov, double* diag) {
     return c->precision ? wide_hv_t<true>(c, n_so, ik2, iv, n_in, nd, lambda, coeff, out_cap, ok2, ov, diag)
                         : wide_hv_t<false>(c, n_so, ik2, iv, n_in, nd, lambda, coeff, out_cap, ok2, ov, diag);
+}
+
+
+// Key-sorted distinct keys of the concatenated list (lo, hi, v)[n] (n <= the WBufs size):
+// sort, run heads, merged values into (olo, ohi, ov).  Uses a[0..7], pos and start.
+static size_t wide_merge(Ctx* c, const WBufs& w, const uint64_t* lo, const uint64_t* hi, const double* v, size_t n,
+                         bool exact, uint64_t* olo, uint64_t* ohi, double* ov) {
+    if (n == 0) return 0;
+    sort128(c, lo, hi, v, n, c->n_so, w.a[0], w.a[1], w.a[2], w.a[3], w.a[4], w.a[5], (double*)w.a[6]);
+    uint32_t* head = (uint32_t*)w.a[7];
+    const int g = grid_of_w(n);
+    k_wide_keyheads<<<g, 256, 0, c->stream>>>(w.a[4], w.a[5], n, head);
+    exclusive_scan_u32(c, head, n, w.pos, w.pos + n);
+    const uint64_t n_seg = read_u64(c, w.pos + n);
+    k_wide_seg_start<<<g, 256, 0, c->stream>>>(head, w.pos, n, w.start);
+    if (exact)
+        k_wide_keymerge<true><<<grid_of_w(n_seg), 256, 0, c->stream>>>(w.a[4], w.a[5], (double*)w.a[6], w.start,
+                                                                       n_seg, olo, ohi, ov, c->d_flags);
+    else
+        k_wide_keymerge<false><<<grid_of_w(n_seg), 256, 0, c->stream>>>(w.a[4], w.a[5], (double*)w.a[6], w.start,
+                                                                        n_seg, olo, ohi, ov, c->d_flags);
+    HIPCHK(hipGetLastError());
+    if (flags_read_clear(c) & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", SFCI_E_RANGE);
+    return n_seg;
+}
+
+double dev_wide_dot(Ctx* c, const uint64_t* ak2, const double* av, size_t na, const uint64_t* bk2, const double* bv,
+                    size_t nb) {
+    acc_clear(c, 1);
+    if (na && nb)
+        k_wide_dot<<<grid_of_w(na), W_BLOCK, 0, c->stream>>>(ak2, av, na, bk2, bv, nb, c->d_acc + 2, c->d_flags);
+    HIPCHK(hipGetLastError());
+    const double d = acc_read(c, 1);
+    if (flags_read_clear(c) & FLAG_RANGE) throw HipError("dot: term outside the fixed-point range", SFCI_E_RANGE);
+    return d;
+}
+
+size_t dev_wide_add_two(Ctx* c, const uint64_t* vk2, const double* vv, size_t nv, const uint64_t* wk2,
+                        const double* wv, size_t nw, double a, size_t cap, uint64_t* ok2, double* ov) {
+    const size_t n = nv + nw;
+    if (n == 0) return 0;
+    WBufs w = wide_bufs(c, n);
+    uint64_t *clo = w.a[8], *chi = w.a[9];
+    double* cv = (double*)w.a[10];
+    if (nv) k_wide_scaled_split<<<grid_of_w(nv), 256, 0, c->stream>>>(vk2, vv, nv, 1.0, clo, chi, cv);
+    if (nw) k_wide_scaled_split<<<grid_of_w(nw), 256, 0, c->stream>>>(wk2, wv, nw, a, clo + nv, chi + nv, cv + nv);
+    // the scan helpers need a[7]/pos/start only after the sort has consumed the list
+    uint64_t *mlo = w.a[8], *mhi = w.a[9];
+    double* mv = (double*)w.a[10];
+    const size_t nm = wide_merge(c, w, clo, chi, cv, n, false, mlo, mhi, mv);
+    const size_t n_out = select128(c, mlo, mhi, mv, nm, cap, w.a[0], w.a[1], w.a[2], w.a[3], w.pos, w.a[4], w.a[5],
+                                   (double*)w.a[6]);
+    k_wide_join<<<grid_of_w(n_out), 256, 0, c->stream>>>(w.a[4], w.a[5], (double*)w.a[6], n_out, ok2, ov);
+    HIPCHK(hipGetLastError());
+    return n_out;
+}
+
+size_t dev_wide_shrink(Ctx* c, const uint64_t* sk2, const double* sv, size_t n, size_t cap, uint64_t* ok2,
+                       double* ov) {
+    if (n == 0) return 0;
+    WBufs w = wide_bufs(c, n);
+    k_wide_scaled_split<<<grid_of_w(n), 256, 0, c->stream>>>(sk2, sv, n, 1.0, w.a[8], w.a[9], (double*)w.a[10]);
+    const size_t n_out = select128(c, w.a[8], w.a[9], (double*)w.a[10], n, cap, w.a[0], w.a[1], w.a[2], w.a[3],
+                                   w.pos, w.a[4], w.a[5], (double*)w.a[6]);
+    k_wide_join<<<grid_of_w(n_out), 256, 0, c->stream>>>(w.a[4], w.a[5], (double*)w.a[6], n_out, ok2, ov);
+    HIPCHK(hipGetLastError());
+    return n_out;
+}
+
+size_t dev_wide_lincomb(Ctx* c, const sfci_wvec* const* vecs, const double* coef, int nvec, size_t cap, uint64_t* ok2,
+                        double* ov) {
+    size_t n = 0;
+    for (int i = 0; i < nvec; ++i) n += (size_t)vecs[i]->len;
+    if (n == 0) return 0;
+    WBufs w = wide_bufs(c, n);
+    size_t o = 0;
+    for (int i = 0; i < nvec; ++i) {
+        const size_t l = (size_t)vecs[i]->len;
+        if (l)
+            k_wide_scaled_split<<<grid_of_w(l), 256, 0, c->stream>>>(vecs[i]->k, vecs[i]->v, l, coef[i], w.a[8] + o,
+                                                                     w.a[9] + o, (double*)w.a[10] + o);
+        o += l;
+    }
+    const size_t nm = wide_merge(c, w, w.a[8], w.a[9], (double*)w.a[10], n, true, w.a[8], w.a[9], (double*)w.a[10]);
+    const size_t n_out = select128(c, w.a[8], w.a[9], (double*)w.a[10], nm, cap, w.a[0], w.a[1], w.a[2], w.a[3],
+                                   w.pos, w.a[4], w.a[5], (double*)w.a[6]);
+    k_wide_join<<<grid_of_w(n_out), 256, 0, c->stream>>>(w.a[4], w.a[5], (double*)w.a[6], n_out, ok2, ov);
+    HIPCHK(hipGetLastError());
+    return n_out;
+}
+
+
+// <v|H|v>: the exact diag of H.v over all of v (every sum exact, one rounding per target
+// value and one for the total; the 64-bit calc_energy pull rounds per source instead).
+double dev_wide_energy(Ctx* c, const uint64_t* k2, const double* v, size_t n) {
+    if (n == 0) return 0.0;
+    uint64_t* ok2 = c->lin_v[1].get<uint64_t>(4);
+    double* ov = (double*)(ok2 + 2);
+    double d = 0.0;
+    dev_hv_wide(c, c->n_so, k2, v, n, n, 0.0, 1.0, 1, ok2, ov, &d);
+    return d;
 }
 
 }  // namespace sfci

@@ -154,6 +154,17 @@ def lib():
         "sfci_wide_set_workspace": (ip, [vp, u64]),
         "sfci_wide_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
         "sfci_wide_set_group_bits": (ip, [vp, ip]),
+        "sfci_wide_vec_copy": (ip, [vp, vp]),
+        "sfci_wide_vec_clear": (ip, [vp]),
+        "sfci_wide_dot": (ip, [vp, vp, vp, C.POINTER(C.c_double)]),
+        "sfci_wide_norm": (ip, [vp, vp, C.POINTER(C.c_double)]),
+        "sfci_wide_scale": (ip, [vp, vp, dbl]),
+        "sfci_wide_add_two": (ip, [vp, vp, vp, dbl]),
+        "sfci_wide_shrink_from": (ip, [vp, vp, vp]),
+        "sfci_wide_lincomb": (ip, [vp, vp, vp, ip, vp]),
+        "sfci_wide_energy": (ip, [vp, vp, C.POINTER(C.c_double)]),
+        "sfci_wide_sync": (ip, [vp]),
+        "sfci_wide_ground_state": (ip, [vp, vp, C.POINTER(LanczosParamsT), vp, vp, C.POINTER(LanczosResultT), vp]),
         "sfci_wide_vec_create": (ip, [vp, i64, pvp]),
         "sfci_wide_vec_destroy": (None, [vp]),
         "sfci_wide_vec_len": (i64, [vp]),
@@ -584,3 +595,49 @@ class WideContext:
         d = C.c_double()
         self._check(lib().sfci_wide_hv(self._h, vin._h, vout._h, int(nd), lam, coeff, C.byref(d)))
         return d.value
+
+    def copy(self, dst: WideVector, src: WideVector):
+        self._check(lib().sfci_wide_vec_copy(dst._h, src._h))
+
+    def dot(self, a: WideVector, b: WideVector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_wide_dot(self._h, a._h, b._h, C.byref(d)))
+        return d.value
+
+    def norm(self, a: WideVector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_wide_norm(self._h, a._h, C.byref(d)))
+        return d.value
+
+    def scale(self, a: WideVector, s: float):
+        self._check(lib().sfci_wide_scale(self._h, a._h, s))
+
+    def add_two(self, v: WideVector, w: WideVector, a: float):
+        self._check(lib().sfci_wide_add_two(self._h, v._h, w._h, a))
+
+    def shrink_from(self, dst: WideVector, src: WideVector):
+        self._check(lib().sfci_wide_shrink_from(self._h, dst._h, src._h))
+
+    def lincomb(self, vecs: Sequence[WideVector], coeffs, out: WideVector):
+        arr = (C.c_void_p * len(vecs))(*[v._h.value for v in vecs])
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self._check(lib().sfci_wide_lincomb(self._h, C.cast(arr, C.c_void_p), _ptr(c), len(vecs), out._h))
+
+    def energy(self, v: WideVector) -> float:
+        d = C.c_double()
+        self._check(lib().sfci_wide_energy(self._h, v._h, C.byref(d)))
+        return d.value
+
+    def ground_state(self, init_det: int, nd: int, nl: int, num_restart: int, hpsi_cap: int = 60_000_000,
+                     delta_e: float = 1e-5, verbose: bool = False, v_out: Optional[WideVector] = None,
+                     e_prev_init: float = 0.0) -> dict:
+        ritz, et, ms = np.zeros(num_restart), np.zeros(num_restart), np.zeros(num_restart)
+        p = LanczosParamsT(nd, nl, num_restart, hpsi_cap, delta_e, 1 if verbose else 0, e_prev_init,
+                           ms.ctypes.data)
+        r = LanczosResultT()
+        init = np.array([init_det & 0xFFFFFFFFFFFFFFFF, init_det >> 64], np.uint64)
+        self._check(lib().sfci_wide_ground_state(self._h, _ptr(init), C.byref(p), _ptr(ritz), _ptr(et), C.byref(r),
+                                                 v_out._h if v_out is not None else None))
+        n = r.n_restarts
+        return dict(restarts=n, ritz=ritz[:n].copy(), e_trunc=et[:n].copy(), ms_restart=ms[:n].copy(),
+                    e_final=r.e_final, ms_total=r.ms_total, ms_hv=r.ms_hv, n_hv=r.n_hv)

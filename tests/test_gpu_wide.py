@@ -153,3 +153,106 @@ def test_wide_rejects_bad_input(wctx, wsys):
     hf = np.array([wsys["init_hf"]], np.uint64)
     gl, gh, gv, gd = gpu_wide_hv(wctx, hf, np.zeros(1, np.uint64), np.ones(1), 1, 0.0, 1.0, 100_000)
     assert len(gl) > 0
+
+
+def _pair_contexts(make_system):
+    from sfci_amd import gpu
+    s = make_system(12, 8, 0)
+    return s, gpu.Context(s["in_path"], s["n_so"], s["n_elec"]), gpu.WideContext(s["in_path"], s["n_so"], s["n_elec"])
+
+
+def test_wide_vector_ops_match_64bit(make_system):
+    """dot, norm, scale, addTwo, shrinkFrom and recombination on two-word keys reproduce the
+    64-bit operations bit for bit (truncation ties included)."""
+    s, c64, cw = _pair_contexts(make_system)
+    try:
+        rng = np.random.default_rng(5)
+        v0 = c64.vector(10, np.array([s["init_hf"]], np.uint64), np.array([1.0]))
+        big = c64.vector(100_000)
+        c64.hv(v0, big, 10)
+        k, x = big.download()
+        z = np.zeros(len(k), np.uint64)
+
+        def pick(n, seed, ties=False):
+            r = np.random.default_rng(seed)
+            i = np.sort(r.choice(len(k), size=n, replace=False))
+            val = r.choice([0.5, -0.25, 0.125], size=n) if ties else r.normal(size=n)
+            return k[i], val
+
+        a_k, a_v = pick(900, 1)
+        b_k, b_v = pick(700, 2, ties=True)
+        A64, B64 = c64.vector(2000, a_k, a_v), c64.vector(2000, b_k, b_v)
+        Aw = cw.vector(2000, a_k, np.zeros(len(a_k), np.uint64), a_v)
+        Bw = cw.vector(2000, b_k, np.zeros(len(b_k), np.uint64), b_v)
+        assert bits([c64.dot(A64, B64)])[0] == bits([cw.dot(Aw, Bw)])[0]
+        assert bits([c64.norm(A64)])[0] == bits([cw.norm(Aw)])[0]
+        c64.scale(A64, 0.37)
+        cw.scale(Aw, 0.37)
+        for cap in (2000, 1000):
+            V64, Vw = c64.vector(cap), cw.vector(cap)
+            V64.copy_from(A64)
+            cw.copy(Vw, Aw)
+            c64.add_two(V64, B64, -1.5)
+            cw.add_two(Vw, Bw, -1.5)
+            k1, x1 = V64.download()
+            l2, h2, x2 = Vw.download()
+            assert_same(l2, h2, x2, k1, np.zeros(len(k1), np.uint64), x1)
+            S64, Sw = c64.vector(cap // 3), cw.vector(cap // 3)
+            c64.shrink_from(S64, V64)
+            cw.shrink_from(Sw, Vw)
+            k1, x1 = S64.download()
+            l2, h2, x2 = Sw.download()
+            assert_same(l2, h2, x2, k1, np.zeros(len(k1), np.uint64), x1)
+        coef = rng.normal(size=2)
+        L64, Lw = c64.vector(800), cw.vector(800)
+        c64.lincomb([A64, B64], coef, L64)
+        cw.lincomb([Aw, Bw], coef, Lw)
+        k1, x1 = L64.download()
+        l2, h2, x2 = Lw.download()
+        assert_same(l2, h2, x2, k1, np.zeros(len(k1), np.uint64), x1)
+        assert abs(c64.energy(L64) - cw.energy(Lw)) < 1e-12 * max(1.0, abs(c64.energy(L64)))
+    finally:
+        c64.close()
+        cw.close()
+
+
+def test_wide_ground_state_matches_64bit(make_system):
+    """Below 65 spin orbitals the 128-bit driver follows the 64-bit driver step for step."""
+    s, c64, cw = _pair_contexts(make_system)
+    try:
+        r64 = c64.ground_state(s["init_hf"], 3000, 8, 3, hpsi_cap=200_000, e_prev_init=float("inf"))
+        rw = cw.ground_state(s["init_hf"], 3000, 8, 3, hpsi_cap=200_000, e_prev_init=float("inf"))
+        log("64-bit %s / 128-bit %s" % (r64["ritz"], rw["ritz"]))
+        assert r64["restarts"] == rw["restarts"]
+        assert np.array_equal(bits(r64["ritz"]), bits(rw["ritz"]))
+        assert np.abs(r64["e_trunc"] - rw["e_trunc"]).max() < 1e-12 * abs(r64["e_final"])
+        assert abs(r64["e_final"] - rw["e_final"]) < 1e-12 * abs(r64["e_final"])
+    finally:
+        c64.close()
+        cw.close()
+
+
+def test_wide_ground_state_full_space_fci(make_system):
+    """66 spin orbitals, 1 + 1 electrons: with Nd above the full space (1089 determinants) and
+    no cut, the 128-bit driver reaches the lowest eigenvalue of the dense Hamiltonian."""
+    from sfci_amd import gpu
+    s = make_system(33, 2, 5)
+    o = O.Oracle(s["in_path"], s["n_so"], s["n_elec"])
+    n_orb = s["n_orb"]
+    dets = sorted({(1 << (2 * i)) | (1 << (2 * j + 1)) for i in range(n_orb) for j in range(n_orb)})
+    idx = {d: t for t, d in enumerate(dets)}
+    M = np.zeros((len(dets), len(dets)))
+    for j, d in enumerate(dets):
+        lo, hi, val = o.connections_wide(d)
+        for a, b, x in zip(lo, hi, val):
+            M[idx[(int(b) << 64) | int(a)], j] += x
+    e0 = np.linalg.eigvalsh(M)[0]
+    ctx = gpu.WideContext(s["in_path"], s["n_so"], s["n_elec"])
+    try:
+        ctx.set_cut(0.0)
+        r = ctx.ground_state(s["init_hf"], 2000, 20, 8, hpsi_cap=20_000, e_prev_init=float("inf"))
+    finally:
+        ctx.close()
+    log("FCI %.12f, 128-bit solve %.12f after %d restarts" % (e0, r["e_final"], r["restarts"]))
+    assert abs(r["e_final"] - e0) < 1e-9
+
