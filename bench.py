@@ -345,6 +345,33 @@ def wide_hv(args):
     n_conn = ctx.info()["n_conn"]
     mixed = args.precision == "mixed"
     hf = w["init_hf"]
+    if args.time_to_e:
+        # the ground-state solve on 128-bit keys, FP64 first, then (with --precision mixed)
+        # mixed precision judged against it
+        nl = args.nl or w["nl"]
+        restarts = args.restarts or w["restarts"]
+        res = dict(metric="time-to-E, 128-bit keys (truncated Lanczos until two restarts' Ritz values differ by "
+                          "< 1e-5 Ha)", unit="s", higher_is_better=False, data="synthetic",
+                   config=dict(workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d (reduced), "
+                                        "Nl=%d, restarts<=%d" % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd,
+                                                                  nl, restarts)))
+        for tag, mx in (("fp64", False), ("mixed", True)) if mixed else (("fp64", False),):
+            ctx.set_precision(mx)
+            t = time.perf_counter()
+            r = ctx.ground_state(hf, nd, nl, restarts, hpsi_cap=args.cap, e_prev_init=float("inf"),
+                                 verbose=args.verbose)
+            dt = time.perf_counter() - t
+            res[tag] = dict(seconds=dt, restarts=r["restarts"], ritz=list(r["ritz"]), e_trunc=list(r["e_trunc"]),
+                            e_final=r["e_final"], n_hv=r["n_hv"], ms_hv_total=r["ms_hv"])
+            log("%s solve: %.2f s, E = %.12f" % (tag, dt, r["e_final"]))
+        ctx.set_precision(False)
+        res["value"] = res["fp64"]["seconds"]
+        if mixed:
+            res["energy_error_mixed_vs_fp64"] = res["mixed"]["e_final"] - res["fp64"]["e_final"]
+            res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+        print(json.dumps(res))
+        ctx.close()
+        return
     lo, hi = gpu.split_keys([hf])
     v = ctx.vector(nd, lo, hi, np.array([1.0]))
     out = ctx.vector(nd)
