@@ -27,8 +27,7 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
-    os.path.join(ROOT, "BASELINE.json")) else "H.v connections/s"
+METRIC = "H\u00b7v connections/s + time-to-E within 1e-5 Ha, NN/FP32 vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (fetch + write) measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on the default workload in FP64, scaled by the counters' reading of a plain
@@ -459,7 +458,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: multi-GPU sharding of H.v is described in DESIGN.md and not built",
+        print("bench.py runs on one GPU: sharding H.v across GPUs (by target row range) is a design, not built",
               file=sys.stderr)
         sys.exit(2)
     if args.screener:

@@ -184,7 +184,7 @@ typedef struct {
     double ms_hv;        /* wall time spent in sfci_hv */
     int64_t n_hv;        /* number of sfci_hv calls */
     int n_screened;      /* restarts whose products were screened (sfci_set_screen) */
-    int n_reverts;       /* screened restarts rejected by the judge (see sfci_ground_state) */
+    int n_reverts;       /* screened restarts rejected by the unscreened energy readout (see sfci_ground_state) */
 } sfci_lanczos_result_t;
 
 /* Runs from the single determinant init_det (coefficient 1).  ritz and e_trunc receive
