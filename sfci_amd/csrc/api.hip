@@ -769,14 +769,25 @@ int64_t sfci_wide_vec_len(const sfci_wvec* V) { return V ? V->len : -1; }
 
 int sfci_wide_vec_upload(sfci_wvec* V, const uint64_t* keys, const double* vals, int64_t n) {
     if (!V || n < 0 || n > V->cap || (n && (!keys || !vals))) return SFCI_E_ARG;
-    for (int64_t i = 1; i < n; ++i) {
-        const uint64_t alo = keys[2 * i - 2], ahi = keys[2 * i - 1], blo = keys[2 * i], bhi = keys[2 * i + 1];
-        if (!(ahi < bhi || (ahi == bhi && alo < blo))) {
-            set_err(V->ctx, "sfci_wide_vec_upload: keys must be strictly ascending (hi:lo)");
+    Ctx* c = V->ctx;
+    const uint64_t ev = 0x5555555555555555ull, od = 0xaaaaaaaaaaaaaaaaull;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t blo = keys[2 * i], bhi = keys[2 * i + 1];
+        if (i > 0) {
+            const uint64_t alo = keys[2 * i - 2], ahi = keys[2 * i - 1];
+            if (!(ahi < bhi || (ahi == bhi && alo < blo))) {
+                set_err(c, "sfci_wide_vec_upload: keys must be strictly ascending (hi:lo)");
+                return SFCI_E_ARG;
+            }
+        }
+        const bool beyond = c->n_so <= 64 ? (bhi != 0 || (c->n_so < 64 && (blo >> c->n_so)))
+                                          : (c->n_so < 128 && (bhi >> (c->n_so - 64)));
+        if (popc64(blo & ev) + popc64(bhi & ev) != c->ham.k || popc64(blo & od) + popc64(bhi & od) != c->ham.k ||
+            beyond) {
+            set_err(c, "sfci_wide_vec_upload: key " + std::to_string(i) + " does not match n_so / n_elec");
             return SFCI_E_ARG;
         }
     }
-    Ctx* c = V->ctx;
     return guarded(c, [&] {
         if (n) {
             HIPCHK(hipMemcpyAsync(V->k, keys, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));

@@ -613,7 +613,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         sink.cap = out_cap;
     }
     __shared__ uint16_t sdec[MAX_SINGLES];
-    __shared__ uint32_t s_wcnt[SFCI_BLOCK / SFCI_WAVE];
+    __shared__ uint32_t s_ccur;
     __shared__ uint64_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
@@ -731,9 +731,15 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             }
             pos += nB;
         }
-        // C
+        // C (spin-0 doubles in range: each wavefront reserves its slots in the source's range
+        // with one LDS atomic; the order of records inside the range is free, every sum
+        // downstream being exact)
         {
             const bool all_in = (off == nullptr);
+            if (!all_in) {
+                if (tid == 0) s_ccur = 0;
+                __syncthreads();
+            }
             DivIter it(tid, SFCI_BLOCK, nvp);
             for (uint32_t f0 = 0; f0 < ndaa; f0 += SFCI_BLOCK) {
                 uint32_t f = f0 + tid;
@@ -770,18 +776,14 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                     if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
                     continue;
                 }
-                uint64_t bal = __ballot(have);
-                uint32_t my = (uint32_t)popc64(bal & lt);
-                if (lane == 0) s_wcnt[w] = (uint32_t)popc64(bal);
-                __syncthreads();
-                uint32_t pre = 0, tot = 0;
-                for (int i2 = 0; i2 < SFCI_BLOCK / SFCI_WAVE; ++i2) {
-                    if (i2 < w) pre += s_wcnt[i2];
-                    tot += s_wcnt[i2];
+                const uint64_t bal = __ballot(have);
+                if (bal) {
+                    const int leader = __ffsll((long long)bal) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&s_ccur, (uint32_t)popc64(bal));
+                    base = __shfl(base, leader, SFCI_WAVE);
+                    if (have) wbad |= !sink.put(pos + base + (uint32_t)popc64(bal & lt), key, val, 2u, tdet);
                 }
-                if (have) wbad |= !sink.put(pos + pre + my, key, val, 2u, tdet);
-                pos += tot;
-                __syncthreads();
             }
         }
         __syncthreads();

@@ -144,7 +144,7 @@ def test_wide_rejects_bad_input(wctx, wsys):
     # keys out of order: refused on upload
     with pytest.raises(gpu.SfciError):
         wctx.vector(4, np.array([5, 3], np.uint64), np.zeros(2, np.uint64), np.ones(2))
-    # wrong electron count: reported by the generation kernel's check, nothing emitted
+    # wrong electron count: refused on upload (the generation kernel checks it again)
     bad = np.array([(1 << 9) - 1], np.uint64)
     with pytest.raises(gpu.SfciError) as e:
         gpu_wide_hv(wctx, bad, np.zeros(1, np.uint64), np.ones(1), 1, 0.0, 1.0, 100)
