@@ -227,7 +227,7 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                 const bool in = lane < N;
                 const bool oc = in && bit_of(lo, hi, 2 * lane + sp);
                 const uint64_t b_oc = __ballot(oc), b_vi = __ballot(in && !oc);
-                if (__popcll(b_oc) != k) ok = false;
+                if ((int)__popcll(b_oc) != k) ok = false;
                 const int t = __popcll(b_oc & lt);
                 if (oc && t < 32) s.occ[sp][t] = (uint8_t)lane;
                 else if (in && !oc) s.vir[sp][__popcll(b_vi & lt)] = (uint8_t)lane;
