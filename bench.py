@@ -410,7 +410,7 @@ def wide_hv(args):
         orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
         vl, vh, vv = v.download()
         threads = max(1, min(16, os.cpu_count() or 1))
-        m = min(len(vl), 200)
+        m = min(len(vl), 1000)
         t = time.perf_counter()
         orc.hv_wide(vl[:m], vh[:m], vv[:m], nthreads=threads)
         tc = time.perf_counter() - t

@@ -764,12 +764,21 @@ int oracle_hv_wide(const oracle_ham* hp, const uint64_t* in_lo, const uint64_t* 
     for (int64_t i = 0; i < n_in; ++i) src[i] = {((u128)in_hi[i] << 64) | in_lo[i], in_v[i]};
     if (n_in > nd) top_cap(src, (size_t)nd);
     const int64_t ns = (int64_t)src.size();
+    // generation into per-(thread, bucket) lists, bucket = a hash of the target key; then
+    // every bucket sorted and merged on its own (the decomposition of oracle_hv)
     struct CW { u128 k; double v; uint8_t tag; };
-    std::vector<std::vector<CW>> res(nthreads);
+    const int T = nthreads;
+    const size_t nb = (size_t)4 * T;
+    auto bucket_of = [nb](u128 k) {
+        uint64_t z = (uint64_t)k * 0x9E3779B97F4A7C15ull ^ (uint64_t)(k >> 64) * 0xC2B2AE3D27D4EB4Full;
+        z ^= z >> 29;
+        return (size_t)(z % nb);
+    };
+    std::vector<std::vector<CW>> res((size_t)T * nb);
     int bad = 0;
-#pragma omp parallel num_threads(nthreads)
+#pragma omp parallel num_threads(T)
     {
-        auto& mine = res[omp_get_thread_num()];
+        const int tid = omp_get_thread_num();
 #pragma omp for schedule(static, 25)
         for (int64_t i = 0; i < ns; ++i) {
             const double temp = coeff * src[i].v;
@@ -781,30 +790,42 @@ int oracle_hv_wide(const oracle_ham* hp, const uint64_t* in_lo, const uint64_t* 
                 else if (mode == 2) val = (double)((float)hij * (float)temp);
                 else val = hij * temp;
                 if (!fx_ok(val)) bad = 1;
-                mine.push_back({t, val, tag});
+                res[tid + (size_t)T * bucket_of(t)].push_back({t, val, tag});
             });
         }
     }
     if (bad) return -1;
-    std::vector<CW> all;
-    for (auto& r : res) { all.insert(all.end(), r.begin(), r.end()); std::vector<CW>().swap(r); }
-    std::sort(all.begin(), all.end(), [](const CW& a, const CW& b) { return a.k < b.k; });
-    std::vector<KW> fin;
-    fx_t dacc = 0;
-    size_t i = 0;
-    while (i < all.size()) {
-        size_t j = i;
-        fx_t acc = 0;
-        bool is_input = false;
-        for (; j < all.size() && all[j].k == all[i].k; ++j) { acc += fx_from(all[j].v); is_input |= all[j].tag; }
-        const double y = fx_to(acc);
-        if (is_input) {
-            auto it = std::lower_bound(src.begin(), src.end(), all[i].k, [](const KW& a, u128 k) { return a.k < k; });
-            dacc += fx_from(y * it->v);
+    std::vector<std::vector<KW>> kept(nb);
+    std::vector<fx_t> dpart(nb, 0);
+#pragma omp parallel for schedule(dynamic) num_threads(T)
+    for (int64_t b = 0; b < (int64_t)nb; ++b) {
+        std::vector<CW> buf;
+        for (int t = 0; t < T; ++t) {
+            auto& r = res[t + (size_t)T * b];
+            buf.insert(buf.end(), r.begin(), r.end());
+            std::vector<CW>().swap(r);
         }
-        if (std::fabs(y) > cut) fin.push_back({all[i].k, y});
-        i = j;
+        std::sort(buf.begin(), buf.end(), [](const CW& a, const CW& c) { return a.k < c.k; });
+        size_t i = 0;
+        while (i < buf.size()) {
+            size_t j = i;
+            fx_t acc = 0;
+            bool is_input = false;
+            for (; j < buf.size() && buf[j].k == buf[i].k; ++j) { acc += fx_from(buf[j].v); is_input |= buf[j].tag; }
+            const double y = fx_to(acc);
+            if (is_input) {
+                auto it = std::lower_bound(src.begin(), src.end(), buf[i].k,
+                                           [](const KW& a, u128 k) { return a.k < k; });
+                dpart[b] += fx_from(y * it->v);
+            }
+            if (std::fabs(y) > cut) kept[b].push_back({buf[i].k, y});
+            i = j;
+        }
     }
+    fx_t dacc = 0;
+    for (auto d : dpart) dacc += d;
+    std::vector<KW> fin;
+    for (auto& k : kept) fin.insert(fin.end(), k.begin(), k.end());
     top_cap(fin, (size_t)out_cap);
     for (size_t q = 0; q < fin.size(); ++q) {
         out_lo[q] = (uint64_t)fin[q].k;
