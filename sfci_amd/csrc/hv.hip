@@ -1293,6 +1293,20 @@ __global__ void k_bucket_bounds(const R* __restrict__ ck, size_t n, int hb, uint
     }
 }
 
+// First record whose digit (low 8 bits of the hb-bit target hash) is 128 or more, in a list
+// ascending in that digit (one thread: a binary search).
+template <typename R>
+__global__ void k_digit_split(const R* __restrict__ ck, size_t n, int hb, uint64_t* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if ((target_hash(rec_key(ck[mid]) >> 1, hb) & 255u) < 128u) lo = mid + 1;
+        else hi = mid;
+    }
+    *out = lo;
+}
+
 // Exact accumulation without a carry chain: a term x (|x| < 2^42) is split into three
 // signed limbs with x * 2^84 = l2 * 2^84 + l1 * 2^42 + l0 truncated toward zero (the same
 // integer fx_from gives), using exact FP64 steps: l2 = trunc(x), l1 = trunc(frac * 2^42),
@@ -1844,11 +1858,30 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             st.ms_sort += ts.stop();
         };
         if (!hashed) sort_path();
+        // overflow of a hash bucket: the pass is split by the next hash bit into two halves
+        // that are merged with twice the buckets (hb + 1 bits); only if that overflows too
+        // does the pass take the full sort
+        bool split = false;
+        size_t split_m = 0;
+        uint64_t* bs2 = nullptr;
+        uint64_t* be2 = nullptr;
         for (int attempt = 0;; ++attempt) {
             Timer t(c->stream);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
-            if (hashed) {
+            if (hashed && split) {
+                const uint64_t nb = (uint64_t)1 << (hb + 1);
+                int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
+                if (split_m)
+                    k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(
+                        H, rd, bstart, bend, nb, hb + 1, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
+                if (n_pass - split_m)
+                    k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(
+                        H, rd + split_m, bs2, be2, nb, hb + 1, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
+                kt_end(c, tr);
+            } else if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
                 int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
                 k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, rd, bstart, bend, nb, hb, row_lo * H.dim, sk,
@@ -1871,14 +1904,66 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             if (hashed) {
                 fl_now = flags_read_clear(c);
                 if (fl_now & FLAG_BUCKET_FULL) {
-                    // roll back this pass's output and merge it through the full sort instead
+                    // roll back this pass's output; split it by the next hash bit, or (already
+                    // split, or too few hash bits for the two-level partition) merge it through
+                    // the full sort instead
                     st.n_fallback += 1;
                     c->unique_ratio = std::min(1.0, c->unique_ratio * 2.0);
                     uint64_t restore[2] = {before, uniq_host};
                     HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
-                    hashed = false;
-                    sort_path();
                     if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
+                    Timer ts(c->stream);
+                    bool ok = false;
+                    if (!split && hb > 8 && hb <= 16) {
+                        // records (grouped by hb bits) -> the other buffer, those with hash bit
+                        // hb + 1 clear first; each half regrouped by hb bits in place, with the
+                        // record buffer (grown if needed) as scratch
+                        hash_digit_pass<R>(c, rd, other, n_pass, hb + 8);
+                        uint64_t* d_m = c->d_counter + 2;
+                        k_digit_split<R><<<1, 1, 0, c->stream>>>(other, n_pass, hb + 8, d_m);
+                        HIPCHK(hipGetLastError());
+                        uint64_t m = 0;
+                        HIPCHK(hipMemcpyAsync(&m, d_m, 8, hipMemcpyDeviceToHost, c->stream));
+                        HIPCHK(hipStreamSynchronize(c->stream));
+                        R* recs = other;
+                        DevBuf& sbuf = (rd == A) ? c->cand_k[0] : c->cand_k[1];
+                        R* scratch = (R*)sbuf.get<uint8_t>(
+                            std::max<size_t>(part_capacity(std::max<size_t>(m, n_pass - m)) * sizeof(R),
+                                             n_pass * (sizeof(uint64_t) + sizeof(T))));
+                        if (rd == A) A = scratch;
+                        else B = scratch;
+                        R* g0 = m ? hash_partition<R>(c, recs, scratch, m, hb) : recs;
+                        R* g1 = (n_pass - m) ? hash_partition<R>(c, recs + m, scratch, n_pass - m, hb) : recs + m;
+                        rd = recs;
+                        other = scratch;
+                        if (g0 == recs && g1 == recs + m) {
+                            const size_t nb2 = (size_t)1 << (hb + 1);
+                            bstart = c->bucket_bounds.get<uint64_t>(4 * nb2);
+                            bend = bstart + nb2;
+                            bs2 = bend + nb2;
+                            be2 = bs2 + nb2;
+                            const unsigned gb = (unsigned)((nb2 + SFCI_BLOCK - 1) / SFCI_BLOCK);
+                            if (m)
+                                k_bucket_bounds<R><<<gb, SFCI_BLOCK, 0, c->stream>>>(rd, m, hb + 1, nb2, bstart, bend);
+                            if (n_pass - m)
+                                k_bucket_bounds<R><<<gb, SFCI_BLOCK, 0, c->stream>>>(rd + m, n_pass - m, hb + 1, nb2,
+                                                                                    bs2, be2);
+                            HIPCHK(hipGetLastError());
+                            split = true;
+                            split_m = m;
+                            ok = true;
+                        } else {
+                            // a region of the regrouping overflowed: the records are all in rd
+                            uint32_t f2 = flags_read_clear(c);
+                            if (f2 & ~(uint32_t)FLAG_REGION_FULL) atomic_flags_restore(c, f2 & ~(uint32_t)FLAG_REGION_FULL);
+                        }
+                    }
+                    st.ms_sort += ts.stop();
+                    if (!ok) {
+                        hashed = false;
+                        split = false;
+                        sort_path();
+                    }
                     continue;
                 }
                 if (fl_now & ~(uint32_t)FLAG_OUT_OVERFLOW) atomic_flags_restore(c, fl_now);
@@ -1914,7 +1999,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             uint32_t fl = flags_read_clear(c);
             if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
         }
-        st.hash_bits = hashed ? hb : 0;
+        st.hash_bits = hashed ? (split ? hb + 1 : hb) : 0;
     }
     uint64_t uniq = uniq_host;
     *diag = fx_to(diag_total);

@@ -275,6 +275,10 @@ void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* 
 size_t part_capacity(size_t n);
 template <typename R>
 R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb);
+// One dense partition pass A -> B by the low 8 bits of the hb-bit target hash (hb = h + 8:
+// the records whose hash bit h + 1 is clear come first).
+template <typename R>
+void hash_digit_pass(Ctx* c, R* A, R* B, size_t n, int hb);
 
 // vecops.hip
 // Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.

@@ -178,6 +178,33 @@ def bits_of(x):
     return bits([x])[0]
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_hv_bucket_overflow_split(make_system, mixed):
+    """Hash buckets too few for a pass's distinct targets (9 bits forced, 3.2e6 targets for
+    512 tables of 4608 slots): the pass is split by the next hash bit and merged with 10 bits,
+    with the same bits as the oracle."""
+    from sfci_amd import gpu
+    s = make_system(16, 8, 3)
+    o = O.Oracle(s["in_path"], s["n_so"], s["n_elec"])
+    dets = all_dets(16, 4)
+    rng = np.random.default_rng(7)
+    pick = np.sort(rng.choice(len(dets), size=4000, replace=False))
+    k, v = dets[pick], rng.normal(size=4000)
+    ctx = gpu.Context(s["in_path"], s["n_so"], s["n_elec"])
+    try:
+        ctx.set_precision(mixed)
+        ctx.set_merge_mode(2, 9)
+        gk, gv, gd = gpu_hv(ctx, k, v, len(k), 0.1, 1.0, 60_000_000)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    log("overflow split: fallback %d hash_bits %d unique %d" % (st["n_fallback"], st["hash_bits"], st["n_unique"]))
+    assert st["n_fallback"] >= 1 and st["hash_bits"] == 10
+    ok, ov, od, _ = o.hv(k, v, nd=len(k), lam=0.1, coeff=1.0, cut=1e-5, mode=2 if mixed else 0)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits_of(gd) == bits_of(od)
+
+
 def test_hv_empty(ctx1):
     vin, vout = ctx1.vector(10), ctx1.vector(10)
     assert ctx1.hv(vin, vout, 10) == 0.0
