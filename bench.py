@@ -88,6 +88,7 @@ def time_to_e(ctx, w, nd, args):
     """The reference's ground-state solve (truncated Lanczos, restarts until the Ritz
     value drops by less than 1e-5 Ha) on the GPU; optionally the CPU oracle's solve."""
     mixed = args.precision == "mixed"
+    w = dict(w, nl=args.nl or w["nl"], restarts=args.restarts or w["restarts"])
     fp64_ref = None
     if mixed:
         # the plain FP64 solve first: the mixed solve's energy is judged against it
