@@ -148,6 +148,20 @@ def test_hv_exact_vs_literal_reference_arithmetic(make_system):
     assert abs(od - ld) <= 1e-12 * abs(ld)
 
 
+def test_energy_readout_equals_hv_diag(make_system):
+    """The reference's two energy paths agree: calc_energy's per-source pull
+    (src/functions.cpp:1272-1313) and the diag output of H.v with lambda 0, coeff 1 and all
+    of v kept (src/functions.cpp:674-677); both sums are exact, rounded differently."""
+    s = make_system(12, 8, 0)
+    o = orc_of(s)
+    k1, v1, _, _ = o.hv(np.array([s["init_hf"]], np.uint64), np.array([1.0]))
+    k2, v2, _, _ = o.hv(k1, v1)
+    v2 = v2 / np.sqrt(v2 @ v2)
+    e = o.energy(k2, v2)
+    _, _, d, _ = o.hv(k2, v2, nd=len(k2), lam=0.0, coeff=1.0, cut=0.0)
+    assert abs(e - d) <= 4e-15 * abs(e)
+
+
 def test_hv_matches_dense_tiny(make_system):
     s = make_system(5, 4, 7)
     o = orc_of(s)
