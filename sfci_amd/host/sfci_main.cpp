@@ -142,6 +142,13 @@ int main(int argc, char** argv) {
         fprintf(stderr, "orb num larger than 128 is not supported\n");
         return 1;
     }
+    {
+        uint64_t lo = 0, hi = 0;
+        if (num_orb <= 64 && (!parse_u128(init_str, lo, hi) || hi != 0)) {
+            fprintf(stderr, "bad value for init_hf: %s\n", init_str.c_str());
+            return 1;
+        }
+    }
     if (num_orb > 64) {
         if (task != "ground") {
             fprintf(stderr, "task=%s is not available beyond 64 spin orbitals (task=ground is)\n", task.c_str());

@@ -31,3 +31,9 @@ def test_init_hf_must_be_decimal(tmp_path):
     # a 129-bit value does not fit
     r = run(tmp_path, ["task=ground", "num_orb=100", "num_e=20", "filename=x_IN", "init_hf=%d" % (1 << 128)])
     assert r.returncode == 1 and "init_hf" in r.stderr
+
+
+def test_init_hf_beyond_64_bits_needs_more_spin_orbitals(tmp_path):
+    r = run(tmp_path, ["task=ground", "num_orb=52", "num_e=14", "filename=x_IN", "init_hf=%d" % (1 << 64)])
+    assert r.returncode == 1 and "init_hf" in r.stderr
+
