@@ -180,3 +180,21 @@ def test_uhf_slater_condon_vs_second_quantized(tmp_path):
             assert abs(got.get(t, 0.0) - x) < 1e-12, (d, t)
         for t, x in got.items():
             assert abs(ref.get(t, 0.0) - x) < 1e-12, (d, t)
+
+
+def test_workload_table():
+    """The benchmark workloads: orbital space, electrons, seed, Nd, Nl, restarts and the
+    generator settings of each synthetic system."""
+    W = I.WORKLOADS
+    assert W["c1"] == dict(n_orb=12, n_elec=8, seed=0, nd=20000, nl=20, restarts=5, gen={})
+    assert W["c2"] == dict(n_orb=26, n_elec=14, seed=1, nd=1_000_000, nl=30, restarts=8, gen={})
+    assert W["c3"] == dict(n_orb=32, n_elec=16, seed=2, nd=4_000_000, nl=30, restarts=8, gen={})
+    assert W["c4"] == dict(n_orb=50, n_elec=20, seed=3, nd=10_000_000, nl=40, restarts=10,
+                           gen=dict(eps_lo=-1.0, eps_hi=0.5, sigma_g=0.04))
+    assert W["c5"] == dict(n_orb=32, n_elec=16, seed=4, nd=4_000_000, nl=30, restarts=8, gen={})
+    # the synthetic generator's defaults: orbital energies from -2.0 to +1.5 Ha, h off-diagonal
+    # N(0, 0.05), (pq|rs) off-diagonal N(0, 0.02)
+    import inspect
+    sig = inspect.signature(I.synth_integrals).parameters
+    assert (sig["eps_lo"].default, sig["eps_hi"].default, sig["sigma_h"].default, sig["sigma_g"].default) == \
+        (-2.0, 1.5, 0.05, 0.02)
