@@ -1585,34 +1585,15 @@ static int grid_for(Ctx* c, size_t work, int per_cu = 8) {
     return (int)std::max<size_t>(g, 1);
 }
 
-struct Timer {
-    hipEvent_t a, b;
-    hipStream_t s;
-    explicit Timer(hipStream_t st) : s(st) {
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
-        HIPCHK(hipEventRecord(a, s));
-    }
-    float stop() {
-        HIPCHK(hipEventRecord(b, s));
-        HIPCHK(hipEventSynchronize(b));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, a, b));
-        return ms;
-    }
-    ~Timer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    }
-};
 
 template <typename T>
 static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in, size_t nd, double lambda,
                        double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag, HvAll* all) {
     const HamDev& H = c->ham;
     auto& st = c->st;
+    kt_resolve(c);   // laps left pending by an earlier call that threw
     st = {};
-    Timer total(c->stream);
+    PhaseTimer total(c);
     *diag = 0.0;
     if (n_in == 0) return 0;
     // 1. input: top-Nd
@@ -1792,14 +1773,14 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         R* A = (R*)c->cand_k[0].get<uint8_t>(n_pass * sizeof(R));
         R* B = (R*)c->cand_k[1].get<uint8_t>(std::max(capB * sizeof(R), n_pass * (sizeof(uint64_t) + sizeof(T))));
         {
-            Timer t(c->stream);
+            PhaseTimer t(c);
             int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
             k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
                                                                         row_hi, off, c->d_sdec, A, n_pass,
                                                                         dterm, c->d_flags);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
-            st.ms_generate += t.stop();
+            t.lap(&st.ms_generate);
         }
         st.n_candidates += n_pass;
         // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (two
@@ -1823,7 +1804,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         uint64_t* bstart = nullptr;
         uint64_t* bend = nullptr;
         if (hashed) {
-            Timer t(c->stream);
+            PhaseTimer t(c);
             R* g = hash_partition<R>(c, A, B, n_pass, hb);
             if (!g) {
                 hashed = false;
@@ -1839,14 +1820,14 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                     rd, n_pass, hb, nb, bstart, bend);
                 kt_end(c, tb);
             }
-            st.ms_sort += t.stop();
+            t.lap(&st.ms_sort);
         }
         // sort path: split the records into keys and values in the free buffer, sort them
         // with the record buffer as scratch
         uint64_t* kd = nullptr;
         T* vd = nullptr;
         auto sort_path = [&]() {
-            Timer ts(c->stream);
+            PhaseTimer ts(c);
             kd = (uint64_t*)other;
             vd = (T*)(kd + n_pass);
             uint64_t* kt = (uint64_t*)rd;
@@ -1855,7 +1836,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                 rd, n_pass, kd, vd);
             int bits = bits_for(2 * (row_hi - row_lo) * H.dim);
             radix_sort_kv<T>(c, &kd, &vd, kt, vt, n_pass, bits);
-            st.ms_sort += ts.stop();
+            ts.lap(&st.ms_sort);
         };
         if (!hashed) sort_path();
         // overflow of a hash bucket: the pass is split by the next hash bit into two halves
@@ -1866,7 +1847,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         uint64_t* bs2 = nullptr;
         uint64_t* be2 = nullptr;
         for (int attempt = 0;; ++attempt) {
-            Timer t(c->stream);
+            PhaseTimer t(c);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
             if (hashed && split) {
@@ -1899,7 +1880,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             HIPCHK(hipGetLastError());
             uint64_t cnts[2];
             HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
-            st.ms_reduce += t.stop();
+            t.lap(&st.ms_reduce);
+            HIPCHK(hipStreamSynchronize(c->stream));
             uint32_t fl_now = 0;
             if (hashed) {
                 fl_now = flags_read_clear(c);
@@ -1912,7 +1894,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                     uint64_t restore[2] = {before, uniq_host};
                     HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
                     if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
-                    Timer ts(c->stream);
+                    PhaseTimer ts(c);
                     bool ok = false;
                     if (!split && hb > 8 && hb <= 16) {
                         // records (grouped by hb bits) -> the other buffer, those with hash bit
@@ -1958,7 +1940,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                             if (f2 & ~(uint32_t)FLAG_REGION_FULL) atomic_flags_restore(c, f2 & ~(uint32_t)FLAG_REGION_FULL);
                         }
                     }
-                    st.ms_sort += ts.stop();
+                    ts.lap(&st.ms_sort);
                     if (!ok) {
                         hashed = false;
                         split = false;
@@ -2018,7 +2000,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     }
     size_t kept = 0;
     if (all) {
-        Timer t(c->stream);
+        PhaseTimer t(c);
         uint64_t* tk = c->tmp_k[1].get<uint64_t>(std::max<size_t>(m, 1));
         double* tv = c->tmp_v[1].get<double>(std::max<size_t>(m, 1));
         uint64_t* kk = stk;
@@ -2028,9 +2010,9 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         all->v = vv;
         all->n = m;
         kept = m;
-        st.ms_select += t.stop();
+        t.lap(&st.ms_select);
     } else if (out_cap > 0 && m > 0) {
-        Timer t(c->stream);
+        PhaseTimer t(c);
         uint64_t* sel_k = c->tmp_k[0].get<uint64_t>(m);
         double* sel_v = c->tmp_v[0].get<double>(m);
         kept = select_top(c, stk, stv, m, out_cap, sel_k, sel_v);
@@ -2040,11 +2022,11 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         double* vv = sel_v;
         radix_sort_pairs(c, &kk, &vv, tk, tv, kept, c->n_so);
         dev_copy(c, ok, ov, kk, vv, kept);
-        st.ms_select += t.stop();
+        t.lap(&st.ms_select);
     }
     st.n_out = kept;
+    total.lap(&st.ms_total);
     kt_resolve(c);
-    st.ms_total = total.stop();
     return kept;
 }
 
@@ -2256,8 +2238,9 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                        double coeff, size_t out_cap, uint64_t* ok, double* ov, double* diag) {
     const HamDev& H = c->ham;
     auto& st = c->st;
+    kt_resolve(c);   // laps left pending by an earlier call that threw
     st = {};
-    Timer total(c->stream);
+    PhaseTimer total(c);
     *diag = 0.0;
     if (n_in == 0) return 0;
     if (c->screen_mode == 2 && !c->screen_net) throw HipError("network screening without a network", -1);
@@ -2279,9 +2262,9 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     double eps_used = 0.0;
     size_t m = 0;
     {
-        Timer t(c->stream);
+        PhaseTimer t(c);
         m = screened_generate(c, sk, sv, ns, lambda, coeff, dterm, c->screen_eps, &recs, &eps_used);
-        st.ms_generate += t.stop();
+        t.lap(&st.ms_generate);
     }
     st.n_candidates = m;
     st.screen_eps = eps_used;
@@ -2297,15 +2280,15 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     float* score = vt + m_alloc;
     size_t kept = m;
     if (m > keep) {
-        Timer t(c->stream);
+        PhaseTimer t(c);
         if (c->screen_mode == 2) {
             mlp_score_records(c, c->screen_net, recs, m, score);
         } else {
             k_score_mag<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, score);
         }
         HIPCHK(hipGetLastError());
-        st.ms_score += t.stop();
-        Timer t2(c->stream);
+        t.lap(&st.ms_score);
+        PhaseTimer t2(c);
         int ts = kt_begin(c, K_SCREEN_SELECT, 20.0 * m + 12.0 * keep);
         const uint32_t thr = score_threshold(c, score, m, keep);
         unsigned long long* cnt = (unsigned long long*)c->d_counter + 3;
@@ -2318,7 +2301,7 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         HIPCHK(hipStreamSynchronize(c->stream));
         kt_end(c, ts);
         kept = (size_t)kc;
-        st.ms_select += t2.stop();
+        t2.lap(&st.ms_select);
     } else if (m > 0) {
         k_recs_split<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, kd, vd);
         HIPCHK(hipGetLastError());
@@ -2335,10 +2318,10 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
     uint64_t cnts[2] = {0, 0};
     if (kept > 0) {
-        Timer ts(c->stream);
+        PhaseTimer ts(c);
         radix_sort_kv<float>(c, &kd, &vd, kt, vt, kept, bits_for(2 * H.dim * H.dim));
-        st.ms_sort += ts.stop();
-        Timer t(c->stream);
+        ts.lap(&st.ms_sort);
+        PhaseTimer t(c);
         acc_clear(c, 1);
         int grid = (int)std::min<size_t>((kept + RED_TILE - 1) / RED_TILE, 256 * 8);
         int tr = kt_begin(c, K_REDUCE, 12.0 * kept);
@@ -2348,7 +2331,7 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
         *diag = acc_read(c, 1);
-        st.ms_reduce += t.stop();
+        t.lap(&st.ms_reduce);
     }
     const uint32_t fl = flags_read_clear(c);
     if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
@@ -2363,7 +2346,7 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     }
     size_t n_out = 0;
     if (out_cap > 0 && mo > 0) {
-        Timer t(c->stream);
+        PhaseTimer t(c);
         uint64_t* sel_k = c->tmp_k[0].get<uint64_t>(mo);
         double* sel_v = c->tmp_v[0].get<double>(mo);
         n_out = select_top(c, stk, stv, mo, out_cap, sel_k, sel_v);
@@ -2373,11 +2356,11 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         double* vv = sel_v;
         radix_sort_pairs(c, &kk, &vv, tk, tv, n_out, c->n_so);
         dev_copy(c, ok, ov, kk, vv, n_out);
-        st.ms_select += t.stop();
+        t.lap(&st.ms_select);
     }
     st.n_out = n_out;
+    total.lap(&st.ms_total);
     kt_resolve(c);
-    st.ms_total = total.stop();
     return n_out;
 }
 

@@ -190,9 +190,15 @@ struct KernelClock {
         double bytes;
         hipEvent_t a, b;
     };
+    struct Phase {
+        double* dst;
+        hipEvent_t a, b;
+    };
     std::vector<hipEvent_t> pool;
     size_t used = 0;
+    int open = 0;                  // phase timers alive (their events stay reserved)
     std::vector<Rec> recs;
+    std::vector<Phase> phases;
     KernelStat stat[K_NUM];
 };
 
@@ -368,10 +374,25 @@ constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_HID * MLP_IN, MLP_OFF_W2 = MLP_OF
 void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score);
 
 // kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
-// stream and folds the elapsed times into c->kclock.stat.
+// stream and folds the elapsed times into c->kclock.stat (and the phase timers' laps into
+// their destinations).
 int kt_begin(Ctx* c, int id, double bytes);
 void kt_end(Ctx* c, int rec);
 void kt_resolve(Ctx* c);
+
+// Phase timer on the library stream.  lap(dst) records the end of the phase that began at
+// construction or at the previous lap, and starts the next one; the elapsed time is added
+// to *dst when the clock is resolved (kt_resolve), so timing a phase adds no host
+// synchronization.  dst must stay valid until then.
+struct PhaseTimer {
+    Ctx* c;
+    hipEvent_t a;
+    explicit PhaseTimer(Ctx* ctx);
+    ~PhaseTimer();
+    void lap(double* dst);
+    PhaseTimer(const PhaseTimer&) = delete;
+    PhaseTimer& operator=(const PhaseTimer&) = delete;
+};
 
 // accumulator helpers
 void acc_clear(Ctx* c, int slot);

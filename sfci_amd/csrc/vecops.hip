@@ -52,18 +52,44 @@ void kt_end(Ctx* c, int rec) {
 
 void kt_resolve(Ctx* c) {
     auto& k = c->kclock;
-    if (k.recs.empty()) return;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (auto& r : k.recs) {
-        if (!r.b) continue;
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
-        k.stat[r.id].ms += ms;
-        k.stat[r.id].launches += 1;
-        k.stat[r.id].bytes += r.bytes;
+    if (!k.recs.empty() || !k.phases.empty()) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (auto& r : k.recs) {
+            if (!r.b) continue;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            k.stat[r.id].ms += ms;
+            k.stat[r.id].launches += 1;
+            k.stat[r.id].bytes += r.bytes;
+        }
+        for (auto& p : k.phases) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+            *p.dst += ms;
+        }
+        k.recs.clear();
+        k.phases.clear();
     }
-    k.recs.clear();
-    k.used = 0;
+    // an open phase timer still holds its start event: the pool is reused only when none is
+    if (k.open == 0) k.used = 0;
+}
+
+PhaseTimer::PhaseTimer(Ctx* ctx) : c(ctx) {
+    a = kt_event(c);
+    ++c->kclock.open;
+    HIPCHK(hipEventRecord(a, c->stream));
+}
+
+PhaseTimer::~PhaseTimer() {
+    auto& k = c->kclock;
+    if (--k.open == 0 && k.recs.empty() && k.phases.empty()) k.used = 0;
+}
+
+void PhaseTimer::lap(double* dst) {
+    hipEvent_t b = kt_event(c);
+    HIPCHK(hipEventRecord(b, c->stream));
+    c->kclock.phases.push_back(KernelClock::Phase{dst, a, b});
+    a = b;
 }
 
 // ---- accumulators ------------------------------------------------------------------

@@ -632,27 +632,6 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_dot(const uint64_t* __restrict
 
 int grid_of_w(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 256 * 16)); }
 
-struct WTimer {
-    hipEvent_t a = nullptr, b = nullptr;
-    hipStream_t s;
-    explicit WTimer(hipStream_t st) : s(st) {
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
-        HIPCHK(hipEventRecord(a, s));
-    }
-    double stop() {
-        HIPCHK(hipEventRecord(b, s));
-        HIPCHK(hipEventSynchronize(b));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, a, b));
-        HIPCHK(hipEventRecord(a, s));   // restart
-        return ms;
-    }
-    ~WTimer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    }
-};
 
 uint64_t read_u64(Ctx* c, const uint64_t* d) {
     uint64_t h = 0;
@@ -770,9 +749,10 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
                  double coeff, size_t out_cap, uint64_t* ok2, double* ov, double* diag) {
     const HamDev& H = c->ham;
     auto& st = c->st;
+    kt_resolve(c);   // laps left pending by an earlier call that threw
     st = {};
     *diag = 0.0;
-    WTimer total(c->stream), tm(c->stream);
+    PhaseTimer total(c), tm(c);
     if (n_in == 0) return 0;
     (void)flags_read_clear(c);
     // 1. input (key order checked), top-nd
@@ -797,8 +777,11 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
         sv = kv;
     }
     st.n_src = ns;
-    st.ms_select += tm.stop();
-    if (ns == 0) return 0;
+    tm.lap(&st.ms_select);
+    if (ns == 0) {
+        kt_resolve(c);
+        return 0;
+    }
     // 2. passes: thirteen 8-byte scratch arrays per candidate record
     const uint64_t n_conn = H.n_conn;
     const uint64_t total_cand = (uint64_t)ns * n_conn;
@@ -831,7 +814,7 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             HIPCHK(hipGetLastError());
             const uint64_t n = n_pass == 1 ? total_cand : read_u64(c, c->d_counter);
             HIPCHK(hipStreamSynchronize(c->stream));
-            st.ms_generate += tm.stop();
+            tm.lap(&st.ms_generate);
             const uint32_t f = flags_read_clear(c);
             if (f & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", SFCI_E_RANGE);
             if (f & FLAG_BAD_INPUT)
@@ -857,7 +840,7 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             k_wide_gather<uint64_t><<<g, 256, 0, c->stream>>>(rhi, I, n, ohi);
             k_wide_gather<double><<<g, 256, 0, c->stream>>>(rv, I, n, ovv);
             HIPCHK(hipGetLastError());
-            st.ms_sort += tm.stop();
+            tm.lap(&st.ms_sort);
             // runs of equal hash, distinct keys per run, exact sums (into the record arrays)
             uint64_t* Kfree = (K == w.a[3]) ? w.a[5] : w.a[3];
             uint64_t* Ifree = (I == w.a[4]) ? w.a[6] : w.a[4];
@@ -895,7 +878,7 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
                                                                   stg.hi + stg.n, stg.y + stg.n);
             HIPCHK(hipGetLastError());
             stg.n += n_keep;
-            st.ms_reduce += tm.stop();
+            tm.lap(&st.ms_reduce);
         }
         if (!overflow) break;
         if (attempt >= 6) throw HipError("candidate passes kept overflowing the workspace", SFCI_E_OVERFLOW);
@@ -922,11 +905,10 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
         k_wide_join<<<grid_of_w(n_out), 256, 0, c->stream>>>(glo, ghi, gy, n_out, ok2, ov);
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    kt_resolve(c);   // fold the sort kernels' event timings (keeps the event pool bounded)
-    st.ms_select += tm.stop();
+    tm.lap(&st.ms_select);
+    total.lap(&st.ms_total);
+    kt_resolve(c);   // synchronizes; folds the kernel and phase timings
     st.n_out = n_out;
-    st.ms_total = total.stop();
     return n_out;
 }
 
