@@ -153,6 +153,14 @@ int sfci_scale(sfci_ctx* ctx, sfci_vec* a, double s);
 int sfci_add_two(sfci_ctx* ctx, sfci_vec* v, const sfci_vec* w, double a);
 /* dynVec::shrinkFrom (src/all.h:281-299): dst <- top-{dst.cap}(src); src is left as is. */
 int sfci_shrink_from(sfci_ctx* ctx, sfci_vec* dst, const sfci_vec* src);
+/* No counterpart in the reference.  sfci_shrink_from for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops): the same result, bit for bit,
+ * with no hipMalloc/hipFree and no allocating library call.  Its scratch is one fixed
+ * 18 KiB device array per GPU (a histogram and per-chunk counts), set up when a context is
+ * created; every call on one GPU shares it, so calls that overlap on one GPU race.
+ * src longer than SFCI_NOALLOC_MAX_N entries is refused with SFCI_E_CAPACITY. */
+#define SFCI_NOALLOC_MAX_N 4294967295LL
+int sfci_shrink_from_noalloc(sfci_ctx* ctx, sfci_vec* dst, const sfci_vec* src);
 /* Restart recombination (src/sparse_solver.cpp:492-516): out <- top-{out.cap}(sum_i c_i v_i),
  * duplicates merged.  out may be one of the inputs. */
 int sfci_lincomb(sfci_ctx* ctx, const sfci_vec* const* vecs, const double* c, int n, sfci_vec* out);

@@ -95,6 +95,7 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         HIPCHK(hipSetDevice(device));
         c.device = device;
         HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        noalloc_setup(device);
         c.n_so = n_so;
         c.n_orb = N;
         c.n_elec = n_elec;
@@ -554,6 +555,16 @@ int sfci_shrink_from(sfci_ctx* X, sfci_vec* dst, const sfci_vec* src) {
     Ctx* c = &X->c;
     return guarded(c, [&] {
         size_t n = select_top(c, src->v.k, src->v.v, (size_t)src->v.len, (size_t)dst->v.cap, dst->v.k, dst->v.v);
+        dst->v.len = (int64_t)n;
+    });
+}
+
+int sfci_shrink_from_noalloc(sfci_ctx* X, sfci_vec* dst, const sfci_vec* src) {
+    if (!X || !dst || !src || dst == src) return SFCI_E_ARG;
+    Ctx* c = &X->c;
+    return guarded(c, [&] {
+        size_t n = dev_shrink_noalloc(c, src->v.k, src->v.v, (size_t)src->v.len, (size_t)dst->v.cap, dst->v.k,
+                                      dst->v.v);
         dst->v.len = (int64_t)n;
     });
 }

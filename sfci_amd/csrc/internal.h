@@ -286,6 +286,13 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb);
 template <typename R>
 void hash_digit_pass(Ctx* c, R* A, R* B, size_t n, int hb);
 
+// noalloc.hip
+// With `device` current: record that device's address of the selection scratch array.
+void noalloc_setup(int device);
+// select_top with the scratch array only (no device allocation); same result as select_top.
+size_t dev_shrink_noalloc(Ctx* c, const uint64_t* k, const double* v, size_t n, size_t cap, uint64_t* ok,
+                          double* ov);
+
 // vecops.hip
 // Keep the `cap` entries ranked highest by (|v| desc, key asc); order-preserving.
 // Output may not alias input.  Returns the kept count.

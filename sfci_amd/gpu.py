@@ -119,6 +119,7 @@ def lib():
         "sfci_scale": (ip, [vp, vp, dbl]),
         "sfci_add_two": (ip, [vp, vp, vp, dbl]),
         "sfci_shrink_from": (ip, [vp, vp, vp]),
+        "sfci_shrink_from_noalloc": (ip, [vp, vp, vp]),
         "sfci_lincomb": (ip, [vp, vp, vp, ip, vp]),
         "sfci_ground_state": (ip, [vp, u64, C.POINTER(LanczosParamsT), vp, vp, C.POINTER(LanczosResultT), vp]),
         "sfci_shrink_cut": (ip, [vp, vp, vp, dbl]),
@@ -322,6 +323,10 @@ class Context:
 
     def shrink_from(self, dst: Vector, src: Vector):
         self._check(lib().sfci_shrink_from(self._h, dst._h, src._h))
+
+    def shrink_from_noalloc(self, dst: Vector, src: Vector):
+        """shrink_from with no device allocation (a fixed scratch array per GPU)."""
+        self._check(lib().sfci_shrink_from_noalloc(self._h, dst._h, src._h))
 
     def lincomb(self, vecs: Sequence[Vector], coeffs, out: Vector):
         arr = (C.c_void_p * len(vecs))(*[v._h.value for v in vecs])
