@@ -58,11 +58,14 @@ typedef struct {
     uint64_t n_out;         /* output length */
     uint64_t n_passes;      /* target-range passes */
     double ms_generate, ms_sort, ms_reduce, ms_select, ms_total;
-    uint64_t n_fallback;    /* passes whose hash-bucket merge overflowed and were re-merged by full sort */
+    uint64_t n_fallback;    /* passes merged by the full sort although planned for hash buckets (a
+                               partition region or a 256-class bucket overflowed) */
     int hash_bits;          /* hash bits of the last pass's bucket merge (0: full sort) */
     uint64_t n_kept;        /* screened H.v: candidates kept for the merge (n_candidates = emitted) */
     double screen_eps;      /* screened H.v: prefilter used (raised when the list would not fit) */
     double ms_score;        /* screened H.v: scoring time */
+    uint64_t n_split;       /* bucket merge: hash classes split because their LDS table filled
+                               (each split rereads the bucket's records once per subclass) */
 } sfci_hv_stats_t;
 
 /* Context from a reference _IN integral file: replaces load_integrals/ReadElements
@@ -95,7 +98,9 @@ int sfci_set_workspace(sfci_ctx* ctx, uint64_t bytes);
 int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
 /* How H.v groups candidates by target (no reference counterpart; a tuning and test knob):
  * mode 0 = automatic (hash buckets for large passes), 1 = full radix sort only,
- * 2 = hash buckets with exactly hash_bits bits (1..24).  Results are identical in every mode. */
+ * 2 = hash buckets with exactly hash_bits bits (1..16), 3 = as 2 with each LDS table counted
+ * full at 1/16 of its usual occupancy (forces nested class splits).  Results are identical in
+ * every mode. */
 int sfci_set_merge_mode(sfci_ctx* ctx, int mode, int hash_bits);
 int sfci_sync(sfci_ctx* ctx);
 

@@ -294,6 +294,7 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     s->n_kept = t.n_kept;
     s->screen_eps = t.screen_eps;
     s->ms_score = t.ms_score;
+    s->n_split = t.n_split;
     return SFCI_OK;
 }
 
@@ -304,7 +305,7 @@ int sfci_set_precision(sfci_ctx* X, int precision) {
 }
 
 int sfci_set_merge_mode(sfci_ctx* X, int mode, int hash_bits) {
-    if (!X || mode < 0 || mode > 2 || (mode == 2 && (hash_bits < 1 || hash_bits > 24))) return SFCI_E_ARG;
+    if (!X || mode < 0 || mode > 3 || (mode >= 2 && (hash_bits < 1 || hash_bits > 16))) return SFCI_E_ARG;
     X->c.merge_mode = mode;
     X->c.merge_hash_bits = hash_bits;
     return SFCI_OK;

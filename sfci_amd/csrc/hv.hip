@@ -1260,12 +1260,20 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // Hash-bucket merge.  After radix_partition_hash the candidates are grouped by the top
 // hb bits of target_hash(target); a bucket holds about a few thousand distinct targets,
 // which one workgroup merges in an LDS open-addressing table with exact 128-bit
-// accumulators (64-bit LDS atomics with carry).  A bucket with more distinct targets
-// than the table holds sets FLAG_BUCKET_FULL and the pass falls back to the full sort.
+// accumulators (64-bit LDS atomics with carry).  When a bucket holds more distinct targets
+// than the table takes at BR_FILL occupancy, the workgroup splits it into hash classes
+// (8 more hash bits below the slot bits, taken modulo a power of two) and merges one class
+// at a time, rereading the bucket's records for each; every finished class is a complete
+// set of targets, so classes split further when they overflow in turn.  Only a class that
+// still overflows at 256 classes sets FLAG_BUCKET_FULL (the pass then takes the full sort).
 constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
-constexpr int BR_BATCH = 8;                    // records per thread loaded together
+constexpr int BR_BATCH = 6;                    // records per thread loaded together
 constexpr int BR_TAGS = 512;                   // tagged targets per bucket finished from a list
+constexpr int BR_FILL = BR_SLOTS * 4 / 5;      // occupied slots at which a table counts as full
+constexpr int BR_TARGET = BR_SLOTS * 3 / 5;    // targets per class aimed at when a bucket splits
+constexpr int BR_STACK = 256;                  // pending classes of one bucket
+constexpr uint32_t BR_MAX_CLASSES = 4096;      // classes of one bucket beyond which the pass takes the full sort
 static_assert(BR_TAGS <= BR_THREADS, "the tag list is finished one entry per thread");
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
@@ -1293,20 +1301,6 @@ __global__ void k_bucket_bounds(const R* __restrict__ ck, size_t n, int hb, uint
     }
 }
 
-// First record whose digit (low 8 bits of the hb-bit target hash) is 128 or more, in a list
-// ascending in that digit (one thread: a binary search).
-template <typename R>
-__global__ void k_digit_split(const R* __restrict__ ck, size_t n, int hb, uint64_t* __restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    size_t lo = 0, hi = n;
-    while (lo < hi) {
-        const size_t mid = (lo + hi) >> 1;
-        if ((target_hash(rec_key(ck[mid]) >> 1, hb) & 255u) < 128u) lo = mid + 1;
-        else hi = mid;
-    }
-    *out = lo;
-}
-
 // Exact accumulation without a carry chain: a term x (|x| < 2^42) is split into three
 // signed limbs with x * 2^84 = l2 * 2^84 + l1 * 2^42 + l0 truncated toward zero (the same
 // integer fx_from gives), using exact FP64 steps: l2 = trunc(x), l1 = trunc(frac * 2^42),
@@ -1332,6 +1326,24 @@ __device__ __forceinline__ fx128 limb_join(unsigned long long l0, unsigned long 
     return fx_add(t, fx128{0, l2 << 20});
 }
 
+// Subclasses for a class whose table filled: its targets D estimated from the fill after
+// n_read of its records (records drawn over D keys: fill = D (1 - exp(-n_read / D)), solved
+// by bisection), at BR_TARGET targets per subclass.  Kept out of line: it runs once per split.
+__device__ __attribute__((noinline)) uint32_t split_count_for(uint32_t fill_u, uint32_t nread_u) {
+    const double fill = (double)fill_u, nread = (double)nread_u;
+    double lo = fill, hi = fmax(fill, 64.0 * nread);
+    double D = hi;
+    if (nread > fill + 0.5) {
+        for (int it = 0; it < 60; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (mid * (1.0 - exp(-nread / mid)) < fill) lo = mid;
+            else hi = mid;
+        }
+        D = hi;
+    }
+    return (uint32_t)fmin(ceil(D / (double)BR_TARGET), (double)BR_STACK);
+}
+
 template <typename T>
 __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     HamDev H, const typename RecOf<T>::type* __restrict__ rec, const uint64_t* __restrict__ bstart,
@@ -1339,203 +1351,260 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     const uint64_t* __restrict__ ik, const double* __restrict__ iv, const double* __restrict__ dterm, size_t n_in,
     double cut,
     uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
-    unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags) {
+    unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags,
+    unsigned long long* split_count, uint32_t fill_max) {
     __shared__ unsigned long long t_key[BR_SLOTS];
     __shared__ unsigned long long t_l0[BR_SLOTS], t_l1[BR_SLOTS], t_l2[BR_SLOTS];
     __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];   // tagged, finished by search
     __shared__ uint32_t t_tl[(BR_SLOTS + 31) / 32];    // tagged, finished from the tag list
     __shared__ uint32_t s_tslot[BR_TAGS], s_tj[BR_TAGS];
-    __shared__ int s_full, s_ntag;
+    __shared__ int s_full, s_ntag, s_sp, s_bad;
+    __shared__ uint32_t s_fill;
+    __shared__ uint32_t s_nread;
+    __shared__ uint32_t s_cm[BR_STACK], s_cq[BR_STACK];   // pending classes: (count, index)
     __shared__ uint32_t s_wtot[BR_THREADS / SFCI_WAVE];
     __shared__ unsigned long long s_obase;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int slot_shift = 48 - hb;             // 16 hash bits below the bucket bits
+    const int cls_shift = 32 - hb;              // the class value: 16 hash bits below the slot bits
+    if (tid == 0) s_bad = 0;
     fx128 dacc = fx_zero();
     unsigned long long uniq = 0;
     bool bad_input = false, overflow = false, any_full = false;
     // one bucket: merge its records [st, en) in the LDS table and finish its targets
+    // one bucket: merge its records [st, en) class by class in the LDS table and finish
+    // their targets; the class stack starts with the whole bucket (modulus 1, class 0)
     auto merge_bucket = [&](const uint64_t st, const uint64_t en) {
-        bool full_here = false;
-        // BR_BATCH records per thread per round, all loaded before any is inserted; full
-        // rounds load without bounds tests, so that their loads are in flight together
-        constexpr uint64_t STEP = (uint64_t)BR_BATCH * BR_THREADS;
-        auto load_full = [&](uint64_t i0, uint64_t* k_, T* v_) {
+        if (tid == 0) {
+            s_sp = 1;
+            s_cm[0] = 1;
+            s_cq[0] = 0;
+        }
+        for (;;) {
+            __syncthreads();
+            const int sp = s_sp;
+            if (sp == 0) break;
+            const uint32_t cm = s_cm[sp - 1], cq = s_cq[sp - 1];
+            __syncthreads();
+            if (tid == 0) s_sp = sp - 1;
+            uint32_t n_ins = 0;     // records of this class this thread inserted
+            // BR_BATCH records per thread per round, all loaded before any is inserted; full
+            // rounds load without bounds tests, so that their loads are in flight together
+            constexpr uint64_t STEP = (uint64_t)BR_BATCH * BR_THREADS;
+            auto load_full = [&](uint64_t i0, uint64_t* k_, T* v_) {
 #pragma unroll
-            for (int r = 0; r < BR_BATCH; ++r) {
-                const auto x = rec[i0 + (uint64_t)r * BR_THREADS + tid];
-                k_[r] = rec_key(x);
-                v_[r] = rec_val(x);
-            }
-        };
-        auto load_round = [&](uint64_t i0, uint64_t* k_, T* v_) {
-#pragma unroll
-            for (int r = 0; r < BR_BATCH; ++r) {
-                const uint64_t i = i0 + (uint64_t)r * BR_THREADS + tid;
-                k_[r] = ~0ull;
-                v_[r] = (T)0;
-                if (i < en) {
-                    const auto x = rec[i];
+                for (int r = 0; r < BR_BATCH; ++r) {
+                    const auto x = rec[i0 + (uint64_t)r * BR_THREADS + tid];
                     k_[r] = rec_key(x);
                     v_[r] = rec_val(x);
                 }
-            }
-        };
-        auto insert_round = [&](const uint64_t* k_, const T* v_) {
+            };
+            auto load_round = [&](uint64_t i0, uint64_t* k_, T* v_) {
 #pragma unroll
-            for (int r = 0; r < BR_BATCH; ++r) {
-                if (k_[r] == ~0ull) continue;
-                const uint64_t raw = k_[r];
-                const unsigned long long key = raw >> 1;
-                uint32_t slot = (uint32_t)((((target_hash_full(key) >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
-                // most records meet a key already in the table (a target collects one
-                // record per connected source): a plain read finds it, and only an empty
-                // slot costs a compare-and-swap.  A slot's key never changes once set.
-                int probe = 0;
-                for (; probe < BR_SLOTS; ++probe) {
-                    const unsigned long long cur =
-                        __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cur == key) break;
-                    if (cur == BR_EMPTY) {
-                        const unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
-                        if (prev == BR_EMPTY || prev == key) break;
+                for (int r = 0; r < BR_BATCH; ++r) {
+                    const uint64_t i = i0 + (uint64_t)r * BR_THREADS + tid;
+                    k_[r] = ~0ull;
+                    v_[r] = (T)0;
+                    if (i < en) {
+                        const auto x = rec[i];
+                        k_[r] = rec_key(x);
+                        v_[r] = rec_val(x);
                     }
-                    slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
                 }
-                if (probe == BR_SLOTS) {
-                    full_here = true;
-                    continue;
+            };
+            auto insert_round = [&](const uint64_t* k_, const T* v_) {
+#pragma unroll
+                for (int r = 0; r < BR_BATCH; ++r) {
+                    if (k_[r] == ~0ull) continue;
+                    const uint64_t raw = k_[r];
+                    const unsigned long long key = raw >> 1;
+                    const uint64_t hf = target_hash_full(key);
+                    // class cq of cm: the cm-th part of the 16-bit class value's range
+                    if ((uint32_t)((((uint32_t)(hf >> cls_shift) & 0xffffu) * cm) >> 16) != cq) continue;
+                    ++n_ins;
+                    uint32_t slot = (uint32_t)((((hf >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
+                    // most records meet a key already in the table (a target collects one
+                    // record per connected source): a plain read finds it, and only an empty
+                    // slot costs a compare-and-swap.  A slot's key never changes once set.
+                    int probe = 0;
+                    for (; probe < BR_SLOTS; ++probe) {
+                        const unsigned long long cur =
+                            __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (cur == key) break;
+                        if (cur == BR_EMPTY) {
+                            const unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
+                            if (prev == BR_EMPTY) {
+                                if (atomicAdd(&s_fill, 1u) + 1u >= fill_max) s_full = 1;
+                                break;
+                            }
+                            if (prev == key) break;
+                        }
+                        slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
+                    }
+                    if (probe == BR_SLOTS) {
+                        s_full = 1;
+                        continue;
+                    }
+                    if (raw & 1ull) {
+                        // the target's own source: (slot, source index) into the tag list, or,
+                        // when that is full, a tag bit for the search in finish_target
+                        const int t = atomicAdd(&s_ntag, 1);
+                        if (t < BR_TAGS) {
+                            s_tslot[t] = slot;
+                            s_tj[t] = tag_index(v_[r]);
+                            atomicOr(&t_tl[slot >> 5], 1u << (slot & 31));
+                        } else {
+                            atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
+                        }
+                        continue;
+                    }
+                    // zero limbs are skipped (l2 is zero for every |term| < 1)
+                    unsigned long long l0, l1, l2;
+                    limb_split((double)v_[r], l0, l1, l2);
+                    if (l0) atomicAdd(&t_l0[slot], l0);
+                    if (l1) atomicAdd(&t_l1[slot], l1);
+                    if (l2) atomicAdd(&t_l2[slot], l2);
                 }
-                if (raw & 1ull) {
-                    // the target's own source: (slot, source index) into the tag list, or,
-                    // when that is full, a tag bit for the search in finish_target
-                    const int t = atomicAdd(&s_ntag, 1);
-                    if (t < BR_TAGS) {
-                        s_tslot[t] = slot;
-                        s_tj[t] = tag_index(v_[r]);
-                        atomicOr(&t_tl[slot >> 5], 1u << (slot & 31));
+            };
+            // the first round's loads are issued before the table is cleared
+            uint64_t ka[BR_BATCH];
+            T va[BR_BATCH];
+            if (st + STEP <= en) load_full(st, ka, va);
+            else load_round(st, ka, va);
+            for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
+                t_key[i] = BR_EMPTY;
+                t_l0[i] = 0;
+                t_l1[i] = 0;
+                t_l2[i] = 0;
+            }
+            for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) {
+                t_tag[i] = 0;
+                t_tl[i] = 0;
+            }
+            if (tid == 0) {
+                s_full = 0;
+                s_fill = 0;
+                s_ntag = 0;
+                s_nread = 0;
+            }
+            __syncthreads();
+            insert_round(ka, va);
+            uint64_t i0 = st + STEP;
+            // a full table ends the class early: each thread stops at its next round
+            for (; i0 + STEP <= en && !__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                 i0 += STEP) {
+                load_full(i0, ka, va);
+                insert_round(ka, va);
+            }
+            if (i0 < en && !__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                load_round(i0, ka, va);
+                insert_round(ka, va);
+                i0 = en;
+            }
+            __syncthreads();
+            const bool full = s_full != 0;
+            if (full) {
+                // split this class into f subclasses (class cq of cm -> classes cq*f .. cq*f+f-1
+                // of cm*f, a refinement of its range).  Its targets are estimated from the
+                // table's fill after n_read of its records: with records drawn over D keys,
+                // fill = D (1 - exp(-n_read / D)), solved for D by bisection.
+                uint32_t r = n_ins;
+                for (int o = SFCI_WAVE / 2; o > 0; o >>= 1) r += __shfl_down(r, o, SFCI_WAVE);
+                if (lane == 0 && r) atomicAdd(&s_nread, r);
+                __syncthreads();
+                if (tid == 0) {
+                    uint32_t f = split_count_for(s_fill, s_nread);
+                    if (f < 2) f = 2;
+                    if (s_sp + (int)f > BR_STACK) f = (uint32_t)(BR_STACK - s_sp);
+                    if (f < 2 || cm * f > BR_MAX_CLASSES) {
+                        s_bad = 1;      // too many classes (or no stack room): the pass takes the full sort
                     } else {
-                        atomicOr(&t_tag[slot >> 5], 1u << (slot & 31));
+                        for (uint32_t i = 0; i < f; ++i) {
+                            s_cm[s_sp + i] = cm * f;
+                            s_cq[s_sp + i] = cq * f + i;
+                        }
+                        s_sp += (int)f;
+                        atomicAdd(split_count, 1ull);
                     }
-                    continue;
                 }
-                // zero limbs are skipped (l2 is zero for every |term| < 1)
-                unsigned long long l0, l1, l2;
-                limb_split((double)v_[r], l0, l1, l2);
-                if (l0) atomicAdd(&t_l0[slot], l0);
-                if (l1) atomicAdd(&t_l1[slot], l1);
-                if (l2) atomicAdd(&t_l2[slot], l2);
+                __syncthreads();
+                if (s_bad) {
+                    any_full = true;
+                    if (tid == 0) s_sp = 0;
+                }
+                continue;
             }
-        };
-        // the first round's loads are issued before the table is cleared
-        uint64_t ka[BR_BATCH];
-        T va[BR_BATCH];
-        if (st + STEP <= en) load_full(st, ka, va);
-        else load_round(st, ka, va);
-        for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
-            t_key[i] = BR_EMPTY;
-            t_l0[i] = 0;
-            t_l1[i] = 0;
-            t_l2[i] = 0;
-        }
-        for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) {
-            t_tag[i] = 0;
-            t_tl[i] = 0;
-        }
-        if (tid == 0) {
-            s_full = 0;
-            s_ntag = 0;
-        }
-        __syncthreads();
-        insert_round(ka, va);
-        uint64_t i0 = st + STEP;
-        for (; i0 + STEP <= en; i0 += STEP) {
-            load_full(i0, ka, va);
-            insert_round(ka, va);
-        }
-        if (i0 < en) {
-            load_round(i0, ka, va);
-            insert_round(ka, va);
-        }
-        if (full_here) s_full = 1;
-        __syncthreads();
-        const bool full = s_full != 0;
-        __syncthreads();
-        if (full) {
-            any_full = true;
-            return;
-        }
-        // finish: every slot's target, and the tag list's, is rounded and cut; the kept
-        // ones (staged under their dense index) are written after one block-wide scan of
-        // the per-thread counts, with one global reservation per bucket
-        constexpr int FR = (BR_SLOTS + BR_THREADS - 1) / BR_THREADS;
-        bool em[FR + 1];
-        double yv[FR + 1];
-        uint64_t dk[FR + 1];
-        uint32_t ne = 0;
+            // finish: every slot's target, and the tag list's, is rounded and cut; the kept
+            // ones (staged under their dense index) are written after one block-wide scan of
+            // the per-thread counts, with one global reservation per class
+            constexpr int FR = (BR_SLOTS + BR_THREADS - 1) / BR_THREADS;
+            bool em[FR + 1];
+            double yv[FR + 1];
+            uint64_t dk[FR + 1];
+            uint32_t ne = 0;
 #pragma unroll
-        for (int r = 0; r < FR; ++r) {
-            const int slot = r * BR_THREADS + tid;
-            em[r] = false;
-            yv[r] = 0.0;
-            dk[r] = 0;
-            const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
-            if (key != BR_EMPTY) {
-                ++uniq;
-                if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
-                    const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                    dk[r] = key + base_dense;
-                    yv[r] = finish_target(H, dk[r], limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik, iv,
-                                          dterm, n_in, cut, em[r], dacc, bad_input);
+            for (int r = 0; r < FR; ++r) {
+                const int slot = r * BR_THREADS + tid;
+                em[r] = false;
+                yv[r] = 0.0;
+                dk[r] = 0;
+                const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
+                if (key != BR_EMPTY) {
+                    ++uniq;
+                    if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
+                        const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
+                        dk[r] = key + base_dense;
+                        yv[r] = finish_target(H, dk[r], limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik,
+                                              iv, dterm, n_in, cut, em[r], dacc, bad_input);
+                    }
+                }
+                ne += em[r] ? 1u : 0u;
+            }
+            // targets that are input determinants, from the tag list (BR_TAGS <= BR_THREADS):
+            // the same finish as finish_target with the source index known, no search
+            {
+                const int ntag = min(s_ntag, BR_TAGS);
+                em[FR] = false;
+                yv[FR] = 0.0;
+                dk[FR] = 0;
+                if (tid < ntag) {
+                    const uint32_t slot = s_tslot[tid], j = s_tj[tid];
+                    const fx128 acc = fx_add(limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), fx_from(dterm[j]));
+                    yv[FR] = fx_to(acc);
+                    em[FR] = fabs(yv[FR]) > cut;
+                    dk[FR] = t_key[slot] + base_dense;
+                    dacc = fx_add(dacc, fx_from(yv[FR] * iv[j]));
+                    ne += em[FR] ? 1u : 0u;
                 }
             }
-            ne += em[r] ? 1u : 0u;
-        }
-        // targets that are input determinants, from the tag list (BR_TAGS <= BR_THREADS):
-        // the same finish as finish_target with the source index known, no search
-        {
-            const int ntag = min(s_ntag, BR_TAGS);
-            em[FR] = false;
-            yv[FR] = 0.0;
-            dk[FR] = 0;
-            if (tid < ntag) {
-                const uint32_t slot = s_tslot[tid], j = s_tj[tid];
-                const fx128 acc = fx_add(limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), fx_from(dterm[j]));
-                yv[FR] = fx_to(acc);
-                em[FR] = fabs(yv[FR]) > cut;
-                dk[FR] = t_key[slot] + base_dense;
-                dacc = fx_add(dacc, fx_from(yv[FR] * iv[j]));
-                ne += em[FR] ? 1u : 0u;
+            uint32_t inc = ne;
+            for (int o = 1; o < SFCI_WAVE; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o, SFCI_WAVE);
+                if (lane >= o) inc += t;
             }
-        }
-        uint32_t inc = ne;
-        for (int o = 1; o < SFCI_WAVE; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, SFCI_WAVE);
-            if (lane >= o) inc += t;
-        }
-        if (lane == SFCI_WAVE - 1) s_wtot[tid / SFCI_WAVE] = inc;
-        __syncthreads();
-        uint32_t pre = inc - ne, tot = 0;
-        for (int i = 0; i < BR_THREADS / SFCI_WAVE; ++i) {
-            const uint32_t wt = s_wtot[i];
-            if (i < tid / SFCI_WAVE) pre += wt;
-            tot += wt;
-        }
-        if (tid == 0) s_obase = tot ? atomicAdd(out_count, (unsigned long long)tot) : 0ull;
-        __syncthreads();
-        uint64_t pos = s_obase + pre;
+            if (lane == SFCI_WAVE - 1) s_wtot[tid / SFCI_WAVE] = inc;
+            __syncthreads();
+            uint32_t pre = inc - ne, tot = 0;
+            for (int i = 0; i < BR_THREADS / SFCI_WAVE; ++i) {
+                const uint32_t wt = s_wtot[i];
+                if (i < tid / SFCI_WAVE) pre += wt;
+                tot += wt;
+            }
+            if (tid == 0) s_obase = tot ? atomicAdd(out_count, (unsigned long long)tot) : 0ull;
+            __syncthreads();
+            uint64_t pos = s_obase + pre;
 #pragma unroll
-        for (int r = 0; r <= FR; ++r) {
-            if (!em[r]) continue;
-            if (pos < out_cap) {
-                sk_out[pos] = dk[r];
-                sv_out[pos] = yv[r];
-            } else {
-                overflow = true;
+            for (int r = 0; r <= FR; ++r) {
+                if (!em[r]) continue;
+                if (pos < out_cap) {
+                    sk_out[pos] = dk[r];
+                    sv_out[pos] = yv[r];
+                } else {
+                    overflow = true;
+                }
+                ++pos;
             }
-            ++pos;
         }
-        __syncthreads();
     };
     uint64_t b = blockIdx.x;
     uint64_t st = 0, en = 0;
@@ -1615,10 +1684,11 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     // two record buffers per pass and one digit byte per first-level slot
     size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
     if (c->merge_mode == 0) {
-        // keep the distinct targets of a pass within 2^16 LDS-table buckets at a load of
-        // at most 0.8, using the distinct/candidate ratio of the previous call
+        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load 0.6, using
+        // the distinct/candidate ratio of the previous call (a bucket that still overflows is
+        // merged in hash classes, each rereading its records)
         const double ratio = std::min(1.0, c->unique_ratio * 1.25);
-        const double bucket_cap = 0.8 * BR_SLOTS * 65536.0 / ratio;
+        const double bucket_cap = 0.6 * BR_SLOTS * 65536.0 / ratio;
         if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
     }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
@@ -1784,21 +1854,22 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         }
         st.n_candidates += n_pass;
         // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (two
-        // partition passes) merged in LDS tables; otherwise, or when a partition region or
-        // a bucket overflows, a full sort on the dense index and the segmented reduction.
+        // partition passes) merged in LDS tables, a bucket with more targets than a table
+        // holds split into hash classes inside the merge kernel; small passes, or a pass whose
+        // partition region overflows, take the full sort on the dense index and the
+        // segmented reduction.
         int hb = 0;
-        if (c->merge_mode == 2) {
+        if (c->merge_mode >= 2) {
             hb = c->merge_hash_bits;
         } else if (c->merge_mode == 0 && n_pass >= ((size_t)1 << 20)) {
             // distinct targets expected in this pass; aim at a table load below 0.6, and drop
             // to a whole number of 8-bit radix digits when that keeps the load below 0.8
             double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.25);
-            while (hb < 24 && est > 0.6 * BR_SLOTS * (double)(1ull << hb)) ++hb;
+            while (hb < 16 && est > 0.6 * BR_SLOTS * (double)(1ull << hb)) ++hb;
             int down = hb & ~7;
             if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS * (double)(1ull << down)) hb = down;
         }
         bool hashed = hb > 0 && hb <= 16;
-        if (hb > 16) st.n_fallback += 1;
         R* rd = A;        // the records in their current order
         R* other = B;     // the free buffer
         uint64_t* bstart = nullptr;
@@ -1839,35 +1910,19 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             ts.lap(&st.ms_sort);
         };
         if (!hashed) sort_path();
-        // overflow of a hash bucket: the pass is split by the next hash bit into two halves
-        // that are merged with twice the buckets (hb + 1 bits); only if that overflows too
-        // does the pass take the full sort
-        bool split = false;
-        size_t split_m = 0;
-        uint64_t* bs2 = nullptr;
-        uint64_t* be2 = nullptr;
+        unsigned long long* d_split = (unsigned long long*)c->d_counter + 3;
         for (int attempt = 0;; ++attempt) {
             PhaseTimer t(c);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
-            if (hashed && split) {
-                const uint64_t nb = (uint64_t)1 << (hb + 1);
-                int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
-                if (split_m)
-                    k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(
-                        H, rd, bstart, bend, nb, hb + 1, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
-                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
-                if (n_pass - split_m)
-                    k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(
-                        H, rd + split_m, bs2, be2, nb, hb + 1, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
-                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
-                kt_end(c, tr);
-            } else if (hashed) {
+            HIPCHK(hipMemsetAsync(d_split, 0, 8, c->stream));
+            if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
                 int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
                 k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, rd, bstart, bend, nb, hb, row_lo * H.dim, sk,
                                                                       sv, dterm, ns, cut_eff, stk, stv, d_out_count,
-                                                                      stage_cap, d_uniq, c->d_acc + 2, c->d_flags);
+                                                                      stage_cap, d_uniq, c->d_acc + 2, c->d_flags,
+                                                                      d_split, c->merge_mode == 3 ? BR_FILL / 16 : BR_FILL);
                 kt_end(c, tr);
             } else {
                 int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
@@ -1878,74 +1933,22 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                 kt_end(c, tr);
             }
             HIPCHK(hipGetLastError());
-            uint64_t cnts[2];
+            uint64_t cnts[3];
             HIPCHK(hipMemcpyAsync(cnts, d_out_count, 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(cnts + 2, d_split, 8, hipMemcpyDeviceToHost, c->stream));
             t.lap(&st.ms_reduce);
             HIPCHK(hipStreamSynchronize(c->stream));
-            uint32_t fl_now = 0;
             if (hashed) {
-                fl_now = flags_read_clear(c);
+                const uint32_t fl_now = flags_read_clear(c);
                 if (fl_now & FLAG_BUCKET_FULL) {
-                    // roll back this pass's output; split it by the next hash bit, or (already
-                    // split, or too few hash bits for the two-level partition) merge it through
-                    // the full sort instead
+                    // a class still overflowed at 256 classes per bucket: roll back this pass's
+                    // output and merge it through the full sort instead
                     st.n_fallback += 1;
-                    c->unique_ratio = std::min(1.0, c->unique_ratio * 2.0);
                     uint64_t restore[2] = {before, uniq_host};
                     HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
                     if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
-                    PhaseTimer ts(c);
-                    bool ok = false;
-                    if (!split && hb > 8 && hb <= 16) {
-                        // records (grouped by hb bits) -> the other buffer, those with hash bit
-                        // hb + 1 clear first; each half regrouped by hb bits in place, with the
-                        // record buffer (grown if needed) as scratch
-                        hash_digit_pass<R>(c, rd, other, n_pass, hb + 8);
-                        uint64_t* d_m = c->d_counter + 2;
-                        k_digit_split<R><<<1, 1, 0, c->stream>>>(other, n_pass, hb + 8, d_m);
-                        HIPCHK(hipGetLastError());
-                        uint64_t m = 0;
-                        HIPCHK(hipMemcpyAsync(&m, d_m, 8, hipMemcpyDeviceToHost, c->stream));
-                        HIPCHK(hipStreamSynchronize(c->stream));
-                        R* recs = other;
-                        DevBuf& sbuf = (rd == A) ? c->cand_k[0] : c->cand_k[1];
-                        R* scratch = (R*)sbuf.get<uint8_t>(
-                            std::max<size_t>(part_capacity(std::max<size_t>(m, n_pass - m)) * sizeof(R),
-                                             n_pass * (sizeof(uint64_t) + sizeof(T))));
-                        if (rd == A) A = scratch;
-                        else B = scratch;
-                        R* g0 = m ? hash_partition<R>(c, recs, scratch, m, hb) : recs;
-                        R* g1 = (n_pass - m) ? hash_partition<R>(c, recs + m, scratch, n_pass - m, hb) : recs + m;
-                        rd = recs;
-                        other = scratch;
-                        if (g0 == recs && g1 == recs + m) {
-                            const size_t nb2 = (size_t)1 << (hb + 1);
-                            bstart = c->bucket_bounds.get<uint64_t>(4 * nb2);
-                            bend = bstart + nb2;
-                            bs2 = bend + nb2;
-                            be2 = bs2 + nb2;
-                            const unsigned gb = (unsigned)((nb2 + SFCI_BLOCK - 1) / SFCI_BLOCK);
-                            if (m)
-                                k_bucket_bounds<R><<<gb, SFCI_BLOCK, 0, c->stream>>>(rd, m, hb + 1, nb2, bstart, bend);
-                            if (n_pass - m)
-                                k_bucket_bounds<R><<<gb, SFCI_BLOCK, 0, c->stream>>>(rd + m, n_pass - m, hb + 1, nb2,
-                                                                                    bs2, be2);
-                            HIPCHK(hipGetLastError());
-                            split = true;
-                            split_m = m;
-                            ok = true;
-                        } else {
-                            // a region of the regrouping overflowed: the records are all in rd
-                            uint32_t f2 = flags_read_clear(c);
-                            if (f2 & ~(uint32_t)FLAG_REGION_FULL) atomic_flags_restore(c, f2 & ~(uint32_t)FLAG_REGION_FULL);
-                        }
-                    }
-                    ts.lap(&st.ms_sort);
-                    if (!ok) {
-                        hashed = false;
-                        split = false;
-                        sort_path();
-                    }
+                    hashed = false;
+                    sort_path();
                     continue;
                 }
                 if (fl_now & ~(uint32_t)FLAG_OUT_OVERFLOW) atomic_flags_restore(c, fl_now);
@@ -1953,11 +1956,12 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             if (cnts[0] <= stage_cap) {
                 out_count_host = cnts[0];
                 uniq_host = cnts[1];
+                st.n_split += cnts[2];
                 diag_total = fx_add(diag_total, acc_read_raw(c, 1));
                 break;
             }
             if (attempt > 2) throw HipError("merged-output staging overflow after regrow", -3);
-            // grow the staging list, keep earlier passes' output, redo this pass
+            // grow the staging list, keep earlier passes' output, redo this pass's merge
             size_t ncap = cnts[0] + (cnts[0] >> 2);
             DevBuf nk, nv;
             uint64_t* nkp = nk.get<uint64_t>(ncap);
@@ -1981,7 +1985,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             uint32_t fl = flags_read_clear(c);
             if (fl & FLAG_RANGE) throw HipError("a term left the fixed-point range (|x| >= 2^42)", -5);
         }
-        st.hash_bits = hashed ? (split ? hb + 1 : hb) : 0;
+        st.hash_bits = hashed ? hb : 0;
     }
     uint64_t uniq = uniq_host;
     *diag = fx_to(diag_total);

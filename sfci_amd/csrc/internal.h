@@ -251,6 +251,7 @@ struct Ctx {
     // statistics of the last H.v
     struct {
         uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0, n_fallback = 0;
+        uint64_t n_split = 0;        // hash-bucket classes split because an LDS table filled
         int hash_bits = 0;
         double ms_generate = 0, ms_sort = 0, ms_reduce = 0, ms_select = 0, ms_total = 0;
         uint64_t n_kept = 0;         // screened: candidates kept for the merge
@@ -281,10 +282,6 @@ void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* 
 size_t part_capacity(size_t n);
 template <typename R>
 R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb);
-// One dense partition pass A -> B by the low 8 bits of the hb-bit target hash (hb = h + 8:
-// the records whose hash bit h + 1 is clear come first).
-template <typename R>
-void hash_digit_pass(Ctx* c, R* A, R* B, size_t n, int hb);
 
 // noalloc.hip
 // With `device` current: record that device's address of the selection scratch array.

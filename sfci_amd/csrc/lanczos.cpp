@@ -122,10 +122,11 @@ extern "C" int sfci_ground_state(sfci_ctx* ctx, uint64_t init_det, const sfci_la
                 sfci_hv_stats_t hs{};
                 sfci_hv_stats(ctx, &hs);
                 printf("iter %d: len %lld alpha %.12f beta %.12f time %.3f ms | hv %.1f ms (gen %.1f sort %.1f "
-                       "reduce %.1f) passes %llu unique %llu out %llu fallback %llu hb %d\n",
+                       "reduce %.1f) passes %llu unique %llu out %llu fallback %llu hb %d split %llu\n",
                        i, (long long)sfci_vec_len(v), alpha, beta, ms_since(t_it), hs.ms_total, hs.ms_generate,
                        hs.ms_sort, hs.ms_reduce, (unsigned long long)hs.n_passes, (unsigned long long)hs.n_unique,
-                       (unsigned long long)hs.n_out, (unsigned long long)hs.n_fallback, hs.hash_bits);
+                       (unsigned long long)hs.n_out, (unsigned long long)hs.n_fallback, hs.hash_bits,
+                       (unsigned long long)hs.n_split);
                 fflush(stdout);
             }
         }

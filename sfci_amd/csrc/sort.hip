@@ -733,18 +733,6 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
     return A;
 }
 
-// One partition pass of the n records of A into B (dense, ascending in the digit) by the
-// low 8 bits of the hb-bit target hash prefix.  With hb = h + 8 the digit's top bit is hash
-// bit h + 1, so the records with that bit clear come first.
-template <typename R>
-void hash_digit_pass(Ctx* c, R* A, R* B, size_t n, int hb) {
-    if (hb < 8 || hb > 24) throw HipError("hash_digit_pass: 8..24 hash bits", -1);
-    Chunks ch = make_chunks(c, 0, {0}, {(uint64_t)n});
-    part_pass<R>(c, A, B, ch, DigitHash{0, hb}, n);
-}
-
-template void hash_digit_pass<Rec64>(Ctx*, Rec64*, Rec64*, size_t, int);
-template void hash_digit_pass<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 template Rec64* hash_partition<Rec64>(Ctx*, Rec64*, Rec64*, size_t, int);
 template Rec32* hash_partition<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 

@@ -38,7 +38,8 @@ class StatsT(C.Structure):
                 ("n_out", C.c_uint64), ("n_passes", C.c_uint64), ("ms_generate", C.c_double),
                 ("ms_sort", C.c_double), ("ms_reduce", C.c_double), ("ms_select", C.c_double),
                 ("ms_total", C.c_double), ("n_fallback", C.c_uint64), ("hash_bits", C.c_int),
-                ("n_kept", C.c_uint64), ("screen_eps", C.c_double), ("ms_score", C.c_double)]
+                ("n_kept", C.c_uint64), ("screen_eps", C.c_double), ("ms_score", C.c_double),
+                ("n_split", C.c_uint64)]
 
 
 class KernelStatT(C.Structure):

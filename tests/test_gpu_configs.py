@@ -87,7 +87,7 @@ def c2(wdir):
 
 
 @pytest.mark.parametrize("case", ["single_pass", "multipass_auto", "multipass_12bits", "multipass_16bits",
-                                  "mixed_multipass", "nd_shift"])
+                                  "mixed_multipass", "nd_shift", "split_9bits", "nested_split_12bits"])
 def test_c2_sample_hv_bit_exact(c2, case):
     w, ctx, orc, k, v = c2
     nd, lam, coeff, mode = len(k), 0.0, 1.0, 0
@@ -103,6 +103,11 @@ def test_c2_sample_hv_bit_exact(c2, case):
         mode = 2
     elif case == "nd_shift":
         nd, lam, coeff = 1500, 0.41, -0.75
+    elif case == "split_9bits":
+        ctx.set_workspace(8 << 30)            # one pass: tens of thousands of targets per bucket
+        ctx.set_merge_mode(2, 9)
+    elif case == "nested_split_12bits":
+        ctx.set_merge_mode(3, 12)             # tables counted full at 1/16: classes split again
     log("c2 hv case %s" % case)
     try:
         gk, gv, gd = gpu_hv(ctx, k, v, nd, lam, coeff, 60_000_000)
@@ -117,8 +122,13 @@ def test_c2_sample_hv_bit_exact(c2, case):
     assert_same_vec(gk, gv, ok, ov)
     assert bits([gd])[0] == bits([od])[0], (gd, od)
     assert st["n_candidates"] == min(nd, len(k)) * ctx.info()["n_conn"]
-    if case != "single_pass":
+    if case not in ("single_pass", "split_9bits"):
         assert st["n_passes"] > 1
+    assert st["n_fallback"] == 0
+    if case == "split_9bits":
+        assert st["n_split"] >= 512 and st["hash_bits"] == 9
+    if case == "nested_split_12bits":
+        assert st["n_split"] > 4096 and st["hash_bits"] == 12
     if case == "multipass_12bits":
         assert st["hash_bits"] == 12
     if case == "multipass_16bits":

@@ -155,10 +155,11 @@ def test_hv_multipass_matches(sys1, orc1, ctx1):
     assert bits([gd])[0] == bits([od])[0]
 
 
-@pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14)])
+@pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14), (3, 4)])
 def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
-    # full sort, hash buckets, and hash buckets too few to hold the targets (each
-    # overflowing bucket set falls back to the full sort) give the same bits
+    # full sort, hash buckets, hash buckets too few to hold the targets (each bucket is
+    # split into hash classes merged one at a time), and tables counted full at 1/16 of
+    # their occupancy (classes split again and again) give the same bits
     k, v = random_input(sys1, 3000, 31)
     ctx1.set_merge_mode(mode, hbits)
     try:
@@ -166,9 +167,14 @@ def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
         st = ctx1.stats()
     finally:
         ctx1.set_merge_mode(0, 0)
-    log("merge mode %d bits %d: fallback %d hash_bits %d" % (mode, hbits, st["n_fallback"], st["hash_bits"]))
-    if mode == 2 and hbits == 1:
-        assert st["n_fallback"] >= 1
+    log("merge mode %d bits %d: fallback %d split %d hash_bits %d" % (
+        mode, hbits, st["n_fallback"], st["n_split"], st["hash_bits"]))
+    assert st["n_fallback"] == 0
+    if (mode == 2 and hbits == 1) or mode == 3:
+        assert st["n_split"] >= 2
+    if mode == 3:
+        # nested: more splits than buckets
+        assert st["n_split"] > (1 << hbits)
     ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=0.05, coeff=1.0, cut=1e-5)
     assert_same_vec(gk, gv, ok, ov)
     assert bits_of(gd) == bits_of(od)
@@ -181,7 +187,7 @@ def bits_of(x):
 @pytest.mark.parametrize("mixed", [False, True])
 def test_hv_bucket_overflow_split(make_system, mixed):
     """Hash buckets too few for a pass's distinct targets (9 bits forced, 3.2e6 targets for
-    512 tables of 4608 slots): the pass is split by the next hash bit and merged with 10 bits,
+    512 tables of 4608 slots): each bucket is split into hash classes merged one at a time,
     with the same bits as the oracle."""
     from sfci_amd import gpu
     s = make_system(16, 8, 3)
@@ -198,8 +204,9 @@ def test_hv_bucket_overflow_split(make_system, mixed):
         st = ctx.stats()
     finally:
         ctx.close()
-    log("overflow split: fallback %d hash_bits %d unique %d" % (st["n_fallback"], st["hash_bits"], st["n_unique"]))
-    assert st["n_fallback"] >= 1 and st["hash_bits"] == 10
+    log("overflow split: fallback %d split %d hash_bits %d unique %d" % (
+        st["n_fallback"], st["n_split"], st["hash_bits"], st["n_unique"]))
+    assert st["n_fallback"] == 0 and st["n_split"] >= 512 and st["hash_bits"] == 9
     ok, ov, od, _ = o.hv(k, v, nd=len(k), lam=0.1, coeff=1.0, cut=1e-5, mode=2 if mixed else 0)
     assert_same_vec(gk, gv, ok, ov)
     assert bits_of(gd) == bits_of(od)
@@ -301,7 +308,7 @@ def test_hv_mixed_bit_exact(case, sys1, orc1, ctx1):
     if case == "multipass":
         assert st["n_passes"] > 1
     if case == "hash_fallback":
-        assert st["n_fallback"] >= 1
+        assert st["n_split"] >= 1 and st["n_fallback"] == 0
     ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=lam, coeff=coeff, cut=1e-5, mode=2)
     assert_same_vec(gk, gv, ok, ov)
     assert bits_of(gd) == bits_of(od)
