@@ -29,19 +29,6 @@ import numpy as np  # noqa: E402
 
 METRIC = "H\u00b7v connections/s + time-to-E within 1e-5 Ha, NN/FP32 vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch (fetch + write) measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on the default workload in FP64, scaled by the counters' reading of a plain
-# copy of known size (summary in profiles/); other workloads and precisions report null
-PMC_TRAFFIC = {
-    ("c2", "fp64"): {
-        "k_generate": 8.559e10,
-        "k_part_scatter_cur<DigitHash>": 1.733e11,
-        "k_part_scatter<DigitHash>": 1.641e11,
-        "k_part_hist<DigitHash>": 5.672e9,
-        "k_bucket_reduce": 8.148e10,
-    },
-}
-
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -131,40 +118,31 @@ def time_to_e(ctx, w, nd, args):
     print(json.dumps(res))
 
 
-def screener(args):
-    """Learned-screener workload (32 spatial / 64 spin orbitals, 16 electrons): training
-    samples from converged plain FP64 solves on seeds 10-19, the network trained on the GPU,
-    then three solves of seed 4 timed from the HF determinant: plain FP64, network-screened
-    mixed precision, and the same screening ranked by |H_ij c_j| alone (no network)."""
+def train_network(ctx, name, data, seeds, nd_tr, nl_tr, restarts_tr, eps, samples_per_seed, steps, batch, lr, cap):
+    """Training samples from plain FP64 solves of the named workload's orbital space on other
+    seeds (the last seed held out), and a network trained on them on the GPU.  Returns
+    (network, quality on the held-out seed, data seconds, training seconds)."""
     from sfci_amd import gpu, integrals as I
-
-    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
-    nd, nl, nd_tr = args.nd or 200_000, args.nl or 30, args.nd_train
     samples, t_data = [], time.perf_counter()
-    for seed in range(10, 20):
-        w = I.make_workload("c5", data, seed=seed)
+    for seed in seeds:
+        w = I.make_workload(name, data, seed=seed)
         c = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
         v = c.vector(nd_tr)
-        r = c.ground_state(w["init_hf"], nd_tr, 10, 3, hpsi_cap=args.cap, v_out=v, e_prev_init=float("inf"))
-        s = c.screen_samples(v, args.eps, args.samples_per_seed, seed=seed)
-        samples.append(s)
-        log("training data seed %d: E %.8f after %d restarts, %d samples" % (seed, r["e_final"], r["restarts"], len(s)))
+        r = c.ground_state(w["init_hf"], nd_tr, nl_tr, restarts_tr, hpsi_cap=cap, v_out=v, e_prev_init=float("inf"))
+        smp = c.screen_samples(v, eps, samples_per_seed, seed=seed)
+        samples.append(smp)
+        log("training data seed %d: E %.8f after %d restarts, %d samples" % (seed, r["e_final"], r["restarts"],
+                                                                            len(smp)))
         c.close()
     t_data = time.perf_counter() - t_data
     held = samples[-1]
     train = np.concatenate(samples[:-1])
-    w = I.make_workload("c5", data)
-    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
     net = gpu.Mlp(ctx, seed=0)
-    xs, ys = float(train["logv"].mean()), float(train["logv"].std())
-    yl, ysd = float(train["label"].mean()), float(train["label"].std())
-    net.set_norm(xs, 1.0 / ys, yl, 1.0 / ysd)
-    ctx.kernel_stats(reset=True)
+    net.set_norm(float(train["logv"].mean()), 1.0 / float(train["logv"].std()), float(train["label"].mean()),
+                 1.0 / float(train["label"].std()))
     t = time.perf_counter()
-    curve = net.train(train, steps=args.train_steps, batch=args.batch, lr=args.lr, seed=1)
+    curve = net.train(train, steps=steps, batch=batch, lr=lr, seed=1)
     t_train = time.perf_counter() - t
-    ks = ctx.kernel_stats()["mlp_train_step"]
-    # held-out seed: how well each score ranks candidates by the label (log|y| of the target)
     pred = net.predict_samples(held)
 
     def rank_corr(a, b):
@@ -176,9 +154,93 @@ def screener(args):
         top_true = set(np.argsort(-held["label"])[:k].tolist())
         return len(top_true & set(np.argsort(-score)[:k].tolist())) / k
 
-    quality = dict(spearman_network=rank_corr(pred, held["label"]), spearman_magnitude=rank_corr(held["logv"], held["label"]),
+    quality = dict(spearman_network=rank_corr(pred, held["label"]),
+                   spearman_magnitude=rank_corr(held["logv"], held["label"]),
                    top10pct_recall_network=top_recall(pred), top10pct_recall_magnitude=top_recall(held["logv"]),
-                   loss_first=float(curve[0]), loss_last=float(np.mean(curve[-20:])))
+                   loss_first=float(curve[0]), loss_last=float(np.mean(curve[-20:])), train_samples=int(len(train)))
+    return net, quality, t_data, t_train
+
+
+def time_to_e_block(args):
+    """Time to the ground-state energy at the CPU-runnable workload (12 spatial / 24 spin
+    orbitals, 8 electrons, seed 0, Nd 2e4, Nl 20, at most 5 restarts): the reference's solve
+    (src/sparse_solver.cpp:246-548, timed as src/main.cpp:267-276) in plain FP64, in mixed
+    precision, and in mixed precision with the learned screener (network trained here on seeds
+    10-18 of the same orbital space, 19 held out), each on the GPU, plus the oracle's FP64
+    solve on the host cores.  Every energy is compared with the oracle's."""
+    from oracle import oracle as O
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    w = I.make_workload("c1", data)
+    nd, nl, nr = w["nd"], w["nl"], w["restarts"]
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    net, quality, t_data, t_train = train_network(ctx, "c1", data, range(10, 20), nd, nl, nr, args.eps, 400_000,
+                                                  args.train_steps, args.batch, args.lr, args.cap)
+
+    def solve(mode, mixed):
+        ctx.set_precision(mixed)
+        ctx.set_screen(mode, net if mode == 2 else None, 3.0, args.eps)
+        try:
+            ctx.ground_state(w["init_hf"], nd, nl, nr, hpsi_cap=args.cap, e_prev_init=float("inf"))   # warm
+            ctx.sync()
+            t = time.perf_counter()
+            r = ctx.ground_state(w["init_hf"], nd, nl, nr, hpsi_cap=args.cap, e_prev_init=float("inf"))
+            dt = time.perf_counter() - t
+        finally:
+            ctx.set_screen(0)
+            ctx.set_precision(False)
+        return dict(seconds=dt, e_final=r["e_final"], restarts=r["restarts"], n_hv=r["n_hv"],
+                    e_trunc=[float(x) for x in r["e_trunc"]], s_restart=[x / 1e3 for x in r["ms_restart"]],
+                    screened_restarts=r["n_screened"], rejected_restarts=r["n_reverts"])
+
+    runs = dict(fp64=solve(0, False), mixed=solve(0, True), network_screened_mixed=solve(2, True))
+    ctx.close()
+    orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t = time.perf_counter()
+    o = orc.lanczos(w["init_hf"], nd, nl, nr, hpsi_cap=args.cap, nthreads=threads, e_prev_init=float("inf"))
+    t_cpu = time.perf_counter() - t
+    e64 = runs["fp64"]["e_final"]
+    for tag, x in runs.items():
+        x["energy_error_vs_oracle"] = x["e_final"] - o["e_final"]
+        # time to E: the end of the first restart whose energy is within 1e-5 Ha of the plain
+        # FP64 GPU solve's final energy
+        hit = [t for e, t in zip(x["e_trunc"], x["s_restart"]) if e <= e64 + 1e-5]
+        x["time_to_e_s"] = hit[0] if hit else None
+    t64 = runs["fp64"]["time_to_e_s"]
+    for tag, x in runs.items():
+        x["time_to_e_speedup_vs_fp64"] = t64 / x["time_to_e_s"] if x["time_to_e_s"] else None
+        log("time to E, %s: %.3f s, E %.12f, error vs oracle %.2e" % (tag, x["seconds"], x["e_final"],
+                                                                      x["energy_error_vs_oracle"]))
+    return dict(workload="12 spatial / 24 spin orbitals, 8 electrons, seed 0, Nd=%d, Nl=%d, restarts<=%d, "
+                         "HPsiCap=%d" % (nd, nl, nr, args.cap),
+                stop="two restarts' Ritz values differ by < 1e-5 Ha (first comparison against +inf)",
+                **runs,
+                oracle_fp64=dict(seconds=t_cpu, e_final=o["e_final"], restarts=o["restarts"], cores=threads,
+                                 kind="port"),
+                gpu_fp64_over_oracle=t_cpu / runs["fp64"]["seconds"],
+                screener=dict(keep_factor=3.0, eps_pre=args.eps, train_seeds="10-18 (19 held out)",
+                              data_seconds=t_data, train_seconds=t_train, steps=args.train_steps, batch=args.batch,
+                              **quality))
+
+
+def screener(args):
+    """Learned-screener workload (32 spatial / 64 spin orbitals, 16 electrons): training
+    samples from converged plain FP64 solves on seeds 10-19, the network trained on the GPU,
+    then three solves of seed 4 timed from the HF determinant: plain FP64, network-screened
+    mixed precision, and the same screening ranked by |H_ij c_j| alone (no network)."""
+    from sfci_amd import gpu, integrals as I
+
+    data = args.data or os.path.join(tempfile.gettempdir(), "sfci_bench")
+    nd, nl, nd_tr = args.nd or 200_000, args.nl or 30, args.nd_train
+    w = I.make_workload("c5", data)
+    ctx = gpu.Context(w["in_path"], w["n_so"], w["n_elec"])
+    ctx.kernel_stats(reset=True)
+    net, quality, t_data, t_train = train_network(ctx, "c5", data, range(10, 20), nd_tr, 10, 3, args.eps,
+                                                  args.samples_per_seed, args.train_steps, args.batch, args.lr,
+                                                  args.cap)
+    ks = ctx.kernel_stats()["mlp_train_step"]
     log("network: %s" % quality)
 
     keeps = [float(k) for k in str(args.keep).split(",")]
@@ -248,7 +310,7 @@ def screener(args):
                config=dict(workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d, Nl=%d (reduced Nd)"
                            % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, nl),
                            keep_factor=keeps[0], eps_pre=args.eps, train_seeds="10-18 (19 held out)",
-                           train_nd=nd_tr, train_samples=int(len(train)), train_steps=args.train_steps,
+                           train_nd=nd_tr, train_samples=quality["train_samples"], train_steps=args.train_steps,
                            batch=args.batch),
                plain_fp64=plain, network_screened=nn, magnitude_screened=mag, network_quality=quality,
                by_keep_factor={str(k): v for k, v in runs.items()}, prefilter_only=pre,
@@ -432,6 +494,7 @@ def main():
     ap.add_argument("--nd", type=int, default=None, help="override Nd")
     ap.add_argument("--cap", type=int, default=60_000_000, help="H.v output capacity (HPsiCap)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tte", action="store_true", help="leave out the time-to-E block of the default run")
     ap.add_argument("--data", default=None, help="directory for the generated integral files")
     ap.add_argument("--time-to-e", action="store_true",
                     help="run the whole ground-state solve instead (GPU, and the CPU oracle with --cpu)")
@@ -516,7 +579,9 @@ def main():
     dom = max(ks.items(), key=lambda kv: kv[1]["ms"])
     dom_name, d = dom
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    traffic = PMC_TRAFFIC.get((args.workload, args.precision), {}).get(dom_name) if args.nd is None else None
+    # HBM bytes per launch need rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs of their own (see
+    # profiles/); they are not collected in this run
+    traffic = None
     roofline = dict(bound="hbm", kernel=dom_name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=traffic,
                     avg_launch_ms=d["ms"] / max(1, d["launches"]),
@@ -557,8 +622,11 @@ def main():
         res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     else:
         res["cpu_baseline"] = None
-    print(json.dumps(res))
+    del v, out
     ctx.close()
+    if not args.no_tte and args.nd is None and args.workload == "c2" and not mixed:
+        res["time_to_e"] = time_to_e_block(args)
+    print(json.dumps(res))
 
 
 if __name__ == "__main__":

@@ -1684,11 +1684,12 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     // two record buffers per pass and one digit byte per first-level slot
     size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
     if (c->merge_mode == 0) {
-        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load 0.6, using
-        // the distinct/candidate ratio of the previous call (a bucket that still overflows is
-        // merged in hash classes, each rereading its records)
+        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load 0.9 (more
+        // passes cost more generation than the occasional class split), using the distinct/
+        // candidate ratio of the previous call; a bucket that overflows is merged in hash
+        // classes, each rereading its records
         const double ratio = std::min(1.0, c->unique_ratio * 1.25);
-        const double bucket_cap = 0.6 * BR_SLOTS * 65536.0 / ratio;
+        const double bucket_cap = 1.5 * 0.6 * BR_SLOTS * 65536.0 / ratio;
         if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
     }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
