@@ -166,8 +166,10 @@ def time_to_e_block(args):
     orbitals, 8 electrons, seed 0, Nd 2e4, Nl 20, at most 5 restarts): the reference's solve
     (src/sparse_solver.cpp:246-548, timed as src/main.cpp:267-276) in plain FP64, in mixed
     precision, and in mixed precision with the learned screener (network trained here on seeds
-    10-18 of the same orbital space, 19 held out), each on the GPU, plus the oracle's FP64
-    solve on the host cores.  Every energy is compared with the oracle's."""
+    10-18 of the same orbital space, 19 held out; candidates ranked by the network's predicted
+    ln|y| of their target plus ln|H_ij c_j|, by the network alone, and by |H_ij c_j| alone),
+    each on the GPU, plus the oracle's FP64 solve on the host cores.  Every energy is compared
+    with the oracle's."""
     from oracle import oracle as O
     from sfci_amd import gpu, integrals as I
 
@@ -180,7 +182,7 @@ def time_to_e_block(args):
 
     def solve(mode, mixed):
         ctx.set_precision(mixed)
-        ctx.set_screen(mode, net if mode == 2 else None, 3.0, args.eps)
+        ctx.set_screen(mode, net if mode >= 2 else None, 3.0, args.eps)
         try:
             ctx.ground_state(w["init_hf"], nd, nl, nr, hpsi_cap=args.cap, e_prev_init=float("inf"))   # warm
             ctx.sync()
@@ -194,7 +196,10 @@ def time_to_e_block(args):
                     e_trunc=[float(x) for x in r["e_trunc"]], s_restart=[x / 1e3 for x in r["ms_restart"]],
                     screened_restarts=r["n_screened"], rejected_restarts=r["n_reverts"])
 
-    runs = dict(fp64=solve(0, False), mixed=solve(0, True), network_screened_mixed=solve(2, True))
+    # screened runs keep 3 Nd candidates per product, ranked by: the network's predicted ln|y|
+    # of the target plus ln|H_ij c_j| (mode 3), the network alone (2), |H_ij c_j| alone (1)
+    runs = dict(fp64=solve(0, False), mixed=solve(0, True), network_magnitude_screened_mixed=solve(3, True),
+                network_screened_mixed=solve(2, True), magnitude_screened_mixed=solve(1, True))
     ctx.close()
     orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -247,7 +252,7 @@ def screener(args):
 
     def solve(tag, mode, mixed, keep=keeps[0]):
         ctx.set_precision(mixed)
-        ctx.set_screen(mode, net if mode == 2 else None, keep, args.eps)
+        ctx.set_screen(mode, net if mode >= 2 else None, keep, args.eps)
         ctx.sync()
         t0 = time.perf_counter()
         r = ctx.ground_state(w["init_hf"], nd, nl, args.restarts or 8, hpsi_cap=args.cap, e_prev_init=float("inf"),
@@ -269,9 +274,9 @@ def screener(args):
     out = ctx.vector(args.cap)
     single = {}
     for tag, mode, mixed in (("plain_fp64", 0, False), ("plain_mixed", 0, True), ("magnitude", 1, True),
-                             ("network", 2, True)):
+                             ("network", 2, True), ("network_magnitude", 3, True)):
         ctx.set_precision(mixed)
-        ctx.set_screen(mode, net if mode == 2 else None, keeps[0], args.eps)
+        ctx.set_screen(mode, net if mode >= 2 else None, keeps[0], args.eps)
         ctx.hv(vin, out, nd)
         ctx.hv(vin, out, nd)
         st = ctx.stats()
@@ -293,7 +298,8 @@ def screener(args):
     runs = {}
     abl = [float(k) for k in str(args.ablation_keep).split(",") if k]
     for kf in keeps:
-        runs[kf] = dict(network=solve("network-screened mixed, keep %g Nd" % kf, 2, True, kf))
+        runs[kf] = dict(network_magnitude=solve("network x magnitude screened mixed, keep %g Nd" % kf, 3, True, kf),
+                        network=solve("network-screened mixed, keep %g Nd" % kf, 2, True, kf))
         if kf in abl:
             runs[kf]["magnitude"] = solve("magnitude-screened mixed, keep %g Nd" % kf, 1, True, kf)
         for x in runs[kf].values():
@@ -302,7 +308,7 @@ def screener(args):
     # the prefilter alone: every candidate with |H_ij c_j| > eps merged, nothing ranked
     pre = solve("prefilter only (eps %g, nothing dropped by rank)" % args.eps, 1, True, 1e12)
     annotate(pre)
-    nn, mag = runs[keeps[0]]["network"], runs[keeps[0]].get("magnitude")
+    nn, mag = runs[keeps[0]]["network_magnitude"], runs[keeps[0]].get("magnitude")
     res = dict(metric="time-to-E within 1e-5 Ha of the plain FP64 GPU solve (learned screener)",
                value=nn["time_to_plain_energy_s"] if nn["time_to_plain_energy_s"] is not None else nn["seconds"],
                unit="s", higher_is_better=False, dtype="bf16 network / f32 products / f64",
@@ -312,7 +318,8 @@ def screener(args):
                            keep_factor=keeps[0], eps_pre=args.eps, train_seeds="10-18 (19 held out)",
                            train_nd=nd_tr, train_samples=quality["train_samples"], train_steps=args.train_steps,
                            batch=args.batch),
-               plain_fp64=plain, network_screened=nn, magnitude_screened=mag, network_quality=quality,
+               plain_fp64=plain, network_magnitude_screened=nn, network_only_screened=runs[keeps[0]]["network"],
+               magnitude_screened=mag, network_quality=quality,
                by_keep_factor={str(k): v for k, v in runs.items()}, prefilter_only=pre,
                single_hv=single,
                training=dict(seconds=t_train, data_seconds=t_data, ms_per_step=ks["ms"] / max(1, ks["launches"])))
@@ -356,9 +363,9 @@ def screen_hv(args):
     net.train(smp, steps=200, batch=args.batch, lr=args.lr, seed=1)
     out = ctx.vector(args.cap)
     res = {}
-    for tag, mode in (("plain_mixed", 0), ("magnitude", 1), ("network", 2)):
+    for tag, mode in (("plain_mixed", 0), ("magnitude", 1), ("network", 2), ("network_magnitude", 3)):
         ctx.set_precision(True)
-        ctx.set_screen(mode, net if mode == 2 else None, float(str(args.keep).split(",")[0]), args.eps)
+        ctx.set_screen(mode, net if mode >= 2 else None, float(str(args.keep).split(",")[0]), args.eps)
         ctx.hv(v, out, nd)
         ctx.sync()
         ctx.kernel_stats(reset=True)

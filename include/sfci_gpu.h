@@ -339,6 +339,7 @@ int sfci_mlp_sample_features(sfci_mlp* net, const sfci_screen_sample_t* samples,
 #define SFCI_SCREEN_OFF 0
 #define SFCI_SCREEN_MAGNITUDE 1   /* rank candidates by |H_ij coeff c_j| */
 #define SFCI_SCREEN_NETWORK 2     /* rank candidates by the network's prediction */
+#define SFCI_SCREEN_NETWORK_MAGNITUDE 3   /* rank by the network's predicted ln|y| of the target plus ln|H_ij c_j| */
 /* Screening of sfci_hv (see above); net is required for SFCI_SCREEN_NETWORK and must
  * outlive its use. */
 int sfci_set_screen(sfci_ctx* ctx, int mode, sfci_mlp* net, double keep_factor, double eps_pre);

@@ -461,12 +461,12 @@ int sfci_hv(sfci_ctx* X, const sfci_vec* in, sfci_vec* out, int64_t nd, double l
 }
 
 int sfci_set_screen(sfci_ctx* X, int mode, sfci_mlp* net, double keep_factor, double eps_pre) {
-    if (!X || mode < SFCI_SCREEN_OFF || mode > SFCI_SCREEN_NETWORK || (mode == SFCI_SCREEN_NETWORK && !net) ||
+    if (!X || mode < SFCI_SCREEN_OFF || mode > SFCI_SCREEN_NETWORK_MAGNITUDE || (mode >= SFCI_SCREEN_NETWORK && !net) ||
         (mode != SFCI_SCREEN_OFF && (!(keep_factor > 0.0) || !(eps_pre >= 0.0))))
         return SFCI_E_ARG;
     Ctx* c = &X->c;
     c->screen_mode = mode;
-    c->screen_net = mode == SFCI_SCREEN_NETWORK ? net : nullptr;
+    c->screen_net = mode >= SFCI_SCREEN_NETWORK ? net : nullptr;
     if (mode != SFCI_SCREEN_OFF) {
         c->screen_keep = keep_factor;
         c->screen_eps = eps_pre;

@@ -2248,7 +2248,7 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     PhaseTimer total(c);
     *diag = 0.0;
     if (n_in == 0) return 0;
-    if (c->screen_mode == 2 && !c->screen_net) throw HipError("network screening without a network", -1);
+    if (c->screen_mode >= 2 && !c->screen_net) throw HipError("network screening without a network", -1);
     const uint64_t* sk = ik;
     const double* sv = iv;
     size_t ns = n_in;
@@ -2286,8 +2286,8 @@ size_t dev_hv_screened(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     size_t kept = m;
     if (m > keep) {
         PhaseTimer t(c);
-        if (c->screen_mode == 2) {
-            mlp_score_records(c, c->screen_net, recs, m, score);
+        if (c->screen_mode >= 2) {
+            mlp_score_records(c, c->screen_net, recs, m, score, c->screen_mode == 3);
         } else {
             k_score_mag<<<grid_for(c, (m + SFCI_BLOCK - 1) / SFCI_BLOCK, 16), SFCI_BLOCK, 0, c->stream>>>(recs, m, score);
         }

@@ -374,8 +374,9 @@ constexpr int MLP_IN = 66, MLP_INP = 80, MLP_HID = 256;
 constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_HID * MLP_IN, MLP_OFF_W2 = MLP_OFF_B1 + MLP_HID,
               MLP_OFF_B2 = MLP_OFF_W2 + MLP_HID * MLP_HID, MLP_OFF_W3 = MLP_OFF_B2 + MLP_HID,
               MLP_OFF_B3 = MLP_OFF_W3 + MLP_HID, MLP_NPARAM = MLP_OFF_B3 + 1;
-// score[i] = network output for record i's features; diagonal tags get +inf.
-void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score);
+// score[i] = network output for record i's features (with_magnitude: the network's predicted
+// ln|y| of the target plus ln|v| of the record); diagonal tags get +inf.
+void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score, bool with_magnitude);
 
 // kernel timing: kt_begin before a launch, kt_end after it; kt_resolve synchronizes the
 // stream and folds the elapsed times into c->kclock.stat (and the phase timers' laps into

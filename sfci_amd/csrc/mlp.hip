@@ -78,6 +78,9 @@ struct MlpDev {
     const float* w3r;    // [HID] BF16-rounded output weights, as FP32
     const float* b3;     // [1]
     float x_shift, x_scale;   // feature 65 = (plog2|v| - x_shift) * x_scale
+    // candidate scores (records in): out_mul * output + mag_mul * plog2|v|; 1 and 0 give the
+    // network's output itself
+    float out_mul, mag_mul;
 };
 
 // Optional activations saved by the training forward (row-major [n][HID]).
@@ -226,7 +229,11 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void*
         __syncthreads();
         if (tid < FW_TM && row0 + tid < n) {
             float v = (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]) + b3;
-            if (MODE == IN_REC && (((const RecS*)in)[row0 + tid].key & 1ull)) v = INFINITY;
+            if (MODE == IN_REC) {
+                const RecS r = ((const RecS*)in)[row0 + tid];
+                v = P.out_mul * v + P.mag_mul * plog2(r.val);
+                if (r.key & 1ull) v = INFINITY;
+            }
             y[row0 + tid] = v;
         }
     }
@@ -430,6 +437,8 @@ struct sfci_mlp {
         d.b3 = (const float*)p.p + MLP_OFF_B3;
         d.x_shift = x_shift;
         d.x_scale = x_scale;
+        d.out_mul = 1.0f;
+        d.mag_mul = 0.0f;
         return d;
     }
     void derive() {
@@ -446,12 +455,17 @@ int grid1(size_t n, int per = 256, int max_blocks = 4096) {
     return (int)std::max<size_t>(1, std::min<size_t>(b, (size_t)max_blocks));
 }
 
-void fwd_launch(sfci_mlp* net, int mode, const void* in, size_t n, float* y, FwdSave sv) {
+void fwd_launch(sfci_mlp* net, int mode, const void* in, size_t n, float* y, FwdSave sv, bool with_magnitude) {
     Ctx* c = net->c;
     if (n == 0) return;
     const size_t tiles = (n + FW_TM - 1) / FW_TM;
     const int grid = (int)std::min<size_t>(tiles, 256);   // one resident workgroup per CU
-    const MlpDev P = net->dev();
+    MlpDev P = net->dev();
+    if (with_magnitude) {
+        // predicted ln|y| of the target plus ln|v| of the candidate
+        P.out_mul = 1.0f / net->y_scale;
+        P.mag_mul = 0.69314718f;
+    }
     if (mode == IN_X)
         k_mlp_fwd<IN_X><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     else if (mode == IN_REC)
@@ -524,7 +538,7 @@ void forward_backward(sfci_mlp* net, size_t n) {
     Ctx* c = net->c;
     const hipStream_t st = c->stream;
     FwdSave sv{(float*)net->z1.p, (bf16_t*)net->h1.p, (float*)net->z2.p, (bf16_t*)net->h2.p};
-    fwd_launch(net, IN_X, net->xb.p, n, (float*)net->y.p, sv);
+    fwd_launch(net, IN_X, net->xb.p, n, (float*)net->y.p, sv, false);
     k_out_grad<<<grid1(n), 256, 0, st>>>((const float*)net->y.p, (const float*)net->t.p, n, (float*)net->dy.p,
                                          (float*)net->e2.p);
     float* loss = net->loss.get<float>(2);
@@ -587,9 +601,10 @@ size_t load_x(sfci_mlp* net, const float* x, size_t n) {
 }  // namespace
 
 namespace sfci {
-void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score) {
+void mlp_score_records(Ctx* c, const sfci_mlp* net, const RecS* rec, size_t n, float* score, bool with_magnitude) {
     int tk = kt_begin(c, K_MLP_SCORE, 20.0 * n);
-    fwd_launch(const_cast<sfci_mlp*>(net), IN_REC, rec, n, score, FwdSave{nullptr, nullptr, nullptr, nullptr});
+    fwd_launch(const_cast<sfci_mlp*>(net), IN_REC, rec, n, score, FwdSave{nullptr, nullptr, nullptr, nullptr},
+               with_magnitude);
     kt_end(c, tk);
 }
 }  // namespace sfci
@@ -679,7 +694,7 @@ int sfci_mlp_forward(sfci_mlp* net, const float* x, int64_t n, float* y) {
     if (n == 0) return SFCI_OK;
     return guarded(net->c, [&] {
         const size_t np = load_x(net, x, (size_t)n);
-        fwd_launch(net, IN_X, net->xb.p, np, (float*)net->y.p, FwdSave{nullptr, nullptr, nullptr, nullptr});
+        fwd_launch(net, IN_X, net->xb.p, np, (float*)net->y.p, FwdSave{nullptr, nullptr, nullptr, nullptr}, false);
         HIPCHK(hipMemcpyAsync(y, net->y.p, n * sizeof(float), hipMemcpyDeviceToHost, net->c->stream));
         HIPCHK(hipStreamSynchronize(net->c->stream));
     });
@@ -750,7 +765,7 @@ int sfci_mlp_predict_samples(sfci_mlp* net, const sfci_screen_sample_t* samples,
         HIPCHK(hipMemcpyAsync(ds, samples, n * sizeof(sfci_screen_sample_t), hipMemcpyHostToDevice, c->stream));
         DevBuf out;
         float* dy = out.get<float>((size_t)n);
-        fwd_launch(net, IN_SAMPLE, ds, (size_t)n, dy, FwdSave{nullptr, nullptr, nullptr, nullptr});
+        fwd_launch(net, IN_SAMPLE, ds, (size_t)n, dy, FwdSave{nullptr, nullptr, nullptr, nullptr}, false);
         HIPCHK(hipMemcpyAsync(y, dy, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         out.release();

@@ -351,7 +351,8 @@ class Context:
 
     # --- candidate screening ---------------------------------------------------------
     def set_screen(self, mode: int, net: Optional["Mlp"] = None, keep_factor: float = 3.0, eps_pre: float = 1e-7):
-        """mode 0 off, 1 rank by |H_ij c_j|, 2 rank by the network (SFCI_SCREEN_*)."""
+        """mode 0 off, 1 rank by |H_ij c_j|, 2 rank by the network, 3 by the network's predicted
+        ln|y| of the target plus ln|H_ij c_j| (SFCI_SCREEN_*)."""
         self._check(lib().sfci_set_screen(self._h, int(mode), net._h if net is not None else None,
                                           float(keep_factor), float(eps_pre)))
         self._screen_net = net   # keep the network alive while the context uses it
