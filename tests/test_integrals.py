@@ -20,6 +20,15 @@ def test_converter_matches_reference_bytes(tmp_path, case):
     assert filecmp.cmp(str(out), os.path.join(GOLDEN, case + "_IN"), shallow=False)
 
 
+@pytest.mark.parametrize("case", ["u3o2e", "u4o4e"])
+def test_uhf_converter_matches_reference_bytes(tmp_path, case):
+    # the reference's conv_uhf (util/convert_fcidump.py:153-276, run with -u by
+    # tests/golden/make_converter_golden.py) on synthetic spin-dependent integrals
+    out = tmp_path / (case + "_IN")
+    I.convert_uhf(os.path.join(GOLDEN, case + "_fcidump"), str(out))
+    assert filecmp.cmp(str(out), os.path.join(GOLDEN, case + "_IN"), shallow=False)
+
+
 def test_synthetic_generator_spec():
     h, canon, ec = I.synth_integrals(12, 0)
     n = 12
