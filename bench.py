@@ -250,13 +250,13 @@ def screener(args):
 
     keeps = [float(k) for k in str(args.keep).split(",")]
 
-    def solve(tag, mode, mixed, keep=keeps[0]):
+    def solve(tag, mode, mixed, keep=keeps[0], delta_e=1e-5):
         ctx.set_precision(mixed)
         ctx.set_screen(mode, net if mode >= 2 else None, keep, args.eps)
         ctx.sync()
         t0 = time.perf_counter()
         r = ctx.ground_state(w["init_hf"], nd, nl, args.restarts or 8, hpsi_cap=args.cap, e_prev_init=float("inf"),
-                             verbose=True)
+                             verbose=True, delta_e=delta_e)
         dt = time.perf_counter() - t0
         ctx.set_screen(0)
         ctx.set_precision(False)
@@ -267,7 +267,13 @@ def screener(args):
             tag, dt, r["e_final"], r["restarts"], r["n_screened"], r["n_reverts"]))
         return out
 
-    plain = solve("plain fp64", 0, False)
+    # the plain FP64 reference runs every restart (no early stop: at a truncated Nd the Ritz
+    # value can rise before it settles); its energy is the lowest restart energy
+    plain = solve("plain fp64 (all restarts)", 0, False, delta_e=-float("inf"))
+    e_ref = min(plain["e_trunc"])
+    plain["e_reference"] = e_ref
+    hit = [t for e, t in zip(plain["e_trunc"], plain["s_restart"]) if e <= e_ref + 1e-5]
+    plain["time_to_reference_s"] = hit[0]
     # one product of a converged-solve-sized vector in each mode (diagnostics)
     vin = ctx.vector(nd)
     ctx.ground_state(w["init_hf"], nd, 4, 1, hpsi_cap=args.cap, v_out=vin, e_prev_init=float("inf"))
@@ -287,13 +293,13 @@ def screener(args):
         log("single H.v %s: %s" % (tag, single[tag]))
     del vin, out
     def annotate(x):
-        """time to E: the first restart whose (judged) energy is at or below the plain FP64
-        solve's final energy + 1e-5 Ha, against the plain solve's own wall time."""
-        x["energy_error_vs_plain_fp64"] = x["e_final"] - plain["e_final"]
-        x["within_1e-5"] = x["e_final"] <= plain["e_final"] + 1e-5
-        hit = [t for e, t in zip(x["e_trunc"], x["s_restart"]) if e <= plain["e_final"] + 1e-5]
+        """time to E: the end of the first restart whose (judged) energy is at or below the
+        plain FP64 reference energy + 1e-5 Ha, against the plain solve's own time to it."""
+        x["energy_error_vs_plain_fp64"] = x["e_final"] - e_ref
+        x["within_1e-5"] = x["e_final"] <= e_ref + 1e-5
+        hit = [t for e, t in zip(x["e_trunc"], x["s_restart"]) if e <= e_ref + 1e-5]
         x["time_to_plain_energy_s"] = hit[0] if hit else None
-        x["speedup_to_plain_energy"] = plain["seconds"] / hit[0] if hit else None
+        x["speedup_to_plain_energy"] = plain["time_to_reference_s"] / hit[0] if hit else None
 
     runs = {}
     abl = [float(k) for k in str(args.ablation_keep).split(",") if k]
@@ -502,6 +508,8 @@ def main():
     ap.add_argument("--cap", type=int, default=60_000_000, help="H.v output capacity (HPsiCap)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tte", action="store_true", help="leave out the time-to-E block of the default run")
+    ap.add_argument("--full-key-records", action="store_true",
+                    help="merge with 16-byte full-key records and 16 hash bits (no compact records)")
     ap.add_argument("--data", default=None, help="directory for the generated integral files")
     ap.add_argument("--time-to-e", action="store_true",
                     help="run the whole ground-state solve instead (GPU, and the CPU oracle with --cpu)")
@@ -561,6 +569,8 @@ def main():
     out = ctx.vector(args.cap)
     mixed = args.precision == "mixed"
     ctx.set_precision(mixed)
+    if args.full_key_records:
+        ctx.set_merge_mode(4, 16)
     rec = 12.0 if mixed else 16.0        # bytes of one candidate record (key + value)
     for i in range(args.warmup):
         ctx.hv(v, out, nd, 0.0, 1.0)

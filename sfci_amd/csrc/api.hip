@@ -305,7 +305,7 @@ int sfci_set_precision(sfci_ctx* X, int precision) {
 }
 
 int sfci_set_merge_mode(sfci_ctx* X, int mode, int hash_bits) {
-    if (!X || mode < 0 || mode > 3 || (mode >= 2 && (hash_bits < 1 || hash_bits > 16))) return SFCI_E_ARG;
+    if (!X || mode < 0 || mode > 4 || (mode >= 2 && (hash_bits < 1 || hash_bits > 16))) return SFCI_E_ARG;
     X->c.merge_mode = mode;
     X->c.merge_hash_bits = hash_bits;
     return SFCI_OK;

@@ -463,6 +463,28 @@ struct PlainSink {
         return put_rec(out, cap, idx, make_rec(key, val));
     }
 };
+// Compact records of a hash-merged pass: the record at its planned position, its hash
+// bucket in the 16-bit side array
+struct CompactOut {
+    void* rec;
+    uint16_t* bucket;
+    CHash hp;
+};
+template <typename T>
+struct CompactSink {
+    typename RecCOf<T>::type* out;
+    uint16_t* bucket;
+    uint64_t cap;
+    CHash hp;
+    __device__ __forceinline__ bool put(uint64_t idx, uint64_t key, T val, uint32_t, uint64_t) {
+        if (idx >= cap) return false;
+        const uint64_t h = ch_hash(hp, key >> 1);
+        const uint32_t rem = (uint32_t)(h & ((1ull << hp.rb) - 1));
+        rec_store(out + idx, make_crec((rem << 1) | (uint32_t)(key & 1ull), val));
+        __builtin_nontemporal_store((uint16_t)(h >> hp.rb), bucket + idx);
+        return true;
+    }
+};
 struct ScreenSink {
     ScreenOut so;
     RecS* lds;
@@ -522,12 +544,16 @@ struct ScreenSink {
         return ok;
     }
 };
-template <typename T, bool SCR>
+template <typename T, bool SCR, bool CR>
 struct SinkOf {
     using type = PlainSink<T>;
 };
+template <typename T>
+struct SinkOf<T, false, true> {
+    using type = CompactSink<T>;
+};
 template <>
-struct SinkOf<float, true> {
+struct SinkOf<float, true, false> {
     using type = ScreenSink;
 };
 
@@ -591,23 +617,29 @@ __device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, u
 // of the pass counts), so the layout is deterministic.
 // With SCR (float only) the candidates go through ScreenSink (screen.eps prefilter) instead
 // of the planned positions; it then runs as one pass over all target rows.
-template <typename T, bool SCR>
+template <typename T, bool SCR, bool CR = false>
 __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* __restrict__ sk,
                                               const double* __restrict__ sv, size_t n_src, double lambda, double coeff,
                                               uint64_t row_lo, uint64_t row_hi, const uint64_t* __restrict__ off,
                                               const uint16_t* __restrict__ sdec_g,
                                               typename RecOf<T>::type* __restrict__ out, uint64_t out_cap,
-                                              double* __restrict__ dterm, uint32_t* flags, const ScreenOut& screen) {
+                                              double* __restrict__ dterm, uint32_t* flags, const ScreenOut& screen,
+                                              const CompactOut& cout = CompactOut{}) {
     __shared__ SrcLds s;
     __shared__ RecS s_scr[SCR ? SCR_LDS : 1];
     __shared__ uint32_t s_scr_n;
     __shared__ uint64_t s_scr_base;
-    typename SinkOf<T, SCR>::type sink;
+    typename SinkOf<T, SCR, CR>::type sink;
     if constexpr (SCR) {
         sink.so = screen;
         sink.lds = s_scr;
         sink.lds_n = &s_scr_n;
         if (threadIdx.x == 0) s_scr_n = 0;
+    } else if constexpr (CR) {
+        sink.out = (typename RecCOf<T>::type*)cout.rec;
+        sink.bucket = cout.bucket;
+        sink.cap = out_cap;
+        sink.hp = cout.hp;
     } else {
         sink.out = out;
         sink.cap = out_cap;
@@ -804,6 +836,19 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                                                          uint32_t* flags) {
     generate_body<T, false>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, out, out_cap, dterm, flags,
                             ScreenOut{});
+}
+
+// the same into compact records and their hash buckets
+template <typename T>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_generate_compact(HamDev Hg, const uint64_t* __restrict__ sk,
+                                                                 const double* __restrict__ sv, size_t n_src,
+                                                                 double lambda, double coeff, uint64_t row_lo,
+                                                                 uint64_t row_hi, const uint64_t* __restrict__ off,
+                                                                 const uint16_t* __restrict__ sdec_g, CompactOut cout,
+                                                                 uint64_t out_cap, double* __restrict__ dterm,
+                                                                 uint32_t* flags) {
+    generate_body<T, false, true>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, nullptr, out_cap,
+                                  dterm, flags, ScreenOut{}, cout);
 }
 
 // all target rows, screened (mixed-precision values)
@@ -1268,7 +1313,12 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // still overflows at 256 classes sets FLAG_BUCKET_FULL (the pass then takes the full sort).
 constexpr int BR_THREADS = 1024;
 constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
-constexpr int BR_BATCH = 6;                    // records per thread loaded together
+// records per thread loaded together: as many as the 128 registers of a 1024-thread
+// workgroup hold without spilling
+template <typename T, bool CR>
+constexpr int br_batch() {
+    return (!CR && sizeof(T) == 8) ? 6 : 4;
+}
 constexpr int BR_TAGS = 512;                   // tagged targets per bucket finished from a list
 constexpr int BR_FILL = BR_SLOTS * 4 / 5;      // occupied slots at which a table counts as full
 constexpr int BR_TARGET = BR_SLOTS * 3 / 5;    // targets per class aimed at when a bucket splits
@@ -1344,9 +1394,11 @@ __device__ __attribute__((noinline)) uint32_t split_count_for(uint32_t fill_u, u
     return (uint32_t)fmin(ceil(D / (double)BR_TARGET), (double)BR_STACK);
 }
 
-template <typename T>
+// CR: compact records (the bucket index and the record key give the target's hash, hp
+// inverts it); otherwise records carry the pass-local dense target.
+template <typename T, bool CR>
 __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
-    HamDev H, const typename RecOf<T>::type* __restrict__ rec, const uint64_t* __restrict__ bstart,
+    HamDev H, const void* __restrict__ rec_in, CHash hp, const uint64_t* __restrict__ bstart,
     const uint64_t* __restrict__ bend, uint64_t n_buckets, int hb, uint64_t base_dense,
     const uint64_t* __restrict__ ik, const double* __restrict__ iv, const double* __restrict__ dterm, size_t n_in,
     double cut,
@@ -1364,10 +1416,19 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     __shared__ uint32_t s_cm[BR_STACK], s_cq[BR_STACK];   // pending classes: (count, index)
     __shared__ uint32_t s_wtot[BR_THREADS / SFCI_WAVE];
     __shared__ unsigned long long s_obase;
+    using R = typename std::conditional<CR, typename RecCOf<T>::type, typename RecOf<T>::type>::type;
+    constexpr int BRB = br_batch<T, CR>();
+    const R* __restrict__ rec = (const R*)rec_in;
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1);
     const int slot_shift = 48 - hb;             // 16 hash bits below the bucket bits
     const int cls_shift = 32 - hb;              // the class value: 16 hash bits below the slot bits
     if (tid == 0) s_bad = 0;
+    uint64_t b = blockIdx.x;                    // the bucket being merged
+    // table key -> dense target index
+    auto to_dense = [&](unsigned long long key) -> uint64_t {
+        if constexpr (CR) return base_dense + ch_unhash(hp, (b << hp.rb) | key);
+        else return base_dense + key;
+    };
     fx128 dacc = fx_zero();
     unsigned long long uniq = 0;
     bool bad_input = false, overflow = false, any_full = false;
@@ -1388,12 +1449,12 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             __syncthreads();
             if (tid == 0) s_sp = sp - 1;
             uint32_t n_ins = 0;     // records of this class this thread inserted
-            // BR_BATCH records per thread per round, all loaded before any is inserted; full
+            // BRB records per thread per round, all loaded before any is inserted; full
             // rounds load without bounds tests, so that their loads are in flight together
-            constexpr uint64_t STEP = (uint64_t)BR_BATCH * BR_THREADS;
+            constexpr uint64_t STEP = (uint64_t)BRB * BR_THREADS;
             auto load_full = [&](uint64_t i0, uint64_t* k_, T* v_) {
 #pragma unroll
-                for (int r = 0; r < BR_BATCH; ++r) {
+                for (int r = 0; r < BRB; ++r) {
                     const auto x = rec[i0 + (uint64_t)r * BR_THREADS + tid];
                     k_[r] = rec_key(x);
                     v_[r] = rec_val(x);
@@ -1401,7 +1462,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             };
             auto load_round = [&](uint64_t i0, uint64_t* k_, T* v_) {
 #pragma unroll
-                for (int r = 0; r < BR_BATCH; ++r) {
+                for (int r = 0; r < BRB; ++r) {
                     const uint64_t i = i0 + (uint64_t)r * BR_THREADS + tid;
                     k_[r] = ~0ull;
                     v_[r] = (T)0;
@@ -1414,15 +1475,27 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             };
             auto insert_round = [&](const uint64_t* k_, const T* v_) {
 #pragma unroll
-                for (int r = 0; r < BR_BATCH; ++r) {
+                for (int r = 0; r < BRB; ++r) {
                     if (k_[r] == ~0ull) continue;
                     const uint64_t raw = k_[r];
                     const unsigned long long key = raw >> 1;
-                    const uint64_t hf = target_hash_full(key);
+                    uint32_t s16, c16;   // 16 hash bits for the slot, the next 16 for the class
+                    if constexpr (CR) {
+                        // the key's rb bits mixed into 32 (a bijection mod 2^32): the top 16
+                        // bits for the slot; the low 16 for the class, distinct for every key
+                        // of a bucket that has at most 2^16 keys
+                        const uint32_t t = (uint32_t)key * 0x9E3779B1u;
+                        s16 = t >> 16;
+                        c16 = t & 0xffffu;
+                    } else {
+                        const uint64_t hf = target_hash_full(key);
+                        s16 = (uint32_t)(hf >> slot_shift) & 0xffffu;
+                        c16 = (uint32_t)(hf >> cls_shift) & 0xffffu;
+                    }
                     // class cq of cm: the cm-th part of the 16-bit class value's range
-                    if ((uint32_t)((((uint32_t)(hf >> cls_shift) & 0xffffu) * cm) >> 16) != cq) continue;
+                    if ((uint32_t)((c16 * cm) >> 16) != cq) continue;
                     ++n_ins;
-                    uint32_t slot = (uint32_t)((((hf >> slot_shift) & 0xffffull) * BR_SLOTS) >> 16);
+                    uint32_t slot = (uint32_t)((s16 * (uint32_t)BR_SLOTS) >> 16);
                     // most records meet a key already in the table (a target collects one
                     // record per connected source): a plain read finds it, and only an empty
                     // slot costs a compare-and-swap.  A slot's key never changes once set.
@@ -1467,8 +1540,8 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 }
             };
             // the first round's loads are issued before the table is cleared
-            uint64_t ka[BR_BATCH];
-            T va[BR_BATCH];
+            uint64_t ka[BRB];
+            T va[BRB];
             if (st + STEP <= en) load_full(st, ka, va);
             else load_round(st, ka, va);
             for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
@@ -1553,7 +1626,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     ++uniq;
                     if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
                         const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;
-                        dk[r] = key + base_dense;
+                        dk[r] = to_dense(key);
                         yv[r] = finish_target(H, dk[r], limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), tag, ik,
                                               iv, dterm, n_in, cut, em[r], dacc, bad_input);
                     }
@@ -1572,7 +1645,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     const fx128 acc = fx_add(limb_join(t_l0[slot], t_l1[slot], t_l2[slot]), fx_from(dterm[j]));
                     yv[FR] = fx_to(acc);
                     em[FR] = fabs(yv[FR]) > cut;
-                    dk[FR] = t_key[slot] + base_dense;
+                    dk[FR] = to_dense(t_key[slot]);
                     dacc = fx_add(dacc, fx_from(yv[FR] * iv[j]));
                     ne += em[FR] ? 1u : 0u;
                 }
@@ -1606,7 +1679,6 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             }
         }
     };
-    uint64_t b = blockIdx.x;
     uint64_t st = 0, en = 0;
     if (b < n_buckets) {
         st = bstart[b];
@@ -1838,27 +1910,12 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         }
         if (n_pass == 0) continue;
         using R = typename RecOf<T>::type;
-        // A: generation output; B: the (gapped) first-level partition, also room for the
-        // key/value split of the sort path
-        const size_t capB = part_capacity(n_pass);
-        R* A = (R*)c->cand_k[0].get<uint8_t>(n_pass * sizeof(R));
-        R* B = (R*)c->cand_k[1].get<uint8_t>(std::max(capB * sizeof(R), n_pass * (sizeof(uint64_t) + sizeof(T))));
-        {
-            PhaseTimer t(c);
-            int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
-            k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
-                                                                        row_hi, off, c->d_sdec, A, n_pass,
-                                                                        dterm, c->d_flags);
-            kt_end(c, tg);
-            HIPCHK(hipGetLastError());
-            t.lap(&st.ms_generate);
-        }
-        st.n_candidates += n_pass;
+        using RC = typename RecCOf<T>::type;
         // 3b/3c. group by target and merge exactly.  Large passes: hash buckets (two
         // partition passes) merged in LDS tables, a bucket with more targets than a table
         // holds split into hash classes inside the merge kernel; small passes, or a pass whose
-        // partition region overflows, take the full sort on the dense index and the
-        // segmented reduction.
+        // partition region or bucket classes overflow, take the full sort on the dense index
+        // and the segmented reduction.
         int hb = 0;
         if (c->merge_mode >= 2) {
             hb = c->merge_hash_bits;
@@ -1871,26 +1928,78 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS * (double)(1ull << down)) hb = down;
         }
         bool hashed = hb > 0 && hb <= 16;
+        // compact records (12 or 8 bytes) when the pass's dense range leaves at most 31 bits
+        // below the bucket bits; automatic planning raises hb (up to 16) to get there
+        const int kbits = std::max(1, bits_for((row_hi - row_lo) * H.dim));
+        if (hashed && c->merge_mode == 0 && hb > 8 && kbits - hb > 31 && kbits - 31 <= 16) hb = kbits - 31;
+        bool compact = hashed && hb > 8 && kbits - hb <= 31 && c->merge_mode != 4;
+        CHash hp{};
+        if (compact) {
+            hp.kbits = kbits;
+            hp.hb = hb;
+            hp.rb = kbits - hb;
+            hp.mask = kbits >= 64 ? ~0ull : ((1ull << kbits) - 1);
+            hp.mul = (0x9E3779B97F4A7C15ull & hp.mask) | 1ull;
+            uint64_t inv = hp.mul;   // Newton: inv <- inv (2 - mul inv), exact mod 2^64 after 6 steps
+            for (int it = 0; it < 6; ++it) inv *= 2 - hp.mul * inv;
+            hp.inv = inv & hp.mask;
+        }
+        // A: generation output; B: the (gapped) first-level partition, also room for the
+        // key/value split of the sort path; compact passes: their buckets in cand_v[0]
+        const size_t capB = part_capacity(n_pass);
+        R* A = (R*)c->cand_k[0].get<uint8_t>(n_pass * sizeof(R));
+        R* B = (R*)c->cand_k[1].get<uint8_t>(std::max(capB * sizeof(R), n_pass * (sizeof(uint64_t) + sizeof(T))));
+        auto generate = [&](bool cr) {
+            PhaseTimer t(c);
+            if (cr) {
+                uint16_t* bk = c->cand_v[0].get<uint16_t>(n_pass);
+                int tg = kt_begin(c, K_GENERATE, ((double)sizeof(RC) + 2.0) * n_pass + 16.0 * ns);
+                k_generate_compact<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(
+                    H, sk, sv, ns, lambda, coeff, row_lo, row_hi, off, c->d_sdec, CompactOut{A, bk, hp}, n_pass,
+                    dterm, c->d_flags);
+                kt_end(c, tg);
+            } else {
+                int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
+                k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
+                                                                            row_hi, off, c->d_sdec, A, n_pass,
+                                                                            dterm, c->d_flags);
+                kt_end(c, tg);
+            }
+            HIPCHK(hipGetLastError());
+            t.lap(&st.ms_generate);
+        };
+        generate(compact);
+        st.n_candidates += n_pass;
         R* rd = A;        // the records in their current order
         R* other = B;     // the free buffer
         uint64_t* bstart = nullptr;
         uint64_t* bend = nullptr;
         if (hashed) {
             PhaseTimer t(c);
-            R* g = hash_partition<R>(c, A, B, n_pass, hb);
-            if (!g) {
-                hashed = false;
-                st.n_fallback += 1;
+            const size_t nb = (size_t)1 << hb;
+            bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
+            bend = bstart + nb;
+            if (compact) {
+                if (!hash_partition_c<RC>(c, (RC*)A, c->cand_v[0].get<uint16_t>(n_pass), (RC*)B, n_pass, hb, bstart,
+                                          bend)) {
+                    // a region overflowed: the pass is generated again with full keys and sorted
+                    hashed = compact = false;
+                    st.n_fallback += 1;
+                    generate(false);
+                }
             } else {
-                rd = g;
-                other = (g == A) ? B : A;
-                const size_t nb = (size_t)1 << hb;
-                bstart = c->bucket_bounds.get<uint64_t>(2 * nb);
-                bend = bstart + nb;
-                int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
-                k_bucket_bounds<R><<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
-                    rd, n_pass, hb, nb, bstart, bend);
-                kt_end(c, tb);
+                R* g = hash_partition<R>(c, A, B, n_pass, hb);
+                if (!g) {
+                    hashed = false;
+                    st.n_fallback += 1;
+                } else {
+                    rd = g;
+                    other = (g == A) ? B : A;
+                    int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
+                    k_bucket_bounds<R><<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
+                        rd, n_pass, hb, nb, bstart, bend);
+                    kt_end(c, tb);
+                }
             }
             t.lap(&st.ms_sort);
         }
@@ -1919,12 +2028,20 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             HIPCHK(hipMemsetAsync(d_split, 0, 8, c->stream));
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
-                int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
-                k_bucket_reduce<T><<<256, BR_THREADS, 0, c->stream>>>(H, rd, bstart, bend, nb, hb, row_lo * H.dim, sk,
-                                                                      sv, dterm, ns, cut_eff, stk, stv, d_out_count,
-                                                                      stage_cap, d_uniq, c->d_acc + 2, c->d_flags,
-                                                                      d_split, c->merge_mode == 3 ? BR_FILL / 16 : BR_FILL);
-                kt_end(c, tr);
+                const uint32_t fill = c->merge_mode == 3 ? BR_FILL / 16 : BR_FILL;
+                if (compact) {
+                    int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(RC) * n_pass);
+                    k_bucket_reduce<T, true><<<256, BR_THREADS, 0, c->stream>>>(
+                        H, A, hp, bstart, bend, nb, hb, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill);
+                    kt_end(c, tr);
+                } else {
+                    int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
+                    k_bucket_reduce<T, false><<<256, BR_THREADS, 0, c->stream>>>(
+                        H, rd, hp, bstart, bend, nb, hb, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill);
+                    kt_end(c, tr);
+                }
             } else {
                 int grid = (int)std::min<size_t>((n_pass + RED_TILE - 1) / RED_TILE, 256 * 8);
                 int tr = kt_begin(c, K_REDUCE, (8.0 + sizeof(T)) * n_pass);
@@ -1948,6 +2065,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                     uint64_t restore[2] = {before, uniq_host};
                     HIPCHK(hipMemcpyAsync(d_out_count, restore, 16, hipMemcpyHostToDevice, c->stream));
                     if (fl_now & ~(uint32_t)(FLAG_BUCKET_FULL | FLAG_OUT_OVERFLOW)) atomic_flags_restore(c, fl_now);
+                    if (compact) {
+                        // compact records carry no full key: generate the pass again with them
+                        compact = false;
+                        rd = A;
+                        other = B;
+                        generate(false);
+                    }
                     hashed = false;
                     sort_path();
                     continue;

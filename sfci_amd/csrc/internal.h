@@ -96,6 +96,58 @@ __device__ __forceinline__ void rec_store(Rec64* p, const Rec64& r) {
 }
 __device__ __forceinline__ void rec_store(Rec32* p, const Rec32& r) { *p = r; }
 
+// Compact records of a hash-merged target pass.  The pass-local dense target x (kbits bits)
+// is replaced by its hash h = x * mul mod 2^kbits, a bijection (mul odd; inv is its inverse
+// mod 2^kbits).  The top hb bits of h are the record's hash bucket, kept outside the record
+// (a 16-bit side array at generation, implied by the record's position after the
+// partition), and the record's 32-bit key is (h mod 2^rb) << 1 | is_diagonal, rb = kbits -
+// hb <= 31.  A record is 12 bytes in FP64 and 8 in mixed precision.
+struct CHash {
+    int kbits, hb, rb;
+    uint64_t mul, inv, mask;
+};
+__host__ __device__ __forceinline__ uint64_t ch_hash(const CHash& c, uint64_t x) { return (x * c.mul) & c.mask; }
+__host__ __device__ __forceinline__ uint64_t ch_unhash(const CHash& c, uint64_t h) { return (h * c.inv) & c.mask; }
+struct alignas(4) RecC64 {
+    uint32_t key, vlo, vhi;
+};
+struct alignas(8) RecC32 {
+    uint32_t key;
+    float val;
+};
+template <typename T>
+struct RecCOf;
+template <>
+struct RecCOf<double> {
+    using type = RecC64;
+};
+template <>
+struct RecCOf<float> {
+    using type = RecC32;
+};
+__host__ __device__ __forceinline__ uint64_t rec_key(const RecC64& r) { return r.key; }
+__host__ __device__ __forceinline__ uint64_t rec_key(const RecC32& r) { return r.key; }
+__host__ __device__ __forceinline__ double rec_val(const RecC64& r) {
+    const uint64_t b = (uint64_t)r.vhi << 32 | r.vlo;
+    double v;
+    __builtin_memcpy(&v, &b, 8);
+    return v;
+}
+__host__ __device__ __forceinline__ float rec_val(const RecC32& r) { return r.val; }
+__host__ __device__ __forceinline__ RecC64 make_crec(uint32_t k, double v) {
+    uint64_t b;
+    __builtin_memcpy(&b, &v, 8);
+    return RecC64{k, (uint32_t)b, (uint32_t)(b >> 32)};
+}
+__host__ __device__ __forceinline__ RecC32 make_crec(uint32_t k, float v) { return RecC32{k, v}; }
+// (three 4-byte stores: a 3-element vector type would be padded to 16 bytes)
+__device__ __forceinline__ void rec_store(RecC64* p, const RecC64& r) {
+    __builtin_nontemporal_store(r.key, &p->key);
+    __builtin_nontemporal_store(r.vlo, &p->vlo);
+    __builtin_nontemporal_store(r.vhi, &p->vhi);
+}
+__device__ __forceinline__ void rec_store(RecC32* p, const RecC32& r) { *p = r; }
+
 // A diagonal (tag) record carries no value: its value field holds the bits of the source
 // index j (< 2^32), so that the merge finds the source's FP64 diagonal term and input
 // coefficient without a search.  Consumers that sum values treat tag records as zero.
@@ -282,6 +334,11 @@ void radix_partition_hash(Ctx* c, uint64_t** keys, T** vals, uint64_t* ktmp, T* 
 size_t part_capacity(size_t n);
 template <typename R>
 R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb);
+// The same grouping for compact records (8 < hb <= 16) whose hash buckets are in bkt:
+// records end in A, dense in bucket order, with bucket b's records at [bstart[b], bend[b]).
+// Returns false when a pass-1 region overflowed (A is then unchanged).
+template <typename R>
+bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb, uint64_t* bstart, uint64_t* bend);
 
 // noalloc.hip
 // With `device` current: record that device's address of the selection scratch array.

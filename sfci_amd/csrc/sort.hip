@@ -304,7 +304,7 @@ struct Chunks {
 // chunk table of regions [rstart[r], rstart[r] + rsize[r]); cbase[r] = first chunk of
 // region r (n_regions + 1 entries)
 __global__ void k_make_chunks(const uint64_t* __restrict__ rstart, const uint64_t* __restrict__ rsize,
-                              const uint64_t* __restrict__ cbase, int n_regions, size_t n_chunks,
+                              const uint64_t* __restrict__ cbase, int n_regions, size_t n_chunks, uint32_t tile,
                               uint64_t* __restrict__ start, uint32_t* __restrict__ len, uint64_t* __restrict__ hbase,
                               uint32_t* __restrict__ hstride, uint32_t* __restrict__ region) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks; c += (size_t)gridDim.x * blockDim.x) {
@@ -315,9 +315,9 @@ __global__ void k_make_chunks(const uint64_t* __restrict__ rstart, const uint64_
             else hi = mid;
         }
         const uint64_t cl = c - cbase[lo];
-        const uint64_t off = cl * PART_TILE;
+        const uint64_t off = cl * tile;
         start[c] = rstart[lo] + off;
-        len[c] = (uint32_t)min((uint64_t)PART_TILE, rsize[lo] - off);
+        len[c] = (uint32_t)min((uint64_t)tile, rsize[lo] - off);
         hbase[c] = (uint64_t)RADIX * cbase[lo] + cl;
         hstride[c] = (uint32_t)(cbase[lo + 1] - cbase[lo]);
         region[c] = (uint32_t)lo;
@@ -506,7 +506,8 @@ static void part_pass(Ctx* c, const R* rin, R* rout, const Chunks& ch, Digit dg,
 }
 
 // chunk table of the given regions in c->part_*[slot]
-static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart, const std::vector<uint64_t>& rsize) {
+static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart, const std::vector<uint64_t>& rsize,
+                          uint32_t tile = PART_TILE) {
     const int nr = (int)rstart.size();
     std::vector<uint64_t> meta(3 * nr + 1);
     uint64_t nch = 0;
@@ -514,7 +515,7 @@ static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart,
         meta[r] = rstart[r];
         meta[nr + r] = rsize[r];
         meta[2 * nr + r] = nch;
-        nch += (rsize[r] + PART_TILE - 1) / PART_TILE;
+        nch += (rsize[r] + tile - 1) / tile;
     }
     meta[3 * nr] = nch;
     uint64_t* dmeta = c->part_meta[slot].get<uint64_t>(meta.size());
@@ -527,7 +528,8 @@ static Chunks make_chunks(Ctx* c, int slot, const std::vector<uint64_t>& rstart,
     uint32_t* rg = c->part_region[slot].get<uint32_t>(n_ch);
     if (nch) {
         unsigned g = (unsigned)std::min<uint64_t>((nch + SFCI_BLOCK - 1) / SFCI_BLOCK, 4096);
-        k_make_chunks<<<g, SFCI_BLOCK, 0, c->stream>>>(dmeta, dmeta + nr, dmeta + 2 * nr, nr, nch, st, ln, hb, hs, rg);
+        k_make_chunks<<<g, SFCI_BLOCK, 0, c->stream>>>(dmeta, dmeta + nr, dmeta + 2 * nr, nr, nch, tile, st, ln, hb,
+                                                       hs, rg);
     }
     HIPCHK(hipStreamSynchronize(c->stream));   // meta lives on the host stack
     return Chunks{st, ln, hb, hs, rg, (size_t)nch};
@@ -733,6 +735,232 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
     return A;
 }
 
+// ---- hash partition of compact records ----------------------------------------------
+// The same two passes for compact records, whose hash bucket (hb bits, 8 < hb <= 16) is
+// not in the key: pass 1 takes its digit (the bucket's top 8 bits) from the 16-bit bucket
+// array written by the generation and writes the bucket's low hb - 8 bits as the digit byte
+// of pass 2, which reads both its counts and its scatter digit from those bytes.  The
+// bucket bounds come from pass 2's per-(region, digit) output starts.
+constexpr int PART_TILE_C = PART_TILE;     // (4096-record tiles, two workgroups per CU, measured slower)
+constexpr int PART_THREADS_C = PART_THREADS;
+static_assert(PART_TILE_C / PART_THREADS_C == PART_ITEMS, "compact tiles keep the items per thread");
+
+template <typename R>
+__global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* __restrict__ rin,
+                                                                      const uint16_t* __restrict__ bkt, size_t n,
+                                                                      R* __restrict__ rout, int sh1,
+                                                                      unsigned long long* __restrict__ cursor,
+                                                                      const uint64_t* __restrict__ rend,
+                                                                      uint32_t* __restrict__ flags,
+                                                                      uint8_t* __restrict__ dout) {
+    __shared__ R s_rec[PART_TILE_C];
+    __shared__ uint16_t s_bk[PART_TILE_C];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX], s_ok[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * PART_TILE_C;
+    const uint32_t len = (uint32_t)min((size_t)PART_TILE_C, n - base);
+    const uint32_t m2 = (1u << sh1) - 1u;
+    constexpr int NW = sizeof(R) / 4;   // records held as 32-bit words (kept in registers)
+    uint32_t x[PART_ITEMS][NW];
+    uint16_t bk[PART_ITEMS];
+    uint32_t rank[PART_ITEMS];
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        if (li < len) {
+            const uint32_t* src = (const uint32_t*)(rin + base + li);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) x[r][q] = src[q];
+            bk[r] = bkt[base + li];
+        }
+    }
+    if (tid < RADIX) s_cnt[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        rank[r] = 0;
+        if (li < len) rank[r] = atomicAdd(&s_cnt[(uint32_t)bk[r] >> sh1], 1u);
+    }
+    __syncthreads();
+    const uint32_t h = tid < RADIX ? s_cnt[tid] : 0u;
+    const uint32_t ls = digit_scan(h, s_wsum);
+    if (tid < RADIX) {
+        s_lstart[tid] = ls;
+        uint32_t ok = 1;
+        if (h) {
+            const unsigned long long g = atomicAdd(&cursor[tid], (unsigned long long)h);
+            s_gbase[tid] = g;
+            if (g + h > rend[tid]) {
+                ok = 0;
+                atomicOr(flags, FLAG_REGION_FULL);
+            }
+        }
+        s_ok[tid] = ok;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        if (li < len) {
+            const uint32_t p = s_lstart[(uint32_t)bk[r] >> sh1] + rank[r];
+            uint32_t* dst = (uint32_t*)(s_rec + p);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) dst[q] = x[r][q];
+            s_bk[p] = bk[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += PART_THREADS_C) {
+        const uint32_t b = s_bk[p], d = b >> sh1;
+        if (s_ok[d]) {
+            const uint64_t at = s_gbase[d] + (p - s_lstart[d]);
+            rout[at] = s_rec[p];
+            dout[at] = (uint8_t)(b & m2);
+        }
+    }
+}
+
+// second level: a chunk reserves one run per digit (from the digit bytes) on its region's
+// (digit) cursor
+template <typename R>
+__global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cm_c(const R* __restrict__ rin,
+                                                                     const uint8_t* __restrict__ din,
+                                                                     R* __restrict__ rout, Chunks ch,
+                                                                     const uint32_t* __restrict__ cnt,
+                                                                     unsigned long long* __restrict__ cursor) {
+    __shared__ R s_rec[PART_TILE_C];
+    __shared__ uint8_t s_dig[PART_TILE_C];
+    __shared__ uint64_t s_gbase[RADIX];
+    __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
+    const int tid = threadIdx.x;
+    const size_t c = blockIdx.x;
+    const uint64_t st = ch.start[c];
+    const uint32_t len = ch.len[c];
+    const uint32_t rg = ch.region[c];
+    constexpr int NW = sizeof(R) / 4;   // records held as 32-bit words (kept in registers)
+    uint32_t x[PART_ITEMS][NW];
+    uint32_t dg[PART_ITEMS];
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        if (li < len) {
+            const uint32_t* src = (const uint32_t*)(rin + st + li);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) x[r][q] = src[q];
+            dg[r] = din[st + li];
+        }
+    }
+    const uint32_t h = tid < RADIX ? cnt[c * RADIX + tid] : 0u;
+    const uint32_t ls = digit_scan(h, s_wsum);
+    if (tid < RADIX) {
+        s_lstart[tid] = ls;
+        s_gbase[tid] = h ? atomicAdd(&cursor[(uint64_t)rg * RADIX + tid], (unsigned long long)h) : 0ull;
+        s_cnt[tid] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        if (li < len) {
+            const uint32_t p = s_lstart[dg[r]] + atomicAdd(&s_cnt[dg[r]], 1u);
+            uint32_t* dst = (uint32_t*)(s_rec + p);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) dst[q] = x[r][q];
+            s_dig[p] = (uint8_t)dg[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += PART_THREADS_C) {
+        const uint32_t d = s_dig[p];
+        rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
+    }
+}
+
+// bucket b = (region r, digit d) with r = b >> sh1, d = b & (2^sh1 - 1): its records start
+// at the saved pass-2 output start of (r, d) and end where bucket b + 1 starts
+__global__ void k_bucket_bounds_c(const unsigned long long* __restrict__ start_rd, int sh1, uint64_t n_buckets,
+                                  size_t n, uint64_t* __restrict__ bstart, uint64_t* __restrict__ bend) {
+    const uint64_t m2 = ((uint64_t)1 << sh1) - 1;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_buckets;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        bstart[b] = start_rd[(b >> sh1) * RADIX + (b & m2)];
+        const uint64_t b1 = b + 1;
+        bend[b] = b1 < n_buckets ? start_rd[(b1 >> sh1) * RADIX + (b1 & m2)] : n;
+    }
+}
+
+template <typename R>
+bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb, uint64_t* bstart,
+                      uint64_t* bend) {
+    if (hb <= 8 || hb > 16) throw HipError("hash_partition_c: 9..16 hash bits", -1);
+    const int sh1 = hb - 8;
+    // pass 1: the bucket's top 8 bits, fixed-capacity regions of B
+    const size_t cap1 = part_capacity(n) / RADIX;
+    std::vector<uint64_t> cur(2 * RADIX);
+    for (int d = 0; d < RADIX; ++d) {
+        cur[d] = (uint64_t)d * cap1;
+        cur[RADIX + d] = (uint64_t)(d + 1) * cap1;
+    }
+    uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX);
+    HIPCHK(hipMemcpyAsync(dcur, cur.data(), 2 * RADIX * 8, hipMemcpyHostToDevice, c->stream));
+    const size_t n_tiles = (n + PART_TILE_C - 1) / PART_TILE_C;
+    uint8_t* dig = c->part_dig.get<uint8_t>(part_capacity(n));
+    int t1 = kt_begin(c, K_PART_SCATTER1, (2.0 * sizeof(R) + 3.0) * n);
+    k_part_scatter_cur_c<R><<<(unsigned)n_tiles, PART_THREADS_C, 0, c->stream>>>(
+        A, bkt, n, B, sh1, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig);
+    kt_end(c, t1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cur.data(), dcur, RADIX * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint32_t fl = flags_read_clear(c);
+    if (fl & ~(uint32_t)FLAG_REGION_FULL) atomic_flags_restore(c, fl & ~(uint32_t)FLAG_REGION_FULL);
+    if (fl & FLAG_REGION_FULL) return false;
+    // pass 2: each region by the bucket's low bits (from the digit bytes), dense into A
+    std::vector<uint64_t> rstart(RADIX), rsize(RADIX);
+    for (int d = 0; d < RADIX; ++d) {
+        rstart[d] = (uint64_t)d * cap1;
+        rsize[d] = cur[d] - rstart[d];
+    }
+    const int nr = RADIX;
+    Chunks ch = make_chunks(c, 0, rstart, rsize, PART_TILE_C);
+    const uint64_t* cbase = (const uint64_t*)c->part_meta[0].p + 2 * nr;
+    std::vector<uint64_t> ds(nr);
+    uint64_t acc = 0;
+    for (int r = 0; r < nr; ++r) {
+        ds[r] = acc;
+        acc += rsize[r];
+    }
+    uint64_t* dds = c->part_meta[1].get<uint64_t>(nr);
+    HIPCHK(hipMemcpyAsync(dds, ds.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
+    uint32_t* cnt = c->scratch_hist.get<uint32_t>((size_t)RADIX * std::max<size_t>(ch.n, 1));
+    unsigned long long* cursor = (unsigned long long*)c->part_base.get<uint64_t>((size_t)2 * RADIX * nr);
+    unsigned long long* start_rd = cursor + (size_t)RADIX * nr;
+    int t0 = kt_begin(c, K_PART_HIST, 1.0 * n + 4.0 * RADIX * ch.n);
+    k_part_hist_bytes<<<(unsigned)((ch.n + HIST_CHUNKS - 1) / HIST_CHUNKS), SFCI_BLOCK, 0, c->stream>>>(dig, ch, cnt);
+    kt_end(c, t0);
+    int t2 = kt_begin(c, K_SCAN, 4.0 * RADIX * ch.n);
+    k_region_colscan<<<nr, RADIX, 0, c->stream>>>(cnt, cbase, dds, cursor);
+    kt_end(c, t2);
+    HIPCHK(hipMemcpyAsync(start_rd, cursor, (size_t)RADIX * nr * 8, hipMemcpyDeviceToDevice, c->stream));
+    int t3 = kt_begin(c, K_PART_SCATTER, (2.0 * sizeof(R) + 1.0) * n);
+    k_part_scatter_cm_c<R><<<(unsigned)ch.n, PART_THREADS_C, 0, c->stream>>>(B, dig, A, ch, cnt, cursor);
+    kt_end(c, t3);
+    const uint64_t nb = (uint64_t)1 << hb;
+    int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
+    k_bucket_bounds_c<<<(unsigned)((nb + SFCI_BLOCK - 1) / SFCI_BLOCK), SFCI_BLOCK, 0, c->stream>>>(
+        start_rd, sh1, nb, n, bstart, bend);
+    kt_end(c, tb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));   // ds lives on the host stack
+    return true;
+}
+
+template bool hash_partition_c<RecC64>(Ctx*, RecC64*, const uint16_t*, RecC64*, size_t, int, uint64_t*, uint64_t*);
+template bool hash_partition_c<RecC32>(Ctx*, RecC32*, const uint16_t*, RecC32*, size_t, int, uint64_t*, uint64_t*);
 template Rec64* hash_partition<Rec64>(Ctx*, Rec64*, Rec64*, size_t, int);
 template Rec32* hash_partition<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 

@@ -87,7 +87,8 @@ def c2(wdir):
 
 
 @pytest.mark.parametrize("case", ["single_pass", "multipass_auto", "multipass_12bits", "multipass_16bits",
-                                  "mixed_multipass", "nd_shift", "split_9bits", "nested_split_12bits"])
+                                  "mixed_multipass", "nd_shift", "split_9bits", "nested_split_12bits",
+                                  "full_key_records_16bits"])
 def test_c2_sample_hv_bit_exact(c2, case):
     w, ctx, orc, k, v = c2
     nd, lam, coeff, mode = len(k), 0.0, 1.0, 0
@@ -108,6 +109,8 @@ def test_c2_sample_hv_bit_exact(c2, case):
         ctx.set_merge_mode(2, 9)
     elif case == "nested_split_12bits":
         ctx.set_merge_mode(3, 12)             # tables counted full at 1/16: classes split again
+    elif case == "full_key_records_16bits":
+        ctx.set_merge_mode(4, 16)             # 16-byte records with the dense target (no compaction)
     log("c2 hv case %s" % case)
     try:
         gk, gv, gd = gpu_hv(ctx, k, v, nd, lam, coeff, 60_000_000)
@@ -131,7 +134,7 @@ def test_c2_sample_hv_bit_exact(c2, case):
         assert st["n_split"] > 4096 and st["hash_bits"] == 12
     if case == "multipass_12bits":
         assert st["hash_bits"] == 12
-    if case == "multipass_16bits":
+    if case in ("multipass_16bits", "full_key_records_16bits"):
         assert st["hash_bits"] == 16
 
 

@@ -155,11 +155,12 @@ def test_hv_multipass_matches(sys1, orc1, ctx1):
     assert bits([gd])[0] == bits([od])[0]
 
 
-@pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14), (3, 4)])
+@pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14), (2, 10), (4, 10), (3, 4), (3, 10)])
 def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
-    # full sort, hash buckets, hash buckets too few to hold the targets (each bucket is
-    # split into hash classes merged one at a time), and tables counted full at 1/16 of
-    # their occupancy (classes split again and again) give the same bits
+    # full sort, hash buckets (compact records above 8 bits, full-key records in mode 4),
+    # hash buckets too few to hold the targets (each bucket is split into hash classes
+    # merged one at a time), and tables counted full at 1/16 of their occupancy (classes
+    # split again and again) give the same bits
     k, v = random_input(sys1, 3000, 31)
     ctx1.set_merge_mode(mode, hbits)
     try:
@@ -172,7 +173,7 @@ def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
     assert st["n_fallback"] == 0
     if (mode == 2 and hbits == 1) or mode == 3:
         assert st["n_split"] >= 2
-    if mode == 3:
+    if mode == 3 and hbits == 4:
         # nested: more splits than buckets
         assert st["n_split"] > (1 << hbits)
     ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=0.05, coeff=1.0, cut=1e-5)
