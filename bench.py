@@ -30,6 +30,19 @@ import numpy as np  # noqa: E402
 METRIC = "H\u00b7v connections/s + time-to-E within 1e-5 Ha, NN/FP32 vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 
+def _plain(o):
+    """numpy scalars and arrays as JSON values"""
+    if isinstance(o, np.generic):
+        return o.item()
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    raise TypeError("not JSON serializable: %r" % type(o))
+
+
+def jdump(res):
+    return json.dumps(res, default=_plain)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -115,7 +128,7 @@ def time_to_e(ctx, w, nd, args):
                                    restarts=o["restarts"], sample="the same solve with the oracle")
         res["energy_error_vs_oracle"] = r["e_final"] - o["e_final"]
         res["cpu_over_gpu"] = tc / dt
-    print(json.dumps(res))
+    print(jdump(res))
 
 
 def train_network(ctx, name, data, seeds, nd_tr, nl_tr, restarts_tr, eps, samples_per_seed, steps, batch, lr, cap):
@@ -270,7 +283,7 @@ def screener(args):
     # the plain FP64 reference runs every restart (no early stop: at a truncated Nd the Ritz
     # value can rise before it settles); its energy is the lowest restart energy
     plain = solve("plain fp64 (all restarts)", 0, False, delta_e=-float("inf"))
-    e_ref = min(plain["e_trunc"])
+    e_ref = float(min(plain["e_trunc"]))
     plain["e_reference"] = e_ref
     hit = [t for e, t in zip(plain["e_trunc"], plain["s_restart"]) if e <= e_ref + 1e-5]
     plain["time_to_reference_s"] = hit[0]
@@ -296,7 +309,7 @@ def screener(args):
         """time to E: the end of the first restart whose (judged) energy is at or below the
         plain FP64 reference energy + 1e-5 Ha, against the plain solve's own time to it."""
         x["energy_error_vs_plain_fp64"] = x["e_final"] - e_ref
-        x["within_1e-5"] = x["e_final"] <= e_ref + 1e-5
+        x["within_1e-5"] = bool(x["e_final"] <= e_ref + 1e-5)
         hit = [t for e, t in zip(x["e_trunc"], x["s_restart"]) if e <= e_ref + 1e-5]
         x["time_to_plain_energy_s"] = hit[0] if hit else None
         x["speedup_to_plain_energy"] = plain["time_to_reference_s"] / hit[0] if hit else None
@@ -329,7 +342,7 @@ def screener(args):
                by_keep_factor={str(k): v for k, v in runs.items()}, prefilter_only=pre,
                single_hv=single,
                training=dict(seconds=t_train, data_seconds=t_data, ms_per_step=ks["ms"] / max(1, ks["launches"])))
-    print(json.dumps(res))
+    print(jdump(res))
     ctx.close()
 
 
@@ -347,7 +360,7 @@ def copy_probe():
     ctx._check(gpu.lib().sfci_vec_set_len(a._h, n))
     b.copy_from(a)
     ctx.sync()
-    print(json.dumps(dict(copy_bytes_read=16 * n, copy_bytes_written=16 * n)))
+    print(jdump(dict(copy_bytes_read=16 * n, copy_bytes_written=16 * n)))
     ctx.close()
 
 
@@ -395,7 +408,7 @@ def screen_hv(args):
                                   frac_of_bf16_peak=flops * st["n_candidates"] / (per * 1e-3) / 2.5e15)
         res[tag] = r
         log("%s: %s" % (tag, r))
-    print(json.dumps(dict(metric="screened H.v (learned-screener workload)", unit="ms", config=dict(
+    print(jdump(dict(metric="screened H.v (learned-screener workload)", unit="ms", config=dict(
         workload="%d spatial / %d spin orbitals, %d electrons, seed %d, Nd=%d" % (w["n_orb"], w["n_so"], w["n_elec"],
                                                                                w["seed"], nd),
         eps_pre=args.eps, keep_factor=args.keep), modes=res)))
@@ -444,7 +457,7 @@ def wide_hv(args):
         if mixed:
             res["energy_error_mixed_vs_fp64"] = res["mixed"]["e_final"] - res["fp64"]["e_final"]
             res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
-        print(json.dumps(res))
+        print(jdump(res))
         ctx.close()
         return
     lo, hi = gpu.split_keys([hf])
@@ -472,7 +485,8 @@ def wide_hv(args):
     n_in = v.len
     conn = n_in * n_conn
     avg = {k: sum(x[k] for x in sts) / len(sts) for k in ("ms_generate", "ms_sort", "ms_reduce", "ms_select",
-                                                          "ms_total", "n_passes", "n_unique", "n_out")}
+                                                          "ms_total", "n_passes", "n_unique", "n_out", "n_split",
+                                                          "n_fallback")}
     res = dict(metric="H.v connections/s, 128-bit keys", value=conn / dt, unit="connections/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=dt * 1e3, higher_is_better=True,
                dtype="f32/f64" if mixed else "f64", data="synthetic",
@@ -480,7 +494,7 @@ def wide_hv(args):
                            "cap=%d" % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, args.cap),
                            n_in=n_in, n_conn=n_conn, candidates=conn),
                phases_ms=avg,
-               generate_write_gbs=24.0 * conn / (avg["ms_generate"] * 1e-3) / 1e9 if avg["ms_generate"] else None)
+               generate_write_gbs=26.0 * conn / (avg["ms_generate"] * 1e-3) / 1e9 if avg["ms_generate"] else None)
     if not args.no_cpu:
         from oracle import oracle as O
         orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
@@ -494,7 +508,7 @@ def wide_hv(args):
                                    sample="first %d of the %d input determinants (key order), oracle H.v on "
                                           "128-bit keys with %d OpenMP threads: %.2f s" % (m, n_in, threads, tc))
         res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
-    print(json.dumps(res))
+    print(jdump(res))
     ctx.close()
 
 
@@ -643,7 +657,7 @@ def main():
     ctx.close()
     if not args.no_tte and args.nd is None and args.workload == "c2" and not mixed:
         res["time_to_e"] = time_to_e_block(args)
-    print(json.dumps(res))
+    print(jdump(res))
 
 
 if __name__ == "__main__":

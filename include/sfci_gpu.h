@@ -364,20 +364,23 @@ int sfci_screen_samples(sfci_ctx* ctx, const sfci_vec* v, double eps_pre, int64_
 typedef struct sfci_wctx sfci_wctx;
 typedef struct sfci_wvec sfci_wvec;
 
-/* n_so even <= 128, n_elec even <= 64, n_elec / 2 <= n_so / 2. */
+/* n_so even <= 128, n_elec even < 64, n_elec / 2 <= n_so / 2. */
 int sfci_wide_create(const char* in_path, int n_so, int n_elec, int device, sfci_wctx** out);
 void sfci_wide_destroy(sfci_wctx* ctx);
 const char* sfci_wide_last_error(const sfci_wctx* ctx);
 int sfci_wide_info(const sfci_wctx* ctx, sfci_info_t* info);
 int sfci_wide_set_cut(sfci_wctx* ctx, double cut);
 int sfci_wide_set_precision(sfci_wctx* ctx, int precision);
-/* Device bytes the candidate buffers of one pass may use (104 bytes per candidate). */
+/* Device bytes the candidate buffers of one pass may use (about 52 bytes per candidate
+ * plus a fixed 420 MB of partition slack). */
 int sfci_wide_set_workspace(sfci_wctx* ctx, uint64_t bytes);
 int sfci_wide_hv_stats(const sfci_wctx* ctx, sfci_hv_stats_t* st);
-/* How many bits of the key hash group candidate records before the exact merge (8..64,
- * default 48; a tuning and test knob: fewer bits make distinct keys share groups, which
- * are then told apart by their full keys).  Results are identical for every value. */
-int sfci_wide_set_group_bits(sfci_wctx* ctx, int bits);
+/* Tuning and test knobs of the 128-bit merge; results are identical for every value.
+ * hash_bits: bits of the key hash that bucket candidate records (9..16; 0 = from the pass
+ * size, the default).  table_fill: occupied slots (1..2867) at which a bucket's LDS table
+ * counts as full and the bucket is merged in hash classes (0 = 2867, the default; small
+ * values force class splits). */
+int sfci_wide_set_merge(sfci_wctx* ctx, int hash_bits, int table_fill);
 int sfci_wide_vec_create(sfci_wctx* ctx, int64_t cap, sfci_wvec** out);
 void sfci_wide_vec_destroy(sfci_wvec* v);
 int64_t sfci_wide_vec_len(const sfci_wvec* v);

@@ -707,6 +707,12 @@ int sfci_add_three(sfci_ctx* X, sfci_vec* out, const sfci_vec* x, const sfci_vec
 int sfci_wide_create(const char* in_path, int n_so, int n_elec, int device, sfci_wctx** out) {
     if (!in_path || !out) return SFCI_E_ARG;
     *out = nullptr;
+    if (n_elec >= 64) {
+        // the merge marks empty LDS-table slots with an all-ones key word, which a
+        // determinant with fewer than 64 electrons never has
+        fprintf(stderr, "sfci_wide_create: n_elec must be below 64\n");
+        return SFCI_E_ARG;
+    }
     std::vector<Entry> ent;
     int rc = read_in_file(in_path, ent);
     if (rc) return rc;
@@ -739,9 +745,11 @@ int sfci_wide_set_precision(sfci_wctx* W, int precision) {
 
 int sfci_wide_set_workspace(sfci_wctx* W, uint64_t bytes) { return W ? sfci_set_workspace(W->x, bytes) : SFCI_E_ARG; }
 
-int sfci_wide_set_group_bits(sfci_wctx* W, int bits) {
-    if (!W || bits < 8 || bits > 64) return SFCI_E_ARG;
-    W->x->c.wide_group_bits = bits;
+int sfci_wide_set_merge(sfci_wctx* W, int hash_bits, int table_fill) {
+    if (!W || !(hash_bits == 0 || (hash_bits >= 9 && hash_bits <= 16)) || table_fill < 0 || table_fill > 2867)
+        return SFCI_E_ARG;
+    W->x->c.wide_merge_bits = hash_bits;
+    W->x->c.wide_fill = table_fill;
     return SFCI_OK;
 }
 

@@ -1351,49 +1351,6 @@ __global__ void k_bucket_bounds(const R* __restrict__ ck, size_t n, int hb, uint
     }
 }
 
-// Exact accumulation without a carry chain: a term x (|x| < 2^42) is split into three
-// signed limbs with x * 2^84 = l2 * 2^84 + l1 * 2^42 + l0 truncated toward zero (the same
-// integer fx_from gives), using exact FP64 steps: l2 = trunc(x), l1 = trunc(frac * 2^42),
-// l0 = trunc(rest * 2^42).  Every limb has the sign of x and |l1|, |l0| < 2^42.  Each limb
-// is added to its own 64-bit LDS word with an atomic whose result is unused.  A target
-// receives fewer than 2^21 terms (at most one per source), so no limb sum overflows; the
-// limbs are recombined exactly when the target is finished.
-__device__ __forceinline__ void limb_split(double x, unsigned long long& l0, unsigned long long& l1,
-                                           unsigned long long& l2) {
-    if (!(fabs(x) < 4398046511104.0)) x = 0.0;   // out of range: flagged where the term was made
-    const double t2 = trunc(x);
-    const double r1 = (x - t2) * 0x1p42;
-    const double t1 = trunc(r1);
-    const double t0 = trunc((r1 - t1) * 0x1p42);
-    l2 = (unsigned long long)(long long)t2;
-    l1 = (unsigned long long)(long long)t1;
-    l0 = (unsigned long long)(long long)t0;
-}
-
-__device__ __forceinline__ fx128 limb_join(unsigned long long l0, unsigned long long l1, unsigned long long l2) {
-    fx128 t{l0, (long long)l0 < 0 ? ~0ull : 0ull};
-    t = fx_add(t, fx128{l1 << 42, (unsigned long long)((long long)l1 >> 22)});
-    return fx_add(t, fx128{0, l2 << 20});
-}
-
-// Subclasses for a class whose table filled: its targets D estimated from the fill after
-// n_read of its records (records drawn over D keys: fill = D (1 - exp(-n_read / D)), solved
-// by bisection), at BR_TARGET targets per subclass.  Kept out of line: it runs once per split.
-__device__ __attribute__((noinline)) uint32_t split_count_for(uint32_t fill_u, uint32_t nread_u) {
-    const double fill = (double)fill_u, nread = (double)nread_u;
-    double lo = fill, hi = fmax(fill, 64.0 * nread);
-    double D = hi;
-    if (nread > fill + 0.5) {
-        for (int it = 0; it < 60; ++it) {
-            const double mid = 0.5 * (lo + hi);
-            if (mid * (1.0 - exp(-nread / mid)) < fill) lo = mid;
-            else hi = mid;
-        }
-        D = hi;
-    }
-    return (uint32_t)fmin(ceil(D / (double)BR_TARGET), (double)BR_STACK);
-}
-
 // CR: compact records (the bucket index and the record key give the target's hash, hp
 // inverts it); otherwise records carry the pass-local dense target.
 template <typename T, bool CR>
@@ -1477,6 +1434,8 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
 #pragma unroll
                 for (int r = 0; r < BRB; ++r) {
                     if (k_[r] == ~0ull) continue;
+                    // a full table's class is merged again as subclasses: stop inserting
+                    if (__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                     const uint64_t raw = k_[r];
                     const unsigned long long key = raw >> 1;
                     uint32_t s16, c16;   // 16 hash bits for the slot, the next 16 for the class
@@ -1501,6 +1460,11 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     // slot costs a compare-and-swap.  A slot's key never changes once set.
                     int probe = 0;
                     for (; probe < BR_SLOTS; ++probe) {
+                        if ((probe & 63) == 63 &&
+                            __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                            probe = BR_SLOTS;   // the class is being abandoned: give up the search
+                            break;
+                        }
                         const unsigned long long cur =
                             __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (cur == key) break;
@@ -1586,7 +1550,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 if (lane == 0 && r) atomicAdd(&s_nread, r);
                 __syncthreads();
                 if (tid == 0) {
-                    uint32_t f = split_count_for(s_fill, s_nread);
+                    uint32_t f = split_count_for(s_fill, s_nread, BR_TARGET, BR_STACK);
                     if (f < 2) f = 2;
                     if (s_sp + (int)f > BR_STACK) f = (uint32_t)(BR_STACK - s_sp);
                     if (f < 2 || cm * f > BR_MAX_CLASSES) {

@@ -148,6 +148,29 @@ __device__ __forceinline__ void rec_store(RecC64* p, const RecC64& r) {
 }
 __device__ __forceinline__ void rec_store(RecC32* p, const RecC32& r) { *p = r; }
 
+// Candidate record of the 128-bit-key H.v: the target key (lo = spin orbitals 0-63, hi =
+// 64-127) and its value; a diagonal record's value is a negative quiet NaN with bit 50 set
+// whose low 50 bits are the source index (its FP64 diagonal term is kept per source).  No
+// arithmetic result has that pattern (the default NaN is positive), and a product that is
+// NaN or out of range is flagged where it is made.
+struct alignas(8) RecW {
+    uint64_t lo, hi;
+    double val;
+};
+constexpr uint64_t WTAG_BITS = 0xfffc000000000000ull;
+__host__ __device__ __forceinline__ double wtag_val(uint64_t j) {
+    const uint64_t b = WTAG_BITS | j;
+    double v;
+    __builtin_memcpy(&v, &b, 8);
+    return v;
+}
+// the source index of a diagonal record, or ~0 for an ordinary one
+__host__ __device__ __forceinline__ uint64_t wtag_index(double v) {
+    uint64_t b;
+    __builtin_memcpy(&b, &v, 8);
+    return (b & WTAG_BITS) == WTAG_BITS ? (b & 0x0003ffffffffffffull) : ~0ull;
+}
+
 // A diagonal (tag) record carries no value: its value field holds the bits of the source
 // index j (< 2^32), so that the merge finds the source's FP64 diagonal term and input
 // coefficient without a search.  Consumers that sum values treat tag records as zero.
@@ -289,7 +312,8 @@ struct Ctx {
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
     int merge_hash_bits = 0;         // hash bits of merge_mode 2
-    int wide_group_bits = 48;        // 128-bit-key H.v: bits of the key hash that group records
+    int wide_merge_bits = 0;         // 128-bit-key H.v: hash bits that bucket records (0: from the pass size)
+    int wide_fill = 0;               // 128-bit-key H.v: occupied slots at which a merge table splits (0: default)
     // candidate screening of sfci_hv (0 off, 1 by |value|, 2 by the network)
     int screen_mode = 0;
     const sfci_mlp* screen_net = nullptr;
@@ -462,6 +486,51 @@ fx128 acc_read_raw(Ctx* c, int slot);
 double acc_read(Ctx* c, int slot);
 uint32_t flags_read_clear(Ctx* c);
 void atomic_flags_restore(Ctx* c, uint32_t f);   // put back flags read by flags_read_clear
+
+// Exact accumulation without a carry chain: a term x (|x| < 2^42) is split into three
+// signed limbs with x * 2^84 = l2 * 2^84 + l1 * 2^42 + l0 truncated toward zero (the same
+// integer fx_from gives), using exact FP64 steps: l2 = trunc(x), l1 = trunc(frac * 2^42),
+// l0 = trunc(rest * 2^42).  Every limb has the sign of x and |l1|, |l0| < 2^42.  Each limb
+// is added to its own 64-bit LDS word with an atomic whose result is unused.  A target
+// receives fewer than 2^21 terms (at most one per source), so no limb sum overflows; the
+// limbs are recombined exactly when the target is finished.
+__device__ __forceinline__ void limb_split(double x, unsigned long long& l0, unsigned long long& l1,
+                                           unsigned long long& l2) {
+    if (!(fabs(x) < 4398046511104.0)) x = 0.0;   // out of range: flagged where the term was made
+    const double t2 = trunc(x);
+    const double r1 = (x - t2) * 0x1p42;
+    const double t1 = trunc(r1);
+    const double t0 = trunc((r1 - t1) * 0x1p42);
+    l2 = (unsigned long long)(long long)t2;
+    l1 = (unsigned long long)(long long)t1;
+    l0 = (unsigned long long)(long long)t0;
+}
+
+__device__ __forceinline__ fx128 limb_join(unsigned long long l0, unsigned long long l1, unsigned long long l2) {
+    fx128 t{l0, (long long)l0 < 0 ? ~0ull : 0ull};
+    t = fx_add(t, fx128{l1 << 42, (unsigned long long)((long long)l1 >> 22)});
+    return fx_add(t, fx128{0, l2 << 20});
+}
+
+// Subclasses for a class whose table filled: its targets D estimated from the fill after
+// n_read of its records (records drawn over D keys: fill = D (1 - exp(-n_read / D)), solved
+// by bisection), at `target` targets per subclass, at most `stack`.  Kept out of line: it
+// runs once per split.
+static __device__ __attribute__((noinline)) uint32_t split_count_for(uint32_t fill_u, uint32_t nread_u, int target,
+                                                                     int stack) {
+    const double fill = (double)fill_u, nread = (double)nread_u;
+    double lo = fill, hi = fmax(fill, 64.0 * nread);
+    double D = hi;
+    if (nread > fill + 0.5) {
+        for (int it = 0; it < 60; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (mid * (1.0 - exp(-nread / mid)) < fill) lo = mid;
+            else hi = mid;
+        }
+        D = hi;
+    }
+    return (uint32_t)fmin(ceil(D / (double)target), (double)stack);
+}
 
 static inline int bits_for(uint64_t max_value_exclusive) {
     // smallest b with 2^b >= max_value_exclusive

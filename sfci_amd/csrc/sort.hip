@@ -741,26 +741,33 @@ R* hash_partition(Ctx* c, R* A, R* B, size_t n, int hb) {
 // array written by the generation and writes the bucket's low hb - 8 bits as the digit byte
 // of pass 2, which reads both its counts and its scatter digit from those bytes.  The
 // bucket bounds come from pass 2's per-(region, digit) output starts.
-constexpr int PART_TILE_C = PART_TILE;     // (4096-record tiles, two workgroups per CU, measured slower)
-constexpr int PART_THREADS_C = PART_THREADS;
-static_assert(PART_TILE_C / PART_THREADS_C == PART_ITEMS, "compact tiles keep the items per thread");
+// records per tile of these passes: 8192 (4096-record tiles, two workgroups per CU, measured
+// slower), 4096 for records above 16 bytes so that a tile fits in LDS
+template <typename R>
+constexpr int ptile() {
+    return sizeof(R) > 16 ? PART_TILE / 2 : PART_TILE;
+}
+template <typename R>
+constexpr int pthreads() {
+    return ptile<R>() / PART_ITEMS;
+}
 
 template <typename R>
-__global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* __restrict__ rin,
+__global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* __restrict__ rin,
                                                                       const uint16_t* __restrict__ bkt, size_t n,
                                                                       R* __restrict__ rout, int sh1,
                                                                       unsigned long long* __restrict__ cursor,
                                                                       const uint64_t* __restrict__ rend,
                                                                       uint32_t* __restrict__ flags,
                                                                       uint8_t* __restrict__ dout) {
-    __shared__ R s_rec[PART_TILE_C];
-    __shared__ uint16_t s_bk[PART_TILE_C];
+    __shared__ R s_rec[ptile<R>()];
+    __shared__ uint16_t s_bk[ptile<R>()];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX], s_ok[RADIX];
     __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
     const int tid = threadIdx.x;
-    const size_t base = (size_t)blockIdx.x * PART_TILE_C;
-    const uint32_t len = (uint32_t)min((size_t)PART_TILE_C, n - base);
+    const size_t base = (size_t)blockIdx.x * ptile<R>();
+    const uint32_t len = (uint32_t)min((size_t)ptile<R>(), n - base);
     const uint32_t m2 = (1u << sh1) - 1u;
     constexpr int NW = sizeof(R) / 4;   // records held as 32-bit words (kept in registers)
     uint32_t x[PART_ITEMS][NW];
@@ -768,7 +775,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* 
     uint32_t rank[PART_ITEMS];
 #pragma unroll
     for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
         if (li < len) {
             const uint32_t* src = (const uint32_t*)(rin + base + li);
 #pragma unroll
@@ -780,7 +787,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* 
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
         rank[r] = 0;
         if (li < len) rank[r] = atomicAdd(&s_cnt[(uint32_t)bk[r] >> sh1], 1u);
     }
@@ -803,7 +810,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* 
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
         if (li < len) {
             const uint32_t p = s_lstart[(uint32_t)bk[r] >> sh1] + rank[r];
             uint32_t* dst = (uint32_t*)(s_rec + p);
@@ -813,7 +820,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* 
         }
     }
     __syncthreads();
-    for (uint32_t p = tid; p < len; p += PART_THREADS_C) {
+    for (uint32_t p = tid; p < len; p += pthreads<R>()) {
         const uint32_t b = s_bk[p], d = b >> sh1;
         if (s_ok[d]) {
             const uint64_t at = s_gbase[d] + (p - s_lstart[d]);
@@ -826,13 +833,13 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cur_c(const R* 
 // second level: a chunk reserves one run per digit (from the digit bytes) on its region's
 // (digit) cursor
 template <typename R>
-__global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cm_c(const R* __restrict__ rin,
+__global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cm_c(const R* __restrict__ rin,
                                                                      const uint8_t* __restrict__ din,
                                                                      R* __restrict__ rout, Chunks ch,
                                                                      const uint32_t* __restrict__ cnt,
                                                                      unsigned long long* __restrict__ cursor) {
-    __shared__ R s_rec[PART_TILE_C];
-    __shared__ uint8_t s_dig[PART_TILE_C];
+    __shared__ R s_rec[ptile<R>()];
+    __shared__ uint8_t s_dig[ptile<R>()];
     __shared__ uint64_t s_gbase[RADIX];
     __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX];
     __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
@@ -846,7 +853,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cm_c(const R* _
     uint32_t dg[PART_ITEMS];
 #pragma unroll
     for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
         if (li < len) {
             const uint32_t* src = (const uint32_t*)(rin + st + li);
 #pragma unroll
@@ -864,7 +871,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cm_c(const R* _
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * PART_THREADS_C + tid;
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
         if (li < len) {
             const uint32_t p = s_lstart[dg[r]] + atomicAdd(&s_cnt[dg[r]], 1u);
             uint32_t* dst = (uint32_t*)(s_rec + p);
@@ -874,7 +881,7 @@ __global__ __launch_bounds__(PART_THREADS_C) void k_part_scatter_cm_c(const R* _
         }
     }
     __syncthreads();
-    for (uint32_t p = tid; p < len; p += PART_THREADS_C) {
+    for (uint32_t p = tid; p < len; p += pthreads<R>()) {
         const uint32_t d = s_dig[p];
         rout[s_gbase[d] + (p - s_lstart[d])] = s_rec[p];
     }
@@ -907,10 +914,10 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
     }
     uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX);
     HIPCHK(hipMemcpyAsync(dcur, cur.data(), 2 * RADIX * 8, hipMemcpyHostToDevice, c->stream));
-    const size_t n_tiles = (n + PART_TILE_C - 1) / PART_TILE_C;
+    const size_t n_tiles = (n + ptile<R>() - 1) / ptile<R>();
     uint8_t* dig = c->part_dig.get<uint8_t>(part_capacity(n));
     int t1 = kt_begin(c, K_PART_SCATTER1, (2.0 * sizeof(R) + 3.0) * n);
-    k_part_scatter_cur_c<R><<<(unsigned)n_tiles, PART_THREADS_C, 0, c->stream>>>(
+    k_part_scatter_cur_c<R><<<(unsigned)n_tiles, pthreads<R>(), 0, c->stream>>>(
         A, bkt, n, B, sh1, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig);
     kt_end(c, t1);
     HIPCHK(hipGetLastError());
@@ -926,7 +933,7 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
         rsize[d] = cur[d] - rstart[d];
     }
     const int nr = RADIX;
-    Chunks ch = make_chunks(c, 0, rstart, rsize, PART_TILE_C);
+    Chunks ch = make_chunks(c, 0, rstart, rsize, ptile<R>());
     const uint64_t* cbase = (const uint64_t*)c->part_meta[0].p + 2 * nr;
     std::vector<uint64_t> ds(nr);
     uint64_t acc = 0;
@@ -947,7 +954,7 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
     kt_end(c, t2);
     HIPCHK(hipMemcpyAsync(start_rd, cursor, (size_t)RADIX * nr * 8, hipMemcpyDeviceToDevice, c->stream));
     int t3 = kt_begin(c, K_PART_SCATTER, (2.0 * sizeof(R) + 1.0) * n);
-    k_part_scatter_cm_c<R><<<(unsigned)ch.n, PART_THREADS_C, 0, c->stream>>>(B, dig, A, ch, cnt, cursor);
+    k_part_scatter_cm_c<R><<<(unsigned)ch.n, pthreads<R>(), 0, c->stream>>>(B, dig, A, ch, cnt, cursor);
     kt_end(c, t3);
     const uint64_t nb = (uint64_t)1 << hb;
     int tb = kt_begin(c, K_BUCKET_BOUNDS, 16.0 * nb);
@@ -961,6 +968,7 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
 
 template bool hash_partition_c<RecC64>(Ctx*, RecC64*, const uint16_t*, RecC64*, size_t, int, uint64_t*, uint64_t*);
 template bool hash_partition_c<RecC32>(Ctx*, RecC32*, const uint16_t*, RecC32*, size_t, int, uint64_t*, uint64_t*);
+template bool hash_partition_c<RecW>(Ctx*, RecW*, const uint16_t*, RecW*, size_t, int, uint64_t*, uint64_t*);
 template Rec64* hash_partition<Rec64>(Ctx*, Rec64*, Rec64*, size_t, int);
 template Rec32* hash_partition<Rec32>(Ctx*, Rec32*, Rec32*, size_t, int);
 

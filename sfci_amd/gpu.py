@@ -155,7 +155,7 @@ def lib():
         "sfci_wide_set_precision": (ip, [vp, ip]),
         "sfci_wide_set_workspace": (ip, [vp, u64]),
         "sfci_wide_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
-        "sfci_wide_set_group_bits": (ip, [vp, ip]),
+        "sfci_wide_set_merge": (ip, [vp, ip, ip]),
         "sfci_wide_vec_copy": (ip, [vp, vp]),
         "sfci_wide_vec_clear": (ip, [vp]),
         "sfci_wide_dot": (ip, [vp, vp, vp, C.POINTER(C.c_double)]),
@@ -584,8 +584,10 @@ class WideContext:
     def set_workspace(self, nbytes: int):
         self._check(lib().sfci_wide_set_workspace(self._h, int(nbytes)))
 
-    def set_group_bits(self, bits: int):
-        self._check(lib().sfci_wide_set_group_bits(self._h, int(bits)))
+    def set_merge(self, hash_bits: int = 0, table_fill: int = 0):
+        """Merge knobs (results identical for every value): bucket hash bits (9..16, 0 =
+        automatic) and the LDS-table fill that splits a bucket into classes (0 = default)."""
+        self._check(lib().sfci_wide_set_merge(self._h, int(hash_bits), int(table_fill)))
 
     def stats(self) -> dict:
         s = StatsT()

@@ -85,7 +85,8 @@ def test_wide_hv_bit_exact(wctx, worc, winput, mixed, nd, cap, lam, coeff):
 
 def test_wide_hv_multipass(wctx, worc, winput):
     lo, hi, v = winput
-    wctx.set_workspace(104 * 2_000_000)    # a few million records per pass
+    # about 3.5 million records per pass (52 B each above the fixed partition slack)
+    wctx.set_workspace(600_000_000)
     try:
         gl, gh, gv, gd = gpu_wide_hv(wctx, lo, hi, v, 2500, 0.2, 1.0, 60_000_000)
         st = wctx.stats()
@@ -98,19 +99,35 @@ def test_wide_hv_multipass(wctx, worc, winput):
     assert bits([gd])[0] == bits([od])[0]
 
 
-def test_wide_hv_shared_group_hashes(wctx, worc, winput):
-    """With 16 grouping bits many distinct keys share a group: they must still merge exactly."""
+@pytest.mark.parametrize("hash_bits,fill", [(9, 0), (9, 179), (16, 0), (12, 40)])
+def test_wide_hv_merge_classes(wctx, worc, winput, hash_bits, fill):
+    """Few buckets with a small table fill: buckets are merged in many hash classes (each
+    rereading the bucket); many buckets: most hold a few targets.  Same result either way."""
     lo, hi, v = winput
-    wctx.set_group_bits(16)
+    wctx.set_merge(hash_bits, fill)
     try:
-        gl, gh, gv, gd = gpu_wide_hv(wctx, lo, hi, v, 300, 0.1, 1.0, 60_000_000)
+        gl, gh, gv, gd = gpu_wide_hv(wctx, lo, hi, v, 2500, 0.1, 1.0, 60_000_000)
         st = wctx.stats()
     finally:
-        wctx.set_group_bits(48)
-    ol, oh, ov, od = worc.hv_wide(lo, hi, v, nd=300, lam=0.1)
-    assert st["n_unique"] > 4 * 65536     # far more distinct targets than groups
+        wctx.set_merge(0, 0)
+    log("hash bits %d fill %d: %d distinct targets, %d class splits" % (hash_bits, fill, st["n_unique"],
+                                                                       st["n_split"]))
+    assert st["hash_bits"] == hash_bits and st["n_fallback"] == 0
+    if fill:
+        assert st["n_split"] >= (1 << hash_bits)
+    ol, oh, ov, od = worc.hv_wide(lo, hi, v, nd=2500, lam=0.1)
     assert_same(gl, gh, gv, ol, oh, ov)
     assert bits([gd])[0] == bits([od])[0]
+
+
+def test_wide_energy_pull(wctx, worc, winput):
+    """<v|H|v> by pull on 128-bit keys against the oracle's exact <v|H v> (the two round at
+    different points: per source against per target)."""
+    lo, hi, v = winput
+    e = wctx.energy(wctx.vector(len(lo), lo, hi, v))
+    _, _, _, od = worc.hv_wide(lo, hi, v, lam=0.0, coeff=1.0)
+    log("pull %.15f, oracle <v|Hv> %.15f" % (e, od))
+    assert abs(e - od) < 1e-12 * max(1.0, abs(od))
 
 
 def test_wide_path_matches_64bit_path(make_system):
@@ -210,7 +227,8 @@ def test_wide_vector_ops_match_64bit(make_system):
         k1, x1 = L64.download()
         l2, h2, x2 = Lw.download()
         assert_same(l2, h2, x2, k1, np.zeros(len(k1), np.uint64), x1)
-        assert abs(c64.energy(L64) - cw.energy(Lw)) < 1e-12 * max(1.0, abs(c64.energy(L64)))
+        # the same pull with the same rounding (per source) on both paths
+        assert bits([c64.energy(L64)])[0] == bits([cw.energy(Lw)])[0]
     finally:
         c64.close()
         cw.close()
