@@ -32,6 +32,10 @@ typedef uint16_t bf16_t;
 constexpr int IN = MLP_IN, INP = MLP_INP, HID = MLP_HID;
 constexpr int XT_ROWS = 96;          // feature rows of the transposed batch (80 padded to 3 x 32)
 constexpr int FW_TM = 64;            // rows per tile of the fused forward
+constexpr int FW_MT = FW_TM / 32;     // 32-row MFMA tiles per tile
+constexpr int FW_THREADS = 512;       // eight wavefronts, two per SIMD
+constexpr int FW_FPT = FW_THREADS / FW_TM;   // threads per row building features
+constexpr int FW_FCOLS = 80 / FW_FPT;        // feature columns per thread
 constexpr int FW_XS = 88;            // LDS row stride (BF16) of the feature tile
 constexpr int FW_HS = 264;           // LDS row stride (BF16) of the hidden tile
 
@@ -102,121 +106,135 @@ __device__ __forceinline__ float feature(int col, uint64_t det, int n_so, uint32
 
 enum { IN_X = 0, IN_REC = 1, IN_SAMPLE = 2 };
 
-// Fused three-layer forward, persistent over tiles of FW_TM rows, one workgroup of four
-// wavefronts.  Wavefront w owns hidden columns [64 w, 64 w + 64) of both layers and keeps
+// Fused three-layer forward, persistent over tiles of FW_TM rows, one workgroup of eight
+// wavefronts (two per SIMD, so that one wavefront's matrix-core work overlaps another's
+// epilogue).  Wavefront w owns hidden columns [32 w, 32 w + 32) of both layers and keeps
 // their weight fragments (B operands) in registers for the whole launch, so the weights are
-// read from memory once per workgroup.  Per tile: features -> LDS; layer 1 (A from LDS)
-// -> bias, SiLU, BF16 -> LDS; layer 2 (A from LDS) -> bias, SiLU, BF16 -> LDS; the output
-// dot over each row is summed by four threads over 64 columns each and the four partials
-// are added in a fixed order.
+// read from memory once per workgroup.  Per tile: features -> LDS (built from inputs loaded
+// during the previous tile); layer 1 (A from LDS) -> bias, SiLU, BF16 -> LDS; layer 2 (A from
+// LDS) -> bias, SiLU, BF16 -> LDS; the output dot over each row is summed by four threads
+// over 64 columns each and the four partials are added in a fixed order.  Every output
+// element is the same chain of MFMA k-steps whatever the tile shape.
 // Inputs: IN_X, BF16 features [n][INP]; IN_REC, screened records; IN_SAMPLE, samples.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev P, int n_so, const void* __restrict__ in, size_t n,
-                                                 float* __restrict__ y, FwdSave sv) {
+__global__ __launch_bounds__(FW_THREADS) void k_mlp_fwd(MlpDev P, int n_so, const void* __restrict__ in, size_t n,
+                                                        float* __restrict__ y, FwdSave sv) {
     __shared__ __attribute__((aligned(16))) bf16_t sx[FW_TM * FW_XS];
     __shared__ __attribute__((aligned(16))) bf16_t sh[FW_TM * FW_HS];
     __shared__ float sw3[HID];
     __shared__ float ypart[4][FW_TM];
+    static_assert(FW_FCOLS * FW_FPT == 80 && FW_FCOLS % 2 == 0, "feature columns split evenly in pairs");
+    static_assert(4 * FW_TM <= FW_THREADS, "the output dot uses four threads per row");
+    static_assert(FW_THREADS / 64 * 32 == HID, "each wavefront owns 32 hidden columns");
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 31, g = lane >> 5;
-    sw3[tid] = P.w3r[tid];
-    bf16x8 w1f[2][INP / 16], w2f[2][HID / 16];
-    float b1c[2], b2c[2];
-    for (int nt = 0; nt < 2; ++nt) {
-        const int col = w * 64 + nt * 32 + li;
-        for (int ks = 0; ks < INP / 16; ++ks) w1f[nt][ks] = ld8(P.w1t + (size_t)col * INP + ks * 16 + g * 8);
-        for (int ks = 0; ks < HID / 16; ++ks) w2f[nt][ks] = ld8(P.w2t + (size_t)col * HID + ks * 16 + g * 8);
-        b1c[nt] = P.b1[col];
-        b2c[nt] = P.b2[col];
-    }
+    if (tid < HID) sw3[tid] = P.w3r[tid];
+    bf16x8 w1f[INP / 16], w2f[HID / 16];
+    const int col = w * 32 + li;
+    for (int ks = 0; ks < INP / 16; ++ks) w1f[ks] = ld8(P.w1t + (size_t)col * INP + ks * 16 + g * 8);
+    for (int ks = 0; ks < HID / 16; ++ks) w2f[ks] = ld8(P.w2t + (size_t)col * HID + ks * 16 + g * 8);
+    const float b1c = P.b1[col], b2c = P.b2[col];
     const float b3 = P.b3[0];
     const size_t n_tiles = (n + FW_TM - 1) / FW_TM;
+    // this thread's feature inputs: row fr of the tile, columns [FW_FCOLS part, + FW_FCOLS)
+    const int fr = tid / FW_FPT, part = tid % FW_FPT;
+    uint64_t p_det = 0;
+    uint32_t p_cls = 0;
+    float p_lv = 0.0f;
+    uint32_t p_x[FW_FCOLS / 2] = {};   // IN_X: BF16 values, two per word
+    auto fetch = [&](size_t tile) {
+        const size_t gr = tile * FW_TM + fr;
+        if (tile >= n_tiles || gr >= n) return;
+        if constexpr (MODE == IN_X) {
+            const uint32_t* src = (const uint32_t*)((const bf16_t*)in + gr * INP + part * FW_FCOLS);
+            for (int c2 = 0; c2 < FW_FCOLS / 2; ++c2) p_x[c2] = src[c2];
+        } else if constexpr (MODE == IN_REC) {
+            const RecS rr = ((const RecS*)in)[gr];
+            p_det = rr.det;
+            p_cls = rr.cls;
+            p_lv = plog2(rr.val);
+        } else {
+            const sfci_screen_sample_t s = ((const sfci_screen_sample_t*)in)[gr];
+            p_det = s.det;
+            p_cls = s.cls;
+            p_lv = s.logv;
+        }
+    };
+    fetch(blockIdx.x);
     for (size_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const size_t row0 = tile * FW_TM;
-        // features: 4 threads per row, 20 columns each
+        // features of this tile from the inputs fetched earlier; then the next tile's inputs
         {
-            const int r = tid >> 2, part = tid & 3;
-            const size_t gr = row0 + r;
-            bf16_t* dst = sx + r * FW_XS + part * 20;
-            if (gr >= n) {
-                for (int c2 = 0; c2 < 20; ++c2) dst[c2] = 0;
-            } else if (MODE == IN_X) {
-                const bf16_t* src = (const bf16_t*)in + gr * INP + part * 20;
-                for (int c2 = 0; c2 < 20; ++c2) dst[c2] = src[c2];
+            bf16_t* dst = sx + fr * FW_XS + part * FW_FCOLS;
+            if (row0 + fr >= n) {
+                for (int c2 = 0; c2 < FW_FCOLS; ++c2) dst[c2] = 0;
+            } else if constexpr (MODE == IN_X) {
+                uint32_t* d32 = (uint32_t*)dst;
+                for (int c2 = 0; c2 < FW_FCOLS / 2; ++c2) d32[c2] = p_x[c2];
             } else {
-                uint64_t det;
-                uint32_t cls;
-                float lv;
-                if (MODE == IN_REC) {
-                    const RecS rr = ((const RecS*)in)[gr];
-                    det = rr.det;
-                    cls = rr.cls;
-                    lv = plog2(rr.val);
-                } else {
-                    const sfci_screen_sample_t s = ((const sfci_screen_sample_t*)in)[gr];
-                    det = s.det;
-                    cls = s.cls;
-                    lv = s.logv;
-                }
-                for (int c2 = 0; c2 < 20; ++c2) dst[c2] = f2bf_hw(feature(part * 20 + c2, det, n_so, cls, lv, P));
+                for (int c2 = 0; c2 < FW_FCOLS; ++c2)
+                    dst[c2] = f2bf_hw(feature(part * FW_FCOLS + c2, p_det, n_so, p_cls, p_lv, P));
             }
         }
+        fetch(tile + gridDim.x);
         __syncthreads();
         // layer 1
-        f32x16 acc[2][2];
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b)
-                for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+        f32x16 acc[FW_MT];
+        for (int a = 0; a < FW_MT; ++a)
+            for (int r = 0; r < 16; ++r) acc[a][r] = 0.0f;
 #pragma unroll
         for (int ks = 0; ks < INP / 16; ++ks) {
-            bf16x8 av[2];
-            for (int mt = 0; mt < 2; ++mt) av[mt] = ld8(sx + (mt * 32 + li) * FW_XS + ks * 16 + g * 8);
-            for (int mt = 0; mt < 2; ++mt)
-                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], w1f[nt][ks], acc[mt][nt]);
+            bf16x8 av[FW_MT];
+#pragma unroll
+            for (int mt = 0; mt < FW_MT; ++mt) av[mt] = ld8(sx + (mt * 32 + li) * FW_XS + ks * 16 + g * 8);
+#pragma unroll
+            for (int mt = 0; mt < FW_MT; ++mt) acc[mt] = mfma32(av[mt], w1f[ks], acc[mt]);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        for (int mt = 0; mt < 2; ++mt)
-            for (int nt = 0; nt < 2; ++nt) {
-                const int col = w * 64 + nt * 32 + li;
-                for (int r = 0; r < 16; ++r) {
-                    const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-                    const float z = acc[mt][nt][r] + b1c[nt];
-                    const bf16_t h = f2bf_hw(silu(z));
-                    sh[row * FW_HS + col] = h;
-                    if (sv.z1 && row0 + row < n) {
-                        sv.z1[(row0 + row) * HID + col] = z;
-                        sv.h1[(row0 + row) * HID + col] = h;
-                    }
+#pragma unroll
+        for (int mt = 0; mt < FW_MT; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+                const float z = acc[mt][r] + b1c;
+                const bf16_t h = f2bf_hw(silu(z));
+                sh[row * FW_HS + col] = h;
+                if (sv.z1 && row0 + row < n) {
+                    sv.z1[(row0 + row) * HID + col] = z;
+                    sv.h1[(row0 + row) * HID + col] = h;
                 }
             }
+        }
         __syncthreads();
         // layer 2
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b)
-                for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+        for (int a = 0; a < FW_MT; ++a)
+            for (int r = 0; r < 16; ++r) acc[a][r] = 0.0f;
 #pragma unroll
         for (int ks = 0; ks < HID / 16; ++ks) {
-            bf16x8 av[2];
-            for (int mt = 0; mt < 2; ++mt) av[mt] = ld8(sh + (mt * 32 + li) * FW_HS + ks * 16 + g * 8);
-            for (int mt = 0; mt < 2; ++mt)
-                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma32(av[mt], w2f[nt][ks], acc[mt][nt]);
+            bf16x8 av[FW_MT];
+#pragma unroll
+            for (int mt = 0; mt < FW_MT; ++mt) av[mt] = ld8(sh + (mt * 32 + li) * FW_HS + ks * 16 + g * 8);
+#pragma unroll
+            for (int mt = 0; mt < FW_MT; ++mt) acc[mt] = mfma32(av[mt], w2f[ks], acc[mt]);
+            __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();   // every wavefront has read the first hidden layer
-        for (int mt = 0; mt < 2; ++mt)
-            for (int nt = 0; nt < 2; ++nt) {
-                const int col = w * 64 + nt * 32 + li;
-                for (int r = 0; r < 16; ++r) {
-                    const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-                    const float z = acc[mt][nt][r] + b2c[nt];
-                    const bf16_t h = f2bf_hw(silu(z));
-                    sh[row * FW_HS + col] = h;
-                    if (sv.z2 && row0 + row < n) {
-                        sv.z2[(row0 + row) * HID + col] = z;
-                        sv.h2[(row0 + row) * HID + col] = h;
-                    }
+#pragma unroll
+        for (int mt = 0; mt < FW_MT; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = mt * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+                const float z = acc[mt][r] + b2c;
+                const bf16_t h = f2bf_hw(silu(z));
+                sh[row * FW_HS + col] = h;
+                if (sv.z2 && row0 + row < n) {
+                    sv.z2[(row0 + row) * HID + col] = z;
+                    sv.h2[(row0 + row) * HID + col] = h;
                 }
             }
+        }
         __syncthreads();
         // layer 3: thread (row, quarter) sums 64 columns in order
-        {
+        if (tid < 4 * FW_TM) {
             const int r = tid >> 2, q = tid & 3;
             const bf16_t* hr = sh + r * FW_HS + q * 64;
             float v = 0.0f;
@@ -467,11 +485,11 @@ void fwd_launch(sfci_mlp* net, int mode, const void* in, size_t n, float* y, Fwd
         P.mag_mul = 0.69314718f;
     }
     if (mode == IN_X)
-        k_mlp_fwd<IN_X><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_X><<<grid, FW_THREADS, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     else if (mode == IN_REC)
-        k_mlp_fwd<IN_REC><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_REC><<<grid, FW_THREADS, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     else
-        k_mlp_fwd<IN_SAMPLE><<<grid, 256, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
+        k_mlp_fwd<IN_SAMPLE><<<grid, FW_THREADS, 0, c->stream>>>(P, c->n_so, in, n, y, sv);
     HIPCHK(hipGetLastError());
 }
 
