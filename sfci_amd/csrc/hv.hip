@@ -654,6 +654,14 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
     const uint64_t base_dense = row_lo * H.dim;
     const uint32_t nsa = (uint32_t)H.n_sa, ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair;
     for (uint32_t x = tid; x < nsa; x += SFCI_BLOCK) sdec[x] = sdec_g[x];
+    // classes (highest orbital m of a spin-0 string; its ranks fill [C(m,k), C(m+1,k))) that
+    // meet the pass's rows: m1 <= m <= m2
+    int m1 = 0, m2 = H.n_orb - 1;
+    {
+        auto cls_hi = [&](int m) { return m + 1 < H.n_orb ? H.binom[(m + 1) * 33 + H.k] : H.dim; };
+        while (m1 < H.n_orb - 1 && cls_hi(m1) <= row_lo) ++m1;
+        while (m2 > 0 && H.binom[m2 * 33 + H.k] >= row_hi) --m2;
+    }
     __syncthreads();
     bool range_bad = false, wbad = false;
     for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
@@ -766,7 +774,98 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         // C (spin-0 doubles in range: each wavefront reserves its slots in the source's range
         // with one LDS atomic; the order of records inside the range is free, every sum
         // downstream being exact)
-        {
+        if (!SCR && off != nullptr) {
+            // a row-range pass: only the doubles whose target class meets the pass.  The
+            // target's highest orbital is max(b, h) for empty pair (a < b) and h the highest
+            // occupied orbital the pair (i, j) leaves; so for each occupied pair, the empty
+            // pairs with b in a range, one run of the empty pairs in colex order (by b, then a):
+            // s_cb[o] is the run's first colex index, s_cpre the runs' exclusive prefix.
+            __shared__ uint16_t s_cb[LDS_PAIRS];
+            __shared__ uint32_t s_cpre[LDS_PAIRS + 1];
+            const uint32_t nop = H.n_opair;
+            const uint32_t vmask = ~s.str[0] & (H.n_orb >= 32 ? ~0u : ((1u << H.n_orb) - 1u));
+            auto nv_le = [&](int m) -> uint32_t {   // empty spin-0 orbitals <= m
+                if (m < 0) return 0u;
+                return (uint32_t)__popc(vmask & (m >= 31 ? ~0u : ((2u << m) - 1u)));
+            };
+            for (uint32_t o = tid; o < nop; o += SFCI_BLOCK) {
+                const uint16_t op = H.occ_pairs[o];
+                const int i = op & 0xff, jo = op >> 8;
+                int hr = -1;
+                for (int t = H.k - 1; t >= 0; --t)
+                    if (t != i && t != jo) {
+                        hr = s.occ[0][t];
+                        break;
+                    }
+                uint32_t lo = 0, hi = 0;
+                if (hr <= m2) {
+                    const uint32_t b2 = nv_le(m2), b1 = hr >= m1 ? 0u : nv_le(m1 - 1);
+                    lo = b1 * (b1 - (b1 ? 1u : 0u)) / 2;
+                    hi = b2 * (b2 - (b2 ? 1u : 0u)) / 2;
+                    if (hi < lo) hi = lo;
+                }
+                s_cb[o] = (uint16_t)lo;
+                s_cpre[o] = hi - lo;
+            }
+            if (tid == 0) s_ccur = 0;
+            __syncthreads();
+            if (w == 0) {   // exclusive prefix of the run lengths
+                uint32_t run = 0;
+                for (uint32_t b0 = 0; b0 < nop; b0 += SFCI_WAVE) {
+                    const uint32_t o = b0 + lane;
+                    const uint32_t len = o < nop ? s_cpre[o] : 0u;
+                    uint32_t inc = len;
+                    for (int d = 1; d < SFCI_WAVE; d <<= 1) {
+                        const uint32_t t = __shfl_up(inc, d, SFCI_WAVE);
+                        if (lane >= d) inc += t;
+                    }
+                    if (o < nop) s_cpre[o] = run + inc - len;
+                    run += __shfl(inc, SFCI_WAVE - 1, SFCI_WAVE);
+                }
+                if (lane == 0) s_cpre[nop] = run;
+            }
+            __syncthreads();
+            const uint32_t total = s_cpre[nop];
+            for (uint32_t f0 = 0; f0 < total; f0 += SFCI_BLOCK) {
+                const uint32_t f = f0 + tid;
+                bool have = false;
+                uint64_t key = 0;
+                T val = (T)0;
+                if (f < total) {
+                    uint32_t lo = 0, hi = nop;   // the last occupied pair whose run starts at or before f
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_cpre[mid] <= f) lo = mid;
+                        else hi = mid;
+                    }
+                    const uint16_t op = H.occ_pairs[lo];
+                    // colex index t -> empty pair (la < lb): lb(lb - 1)/2 <= t < lb(lb + 1)/2
+                    const int t = (int)s_cb[lo] + (int)(f - s_cpre[lo]);
+                    int lb = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)t)) * 0.5f);
+                    if (lb * (lb - 1) / 2 > t) --lb;
+                    if ((lb + 1) * lb / 2 <= t) ++lb;
+                    const int la = t - lb * (lb - 1) / 2;
+                    const int i = s.occ[0][op & 0xff], jo = s.occ[0][op >> 8];
+                    const int a = s.vir[0][la], b = s.vir[0][lb];
+                    const uint32_t st = s.str[0] ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+                    const uint64_t r0 = colex_rank(st, H.binom);
+                    have = r0 >= row_lo && r0 < row_hi;
+                    if (have) {
+                        key = (r0 * H.dim - base_dense + s.rank[1]) << 1;
+                        val = cand_val<T>(double_elem(H, d, 2 * i, 2 * jo, 2 * a, 2 * b), temp);
+                        range_bad |= !fx_in_range((double)val);
+                    }
+                }
+                const uint64_t bal = __ballot(have);
+                if (bal) {
+                    const int leader = __ffsll((long long)bal) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&s_ccur, (uint32_t)popc64(bal));
+                    base = __shfl(base, leader, SFCI_WAVE);
+                    if (have) wbad |= !sink.put(pos + base + (uint32_t)popc64(bal & lt), key, val, 2u, 0ull);
+                }
+            }
+        } else {
             const bool all_in = (off == nullptr);
             if (!all_in) {
                 if (tid == 0) s_ccur = 0;
@@ -1434,8 +1533,6 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
 #pragma unroll
                 for (int r = 0; r < BRB; ++r) {
                     if (k_[r] == ~0ull) continue;
-                    // a full table's class is merged again as subclasses: stop inserting
-                    if (__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                     const uint64_t raw = k_[r];
                     const unsigned long long key = raw >> 1;
                     uint32_t s16, c16;   // 16 hash bits for the slot, the next 16 for the class
