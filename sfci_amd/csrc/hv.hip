@@ -1817,12 +1817,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     // two record buffers per pass and one digit byte per first-level slot
     size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
     if (c->merge_mode == 0) {
-        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load 0.9 (more
-        // passes cost more generation than the occasional class split), using the distinct/
-        // candidate ratio of the previous call; a bucket that overflows is merged in hash
-        // classes, each rereading its records
+        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load
+        // c->pass_load (0.9 at most: more passes cost more generation than an occasional
+        // class split), using the distinct/candidate ratio of the previous call; a bucket that
+        // overflows is merged in hash classes, each rereading its records, and the load aimed
+        // at adapts to how many buckets split in the previous call
         const double ratio = std::min(1.0, c->unique_ratio * 1.25);
-        const double bucket_cap = 1.5 * 0.6 * BR_SLOTS * 65536.0 / ratio;
+        const double bucket_cap = c->pass_load * BR_SLOTS * 65536.0 / ratio;
         if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
     }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
@@ -1954,6 +1955,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     unsigned long long* d_uniq = d_out_count + 1;
     HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
     uint64_t out_count_host = 0;
+    uint64_t buckets_merged = 0;   // hash buckets of the passes merged through LDS tables
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const uint64_t row_lo = passes[pi].first, row_hi = passes[pi].second;
         if (row_lo >= row_hi) continue;
@@ -2143,6 +2145,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                 out_count_host = cnts[0];
                 uniq_host = cnts[1];
                 st.n_split += cnts[2];
+                if (hashed) buckets_merged += (uint64_t)1 << hb;
                 diag_total = fx_add(diag_total, acc_read_raw(c, 1));
                 break;
             }
@@ -2182,6 +2185,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
     st.n_unique = uniq;
     if (st.n_candidates) c->unique_ratio = std::max(1e-6, (double)uniq / (double)st.n_candidates);
+    if (c->merge_mode == 0 && buckets_merged) {
+        // the distinct-target count of a pass varies with its row range: aim lower while more
+        // than 1% of the buckets split, back up while none does
+        const double split_frac = (double)st.n_split / (double)buckets_merged;
+        if (split_frac > 0.01) c->pass_load = std::max(0.5, c->pass_load * 0.85);
+        else if (st.n_split == 0) c->pass_load = std::min(0.9, c->pass_load * 1.05);
+    }
     // 4. staged dense indices -> determinant keys; top out_cap by |y|, then key order
     size_t m = out_count_host;
     if (m > 0) {

@@ -308,6 +308,7 @@ struct Ctx {
     DevBuf plan_k[2], plan_v[2], plan_fl, plan_pos, plan_first, plan_str, plan_cnt;   // row-grouped planning
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
+    double pass_load = 0.9;          // distinct targets per LDS-table bucket aimed at, in table slots
     bool g_one_block = false;        // the four two-electron spin blocks share one table
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
