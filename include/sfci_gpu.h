@@ -66,6 +66,8 @@ typedef struct {
     double ms_score;        /* screened H.v: scoring time */
     uint64_t n_split;       /* bucket merge: hash classes split because their LDS table filled
                                (each split rereads the bucket's records once per subclass) */
+    uint64_t n_str0, n_str1; /* distinct spin-0 / spin-1 source strings whose same-spin double
+                               tables the generator read (0: it enumerated those doubles itself) */
 } sfci_hv_stats_t;
 
 /* Context from a reference _IN integral file: replaces load_integrals/ReadElements
@@ -102,13 +104,19 @@ int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
  * full at 1/16 of its usual occupancy (forces nested class splits), 4 = as 2 with full-key
  * records (no compact 12/8-byte records).  Results are identical in every mode. */
 int sfci_set_merge_mode(sfci_ctx* ctx, int mode, int hash_bits);
+/* How H.v evaluates same-spin doubles (no reference counterpart; a tuning and test knob):
+ * 0 = automatic, 1 = every source enumerates and evaluates its own (OffDiagGen_bit's way,
+ * src/bitstr.cpp:236-300), 2 = from per-string tables always: each distinct source spin
+ * string's doubles are evaluated once per call and streamed to every source with that
+ * string.  Results are identical in every mode. */
+int sfci_set_gen_tables(sfci_ctx* ctx, int mode);
 int sfci_sync(sfci_ctx* ctx);
 
 /* Per-kernel HIP-event timings accumulated on the context's stream since the last reset
  * (no reference counterpart: the reference only prints wall-clock timers,
  * src/functions.cpp:800-801).  Kernel ids 0..SFCI_KERNEL_COUNT-1 are named by
  * sfci_kernel_name(); bytes are the algorithmic bytes each launch reads and writes. */
-#define SFCI_KERNEL_COUNT 17
+#define SFCI_KERNEL_COUNT 18
 typedef struct {
     double ms;
     uint64_t launches;

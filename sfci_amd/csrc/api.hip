@@ -295,6 +295,8 @@ int sfci_hv_stats(const sfci_ctx* X, sfci_hv_stats_t* s) {
     s->screen_eps = t.screen_eps;
     s->ms_score = t.ms_score;
     s->n_split = t.n_split;
+    s->n_str0 = t.n_str0;
+    s->n_str1 = t.n_str1;
     return SFCI_OK;
 }
 
@@ -308,6 +310,12 @@ int sfci_set_merge_mode(sfci_ctx* X, int mode, int hash_bits) {
     if (!X || mode < 0 || mode > 4 || (mode >= 2 && (hash_bits < 1 || hash_bits > 16))) return SFCI_E_ARG;
     X->c.merge_mode = mode;
     X->c.merge_hash_bits = hash_bits;
+    return SFCI_OK;
+}
+
+int sfci_set_gen_tables(sfci_ctx* X, int mode) {
+    if (!X || mode < 0 || mode > 2) return SFCI_E_ARG;
+    X->c.gen_tables = mode;
     return SFCI_OK;
 }
 
@@ -335,7 +343,7 @@ const char* sfci_kernel_name(int id) {
                                        "select",          "vecops",
                                        "k_energy",        "k_part_scatter_cur<DigitHash>",
                                        "k_mlp_fwd",       "screen_select",
-                                       "mlp_train_step"};
+                                       "mlp_train_step",  "k_ss_table"};
     return (id >= 0 && id < K_NUM) ? names[id] : "";
 }
 

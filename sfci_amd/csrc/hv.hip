@@ -331,6 +331,98 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const ui
     }
 }
 
+// Spin orbitals strictly between p and q (the mask sign_between counts in).
+__device__ __forceinline__ uint64_t between_mask(int p, int q) {
+    const int lo = p < q ? p : q, hi = p < q ? q : p;
+    return (hi - lo > 1) ? ((((uint64_t)1 << (hi - lo - 1)) - 1) << (lo + 1)) : 0ull;
+}
+
+// Same-spin double tables (SsEnt) of n_str distinct spin-SP strings, one workgroup per
+// string: its n_daa doubles in the generator's enumeration order (occupied pair, then
+// empty pair), each with the excited string's rank and the element of double_elem split
+// into its spin-SP part h and the other-spin parity mask m.  With n_pass > 1 the entries
+// are grouped by the target pass of their rank (bounds as in k_count_passes; order inside
+// a group unspecified) and pstart[u * (n_pass + 1) + p] receives pass p's first entry.
+template <int SP>
+__global__ __launch_bounds__(SFCI_BLOCK) void k_ss_table(HamDev Hg, const uint64_t* __restrict__ strs, size_t n_str,
+                                                         const uint64_t* __restrict__ bounds_g, int n_pass,
+                                                         SsEnt* __restrict__ tab, uint32_t* __restrict__ pstart) {
+    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
+    __shared__ uint64_t bounds[MAX_PASSES + 1];
+    __shared__ uint32_t s_cnt[MAX_PASSES + 1];
+    __shared__ uint8_t s_occ[SFCI_MAX_K], s_vir[SFCI_MAX_ORB];
+    const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
+    const int tid = threadIdx.x;
+    const bool grouped = n_pass > 1;
+    if (grouped)
+        for (int i = tid; i <= n_pass; i += SFCI_BLOCK) bounds[i] = bounds_g[i];
+    const uint32_t ndaa = (uint32_t)H.n_daa, nvp = H.n_vpair;
+    for (size_t u = blockIdx.x; u < n_str; u += gridDim.x) {
+        const uint32_t S = (uint32_t)strs[u];
+        __syncthreads();   // the previous string's lists and counters are no longer read
+        if (tid == 0) {
+            int a = 0, b = 0;
+            for (int i = 0; i < H.n_orb; ++i) {
+                if (S >> i & 1u) s_occ[a++] = (uint8_t)i;
+                else s_vir[b++] = (uint8_t)i;
+            }
+        }
+        if (grouped)
+            for (int p = tid; p < n_pass; p += SFCI_BLOCK) s_cnt[p] = 0;
+        __syncthreads();
+        auto target = [&](const DivIter& it, int& i, int& jo, int& a, int& b) -> uint32_t {
+            const uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
+            i = s_occ[op & 0xff];
+            jo = s_occ[op >> 8];
+            a = s_vir[vp & 0xff];
+            b = s_vir[vp >> 8];
+            return S ^ (1u << i) ^ (1u << jo) ^ (1u << a) ^ (1u << b);
+        };
+        if (grouped) {
+            DivIter it(tid, SFCI_BLOCK, nvp);
+            for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
+                int i, jo, a, b;
+                const uint64_t r = colex_rank(target(it, i, jo, a, b), H.binom);
+                atomicAdd(&s_cnt[pass_of(bounds, n_pass, r)], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t run = 0;
+                uint32_t* ps = pstart + u * (size_t)(n_pass + 1);
+                for (int p = 0; p < n_pass; ++p) {
+                    const uint32_t c = s_cnt[p];
+                    ps[p] = run;
+                    s_cnt[p] = run;   // the pass's cursor
+                    run += c;
+                }
+                ps[n_pass] = run;
+            }
+            __syncthreads();
+        }
+        SsEnt* out = tab + u * (size_t)ndaa;
+        const uint64_t d = spread_string(S) << SP;
+        DivIter it(tid, SFCI_BLOCK, nvp);
+        for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
+            int i, jo, a, b;
+            const uint64_t r = colex_rank(target(it, i, jo, a, b), H.binom);
+            const uint32_t slot = grouped ? atomicAdd(&s_cnt[pass_of(bounds, n_pass, r)], 1u) : f;
+            const int n = 2 * i + SP, q = 2 * jo + SP, x = 2 * a + SP, y = 2 * b + SP;
+            const uint64_t m64 = between_mask(q, y) ^ between_mask(n, x);
+            out[slot] = SsEnt{(uint32_t)r, spin_string(m64, 1 - SP), double_elem(H, d, n, q, x, y)};
+        }
+    }
+}
+
+// Per-source index of its string in a distinct-string table built from the string-sorted
+// order: sorted position i holds source src_of[i], and starts a new string where fl[i] = 1
+// (pos = exclusive scan of fl).
+__global__ void k_src_string_index(const double* __restrict__ src_of, const uint32_t* __restrict__ fl,
+                                   const uint64_t* __restrict__ pos, size_t n, uint32_t* __restrict__ idx) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        idx[(size_t)src_of[i]] = (uint32_t)(pos[i] + fl[i] - 1);
+}
+
 // Histogram of connection counts over target-row groups (row >> grp_shift), for pass
 // planning, per distinct spin-0 string astr[j] weighted by its number of sources
 // first[j + 1] - first[j].
@@ -382,11 +474,13 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
-// Row table of the sources' spin-0 strings: keys = strings sorted ascending with the
-// source index as payload; fl[i] = 1 where a new string starts.
-__global__ void k_alpha_keys(const uint64_t* __restrict__ sk, size_t n, uint64_t* __restrict__ k, double* __restrict__ v) {
+// String table of the sources' spin-s strings (spin 0: the target rows): keys = strings,
+// sorted ascending afterwards with the source index as payload; fl[i] = 1 where a new
+// string starts.
+__global__ void k_alpha_keys(const uint64_t* __restrict__ sk, size_t n, uint64_t* __restrict__ k, double* __restrict__ v,
+                             int spin) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        k[i] = spin_string(sk[i], 0);
+        k[i] = spin_string(sk[i], spin);
         v[i] = (double)i;
     }
 }
@@ -403,9 +497,9 @@ __global__ void k_row_table(const uint64_t* __restrict__ k, const uint32_t* __re
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         if (fl[i]) {
             str[pos[i]] = k[i];
-            first[pos[i]] = i;
+            if (first) first[pos[i]] = i;
         }
-        if (i == 0) first[n_row] = n;
+        if (i == 0 && first) first[n_row] = n;
     }
 }
 
@@ -624,7 +718,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                                               const uint16_t* __restrict__ sdec_g,
                                               typename RecOf<T>::type* __restrict__ out, uint64_t out_cap,
                                               double* __restrict__ dterm, uint32_t* flags, const ScreenOut& screen,
-                                              const CompactOut& cout = CompactOut{}) {
+                                              const CompactOut& cout = CompactOut{}, const SsTabs& tabs = SsTabs{}) {
     __shared__ SrcLds s;
     __shared__ RecS s_scr[SCR ? SCR_LDS : 1];
     __shared__ uint32_t s_scr_n;
@@ -689,6 +783,21 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                             sink, range_bad, wbad);
             pos += nsa;
             // A2
+            if (!SCR && tabs.b != nullptr) {
+                // from the table of the source's spin-1 string: sign by the spin-0 electrons
+                const SsEnt* E = tabs.b + (size_t)tabs.b_idx[j] * ndaa;
+                const uint32_t s0 = s.str[0];
+                const uint32_t dlt = (uint32_t)(pos & 63u);   // wavefront spans start at a multiple of 64 records
+                for (uint32_t f1 = tid; f1 < ndaa + dlt; f1 += SFCI_BLOCK) {
+                    if (f1 < dlt) continue;
+                    const uint32_t f = f1 - dlt;
+                    const SsEnt e = E[f];
+                    const double hij = (__popc(s0 & e.m) & 1) ? -e.h : e.h;
+                    const T val = cand_val<T>(hij, temp);
+                    range_bad |= !fx_in_range((double)val);
+                    wbad |= !sink.put(pos + f, (rowbase + e.r) << 1, val, 2u, 0ull);
+                }
+            } else {
             DivIter it(tid, SFCI_BLOCK, nvp);
             for (uint32_t f = tid; f < ndaa; f += SFCI_BLOCK, it.next()) {
                 uint16_t op = H.occ_pairs[it.q], vp = H.vir_pairs[it.r];
@@ -700,6 +809,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                 wbad |= !sink.put(pos + f, (rowbase + colex_rank(st, H.binom)) << 1, val, 2u,
                                   d ^ (1ull << (2 * i + 1)) ^ (1ull << (2 * jo + 1)) ^ (1ull << (2 * a + 1)) ^
                                       (1ull << (2 * b + 1)));
+            }
             }
             pos += ndaa;
             if constexpr (SCR) wbad |= !sink.checkpoint(&s_scr_base);
@@ -724,23 +834,30 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         {
             const uint32_t nB = na * nsa;
             const uint32_t N2 = (uint32_t)H.n_orb * (uint32_t)H.n_orb, gts = (uint32_t)H.gt_stride;
-            DivIter it(tid, SFCI_BLOCK, nsa ? nsa : 1);
+            // plain: item f goes to lane (f + dlt) mod 256 of its round, so that every
+            // wavefront's 64 consecutive records start at a record index divisible by 64
+            // (whole 128-byte lines of records and of bucket entries: partly written lines
+            // cost the memory a read-modify-write)
+            const uint32_t dlt = SCR ? 0u : (uint32_t)(pos & 63u);
+            const uint32_t nsa_m = nsa > 1 ? 0xFFFFFFFFu / nsa + 1u : 0u;   // f / nsa = umulhi(f, nsa_m), f < 2^16
             // screened: every thread runs every round (block-uniform), so the list can be
             // copied out between rounds; plain: each thread stops at its last item
-            for (uint32_t f00 = 0; f00 < nB; f00 += B2_UNROLL * SFCI_BLOCK) {
-                const uint32_t f0 = f00 + tid;
-                if (!SCR && f0 >= nB) break;
+            for (uint32_t f00 = 0; f00 < nB + dlt; f00 += B2_UNROLL * SFCI_BLOCK) {
+                const int f0 = (int)(f00 + tid) - (int)dlt;
+                if (!SCR && f0 >= (int)nB) break;
                 double g[B2_UNROLL];
                 uint32_t qa[B2_UNROLL], qb[B2_UNROLL];
                 bool neg[B2_UNROLL];
 #pragma unroll
                 for (int u = 0; u < B2_UNROLL; ++u) {
-                    qa[u] = it.q;
-                    qb[u] = it.r;
-                    it.next();
+                    const int fu = f0 + u * SFCI_BLOCK;
                     g[u] = 0.0;
                     neg[u] = false;
-                    if (f0 + (uint32_t)u * SFCI_BLOCK < nB) {
+                    qa[u] = qb[u] = 0;
+                    if (fu >= 0 && fu < (int)nB) {
+                        const uint32_t q = nsa > 1 ? __umulhi((uint32_t)fu, nsa_m) : (uint32_t)fu;
+                        qa[u] = q;
+                        qb[u] = (uint32_t)fu - q * nsa;
                         const uint32_t aw = s.aw[qa[u]], bw = s.bw[qb[u]];
                         const uint32_t i = (aw >> 10) & 31u, a = (aw >> 15) & 31u;
                         const uint32_t lo = (bw >> 10) & 31u, hi = (bw >> 15) & 31u, j = (bw >> 20) & 31u;
@@ -752,8 +869,9 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                 }
 #pragma unroll
                 for (int u = 0; u < B2_UNROLL; ++u) {
-                    const uint32_t f = f0 + (uint32_t)u * SFCI_BLOCK;
-                    if (f < nB) {
+                    const int fu = f0 + u * SFCI_BLOCK;
+                    if (fu >= 0 && fu < (int)nB) {
+                        const uint32_t f = (uint32_t)fu;
                         T val = cand_val<T>(neg[u] ? -g[u] : g[u], temp);
                         range_bad |= !fx_in_range((double)val);
                         uint64_t tdet = 0;
@@ -774,7 +892,30 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         // C (spin-0 doubles in range: each wavefront reserves its slots in the source's range
         // with one LDS atomic; the order of records inside the range is free, every sum
         // downstream being exact)
-        if (!SCR && off != nullptr) {
+        if (!SCR && tabs.a != nullptr) {
+            // from the table of the source's spin-0 string: this pass's entries, signed by the
+            // spin-1 electrons
+            const uint32_t u = tabs.a_idx[j];
+            uint32_t e0 = 0, e1 = ndaa;
+            if (tabs.n_pass > 1) {
+                const uint32_t* ps = tabs.a_start + (size_t)u * (tabs.n_pass + 1) + tabs.pass;
+                e0 = ps[0];
+                e1 = ps[1];
+            }
+            const SsEnt* E = tabs.a + (size_t)u * ndaa + e0;
+            const uint32_t ne = e1 - e0, s1 = s.str[1];
+            const uint64_t kb = s.rank[1] - base_dense;   // wraps; the sum with r * dim does not
+            const uint32_t dlt = (uint32_t)(pos & 63u);   // wavefront spans start at a multiple of 64 records
+            for (uint32_t f1 = tid; f1 < ne + dlt; f1 += SFCI_BLOCK) {
+                if (f1 < dlt) continue;
+                const uint32_t f = f1 - dlt;
+                const SsEnt e = E[f];
+                const double hij = (__popc(s1 & e.m) & 1) ? -e.h : e.h;
+                const T val = cand_val<T>(hij, temp);
+                range_bad |= !fx_in_range((double)val);
+                wbad |= !sink.put(pos + f, ((uint64_t)e.r * H.dim + kb) << 1, val, 2u, 0ull);
+            }
+        } else if (!SCR && off != nullptr) {
             // a row-range pass: only the doubles whose target class meets the pass.  The
             // target's highest orbital is max(b, h) for empty pair (a < b) and h the highest
             // occupied orbital the pair (i, j) leaves; so for each occupied pair, the empty
@@ -932,9 +1073,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
                                                          const uint16_t* __restrict__ sdec_g,
                                                          typename RecOf<T>::type* __restrict__ out,
                                                          uint64_t out_cap, double* __restrict__ dterm,
-                                                         uint32_t* flags) {
+                                                         uint32_t* flags, SsTabs tabs) {
     generate_body<T, false>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, out, out_cap, dterm, flags,
-                            ScreenOut{});
+                            ScreenOut{}, CompactOut{}, tabs);
 }
 
 // the same into compact records and their hash buckets
@@ -945,9 +1086,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate_compact(HamDev Hg, cons
                                                                  uint64_t row_hi, const uint64_t* __restrict__ off,
                                                                  const uint16_t* __restrict__ sdec_g, CompactOut cout,
                                                                  uint64_t out_cap, double* __restrict__ dterm,
-                                                                 uint32_t* flags) {
+                                                                 uint32_t* flags, SsTabs tabs) {
     generate_body<T, false, true>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, nullptr, out_cap,
-                                  dterm, flags, ScreenOut{}, cout);
+                                  dterm, flags, ScreenOut{}, cout, tabs);
 }
 
 // all target rows, screened (mixed-precision values)
@@ -1839,7 +1980,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         uint64_t* kb = c->plan_k[1].get<uint64_t>(ns);
         double* vb = c->plan_v[1].get<double>(ns);
         const int g = grid_for(c, (ns + SFCI_BLOCK - 1) / SFCI_BLOCK, 16);
-        k_alpha_keys<<<g, SFCI_BLOCK, 0, c->stream>>>(sk, ns, ka, va);
+        k_alpha_keys<<<g, SFCI_BLOCK, 0, c->stream>>>(sk, ns, ka, va, 0);
         radix_sort_pairs(c, &ka, &va, kb, vb, ns, H.n_orb);
         row_fl = c->plan_fl.get<uint32_t>(ns);
         row_pos = c->plan_pos.get<uint64_t>(ns + 1);
@@ -1918,12 +2059,14 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     }
     st.n_passes = passes.size();
     uint32_t* pcnt = nullptr;
+    uint64_t* dbounds = nullptr;
+    std::vector<uint64_t> bounds;
     if (passes.size() > 1) {
         if (passes.size() > (size_t)MAX_PASSES) throw HipError("more target passes than supported; raise the workspace", -4);
-        std::vector<uint64_t> bounds(passes.size() + 1);
+        bounds.resize(passes.size() + 1);
         for (size_t p = 0; p < passes.size(); ++p) bounds[p] = passes[p].first;
         bounds[passes.size()] = n_rows;
-        uint64_t* dbounds = c->part_meta[1].get<uint64_t>(bounds.size());
+        dbounds = c->part_meta[1].get<uint64_t>(bounds.size());
         HIPCHK(hipMemcpyAsync(dbounds, bounds.data(), bounds.size() * 8, hipMemcpyHostToDevice, c->stream));
         pcnt = c->src_cnt.get<uint32_t>(passes.size() * ns);
         row_table();
@@ -1935,8 +2078,72 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             row_src, row_fl, row_pos, ns, n_row_src, (int)passes.size(), pcnt_row, pcnt);
         kt_end(c, tcn);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->stream));   // bounds live on the host stack
     }
+    // same-spin double tables: each distinct source string's doubles (excited rank, element)
+    // computed once, so the generator's spin-0 doubles (class C) and spin-1 doubles (A2)
+    // stream them instead of enumerating and evaluating every double of every source in
+    // every pass.  Automatic when that work is large and the tables fit in a quarter of the
+    // workspace limit.
+    SsTabs tabs{};
+    tabs.n_pass = (int)passes.size();
+    st.n_str0 = st.n_str1 = 0;
+    if (H.n_daa > 0 && (c->gen_tables == 2 || (c->gen_tables == 0 && (uint64_t)ns * H.n_daa >= (1ull << 22)))) {
+        const size_t budget = c->gen_tables == 2 ? SIZE_MAX : c->ws_limit / 4;
+        const size_t ent = sizeof(SsEnt) * (size_t)H.n_daa;
+        const int g = grid_for(c, (ns + SFCI_BLOCK - 1) / SFCI_BLOCK, 16);
+        row_table();
+        uint64_t n1 = 0;
+        uint64_t* str1 = nullptr;
+        double* src1 = nullptr;
+        uint32_t *fl1 = nullptr;
+        uint64_t* pos1 = nullptr;
+        {
+            // distinct spin-1 strings, as row_table does for spin 0
+            uint64_t* ka = c->bstr_k[0].get<uint64_t>(ns);
+            double* va = c->bstr_v[0].get<double>(ns);
+            uint64_t* kb = c->bstr_k[1].get<uint64_t>(ns);
+            double* vb = c->bstr_v[1].get<double>(ns);
+            k_alpha_keys<<<g, SFCI_BLOCK, 0, c->stream>>>(sk, ns, ka, va, 1);
+            radix_sort_pairs(c, &ka, &va, kb, vb, ns, H.n_orb);
+            fl1 = c->bstr_fl.get<uint32_t>(ns);
+            pos1 = c->bstr_pos.get<uint64_t>(ns + 1);
+            k_row_starts<<<g, SFCI_BLOCK, 0, c->stream>>>(ka, ns, fl1);
+            exclusive_scan_u32(c, fl1, ns, pos1, pos1 + ns);
+            HIPCHK(hipMemcpyAsync(&n1, pos1 + ns, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            str1 = c->bstr_str.get<uint64_t>(n1);
+            k_row_table<<<g, SFCI_BLOCK, 0, c->stream>>>(ka, fl1, pos1, ns, n1, str1, nullptr);
+            src1 = va;
+        }
+        const int np = (int)passes.size();
+        const size_t bytes0 = n_row_src * ent + (np > 1 ? n_row_src * (size_t)(np + 1) * sizeof(uint32_t) : 0);
+        const size_t bytes1 = n1 * ent;
+        int tk = kt_begin(c, K_SS_TABLE, (double)(bytes0 + bytes1) + 8.0 * ns);
+        if (bytes0 <= budget) {
+            SsEnt* t0 = c->ss_tab[0].get<SsEnt>(n_row_src * H.n_daa);
+            uint32_t* ps = np > 1 ? c->ss_start.get<uint32_t>(n_row_src * (size_t)(np + 1)) : nullptr;
+            uint32_t* ix = c->ss_idx[0].get<uint32_t>(ns);
+            k_src_string_index<<<g, SFCI_BLOCK, 0, c->stream>>>(row_src, row_fl, row_pos, ns, ix);
+            k_ss_table<0><<<grid_for(c, n_row_src), SFCI_BLOCK, 0, c->stream>>>(H, row_str, n_row_src, dbounds, np, t0,
+                                                                                  ps);
+            tabs.a = t0;
+            tabs.a_start = ps;
+            tabs.a_idx = ix;
+            st.n_str0 = n_row_src;
+        }
+        if (bytes1 <= (budget > bytes0 ? budget - bytes0 : 0) || (tabs.a == nullptr && bytes1 <= budget)) {
+            SsEnt* t1 = c->ss_tab[1].get<SsEnt>(n1 * H.n_daa);
+            uint32_t* ix = c->ss_idx[1].get<uint32_t>(ns);
+            k_src_string_index<<<g, SFCI_BLOCK, 0, c->stream>>>(src1, fl1, pos1, ns, ix);
+            k_ss_table<1><<<grid_for(c, n1), SFCI_BLOCK, 0, c->stream>>>(H, str1, n1, nullptr, 1, t1, nullptr);
+            tabs.b = t1;
+            tabs.b_idx = ix;
+            st.n_str1 = n1;
+        }
+        kt_end(c, tk);
+        HIPCHK(hipGetLastError());
+    }
+    if (passes.size() > 1) HIPCHK(hipStreamSynchronize(c->stream));   // bounds live on the host stack
     // staging for merged output
     size_t stage_cap = std::max<size_t>(out_cap + (out_cap >> 2), (size_t)1 << 16);
     if (all) {
@@ -1959,6 +2166,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const uint64_t row_lo = passes[pi].first, row_hi = passes[pi].second;
         if (row_lo >= row_hi) continue;
+        tabs.pass = (int)pi;
         // 3a. this pass's candidate count and per-source offsets
         const uint64_t n_pass = pass_count[pi];
         const uint64_t* off = nullptr;
@@ -2019,13 +2227,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                 int tg = kt_begin(c, K_GENERATE, ((double)sizeof(RC) + 2.0) * n_pass + 16.0 * ns);
                 k_generate_compact<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(
                     H, sk, sv, ns, lambda, coeff, row_lo, row_hi, off, c->d_sdec, CompactOut{A, bk, hp}, n_pass,
-                    dterm, c->d_flags);
+                    dterm, c->d_flags, tabs);
                 kt_end(c, tg);
             } else {
                 int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
                 k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
                                                                             row_hi, off, c->d_sdec, A, n_pass,
-                                                                            dterm, c->d_flags);
+                                                                            dterm, c->d_flags, tabs);
                 kt_end(c, tg);
             }
             HIPCHK(hipGetLastError());
@@ -2249,7 +2457,7 @@ void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64**
     double* dt = c->dterm.get<double>(ns);
     k_fill<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(ones, ns, 1.0);
     k_generate<double><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ones, ns, 0.0, 1.0, 0, H.dim, nullptr,
-                                                                      c->d_sdec, r0, n, dt, c->d_flags);
+                                                                      c->d_sdec, r0, n, dt, c->d_flags, SsTabs{});
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     const uint32_t fl = flags_read_clear(c);

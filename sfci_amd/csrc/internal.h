@@ -231,6 +231,30 @@ struct HamDev {
     uint32_t n_opair, n_vpair;
 };
 
+// Same-spin double of one spin string (spin s, spatial i < j -> a < b).  Its element in a
+// determinant whose other-spin string is T is (popc(T & m) odd ? -h : h): h is the element
+// with the other spin empty (the magnitude and the sign from spin-s electrons depend only on
+// the spin-s string), m the other-spin orbitals that lie strictly between an odd number of
+// the excitation's two orbital pairs (the parity the other spin adds to both sign factors).
+struct SsEnt {
+    uint32_t r;   // colex rank of the excited spin-s string
+    uint32_t m;
+    double h;
+};
+
+// Same-spin double tables of one H.v, device pointers (a: spin 0, b: spin 1; null = the
+// generator enumerates that class itself).  Spin 0: string u's entries at a + u * n_daa,
+// grouped by target pass, pass p's at [a_start[u * (n_pass + 1) + p], ... + p + 1)
+// (a_start null when n_pass == 1); a_idx[j] / b_idx[j]: source j's string index.
+struct SsTabs {
+    const SsEnt* a = nullptr;
+    const uint32_t* a_start = nullptr;
+    const uint32_t* a_idx = nullptr;
+    const SsEnt* b = nullptr;
+    const uint32_t* b_idx = nullptr;
+    int pass = 0, n_pass = 1;
+};
+
 // Per-kernel timing with HIP events on the library stream (resolved after each call).
 enum KernelId : int {
     K_GENERATE = 0,
@@ -250,6 +274,7 @@ enum KernelId : int {
     K_MLP_SCORE,       // k_mlp_fwd on screened candidates (fused three-layer forward, BF16 MFMA)
     K_SCREEN_SELECT,   // score histograms and the kept-candidate compaction of the screened H.v
     K_MLP_TRAIN,       // one training step of the screening network (forward, backward, Adam)
+    K_SS_TABLE,        // k_ss_table + string indexing: same-spin double tables of the source strings
     K_NUM
 };
 
@@ -306,6 +331,9 @@ struct Ctx {
     DevBuf scratch_scan[8];
     DevBuf rowhist;
     DevBuf plan_k[2], plan_v[2], plan_fl, plan_pos, plan_first, plan_str, plan_cnt;   // row-grouped planning
+    DevBuf ss_tab[2], ss_start, ss_idx[2];   // same-spin double tables and per-source string indices
+    DevBuf bstr_k[2], bstr_v[2], bstr_fl, bstr_pos, bstr_str;   // spin-1 string table
+    int gen_tables = 0;              // same-spin double tables: 0 auto, 1 never, 2 always
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
     double pass_load = 0.9;          // distinct targets per LDS-table bucket aimed at, in table slots
@@ -329,6 +357,7 @@ struct Ctx {
     struct {
         uint64_t n_src = 0, n_candidates = 0, n_unique = 0, n_out = 0, n_passes = 0, n_fallback = 0;
         uint64_t n_split = 0;        // hash-bucket classes split because an LDS table filled
+        uint64_t n_str0 = 0, n_str1 = 0;   // distinct source strings with same-spin tables (0: none)
         int hash_bits = 0;
         double ms_generate = 0, ms_sort = 0, ms_reduce = 0, ms_select = 0, ms_total = 0;
         uint64_t n_kept = 0;         // screened: candidates kept for the merge

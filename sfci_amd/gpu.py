@@ -39,7 +39,10 @@ class StatsT(C.Structure):
                 ("ms_sort", C.c_double), ("ms_reduce", C.c_double), ("ms_select", C.c_double),
                 ("ms_total", C.c_double), ("n_fallback", C.c_uint64), ("hash_bits", C.c_int),
                 ("n_kept", C.c_uint64), ("screen_eps", C.c_double), ("ms_score", C.c_double),
-                ("n_split", C.c_uint64)]
+                ("n_split", C.c_uint64), ("n_str0", C.c_uint64), ("n_str1", C.c_uint64)]
+
+
+KERNEL_COUNT = 18   # SFCI_KERNEL_COUNT
 
 
 class KernelStatT(C.Structure):
@@ -98,6 +101,7 @@ def lib():
         "sfci_set_workspace": (ip, [vp, u64]),
         "sfci_hv_stats": (ip, [vp, C.POINTER(StatsT)]),
         "sfci_set_merge_mode": (ip, [vp, ip, ip]),
+        "sfci_set_gen_tables": (ip, [vp, ip]),
         "sfci_set_precision": (ip, [vp, ip]),
         "sfci_sync": (ip, [vp]),
         "sfci_kernel_stats": (ip, [vp, C.POINTER(KernelStatT), ip, ip]),
@@ -270,13 +274,17 @@ class Context:
     def set_merge_mode(self, mode: int, hash_bits: int = 0):
         self._check(lib().sfci_set_merge_mode(self._h, mode, hash_bits))
 
+    def set_gen_tables(self, mode: int):
+        """0 automatic, 1 every source evaluates its own same-spin doubles, 2 per-string tables"""
+        self._check(lib().sfci_set_gen_tables(self._h, int(mode)))
+
     def stats(self) -> dict:
         s = StatsT()
         self._check(lib().sfci_hv_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in StatsT._fields_}
 
     def kernel_stats(self, reset: bool = False) -> dict:
-        n = 17
+        n = KERNEL_COUNT
         arr = (KernelStatT * n)()
         self._check(lib().sfci_kernel_stats(self._h, arr, n, 1 if reset else 0))
         return {lib().sfci_kernel_name(i).decode(): dict(ms=arr[i].ms, launches=arr[i].launches,
