@@ -1565,6 +1565,7 @@ constexpr int BR_TARGET = BR_SLOTS * 3 / 5;    // targets per class aimed at whe
 constexpr int BR_STACK = 256;                  // pending classes of one bucket
 constexpr uint32_t BR_MAX_CLASSES = 4096;      // classes of one bucket beyond which the pass takes the full sort
 static_assert(BR_TAGS <= BR_THREADS, "the tag list is finished one entry per thread");
+static_assert(BR_SLOTS % 4 == 0, "compact-key tables are probed in aligned groups of four slots");
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
@@ -1602,7 +1603,11 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
     unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags,
     unsigned long long* split_count, uint32_t fill_max) {
-    __shared__ unsigned long long t_key[BR_SLOTS];
+    // table keys: compact records carry a 31-bit key, stored as one 32-bit word (four slots
+    // per 128-bit LDS read); full-key records a 64-bit dense index
+    using KT = typename std::conditional<CR, uint32_t, unsigned long long>::type;
+    constexpr KT EMPTYK = CR ? (KT)0xFFFFFFFFu : (KT)BR_EMPTY;
+    __shared__ __align__(16) KT t_key[BR_SLOTS];
     __shared__ unsigned long long t_l0[BR_SLOTS], t_l1[BR_SLOTS], t_l2[BR_SLOTS];
     __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];   // tagged, finished by search
     __shared__ uint32_t t_tl[(BR_SLOTS + 31) / 32];    // tagged, finished from the tag list
@@ -1675,7 +1680,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 for (int r = 0; r < BRB; ++r) {
                     if (k_[r] == ~0ull) continue;
                     const uint64_t raw = k_[r];
-                    const unsigned long long key = raw >> 1;
+                    const KT key = (KT)(raw >> 1);
                     uint32_t s16, c16;   // 16 hash bits for the slot, the next 16 for the class
                     if constexpr (CR) {
                         // the key's rb bits mixed into 32 (a bijection mod 2^32): the top 16
@@ -1697,24 +1702,68 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     // record per connected source): a plain read finds it, and only an empty
                     // slot costs a compare-and-swap.  A slot's key never changes once set.
                     int probe = 0;
+                    if constexpr (CR) {
+                        // linear probing four slots at a time: one 128-bit read of the aligned
+                        // group holding `slot`; the first of (our key, an empty slot) at or
+                        // after `slot` in probe order decides
+                        typedef uint32_t k4_t __attribute__((ext_vector_type(4)));
+                        for (;;) {
+                            if ((probe & 15) == 15 &&
+                                __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                                probe = BR_SLOTS;   // the class is being abandoned: give up the search
+                                break;
+                            }
+                            if (probe * 4 > BR_SLOTS + 4) {   // every group seen
+                                probe = BR_SLOTS;
+                                break;
+                            }
+                            ++probe;
+                            const uint32_t g = slot & ~3u, q0 = slot & 3u;
+                            const k4_t kk = *(volatile const k4_t*)(t_key + g);
+                            uint32_t hit = 4, emp = 4;
+#pragma unroll
+                            for (int q = 3; q >= 0; --q) {
+                                if ((uint32_t)q >= q0) {
+                                    if (kk[q] == key) hit = (uint32_t)q;
+                                    if (kk[q] == EMPTYK) emp = (uint32_t)q;
+                                }
+                            }
+                            if (hit < emp) {
+                                slot = g + hit;
+                                break;
+                            }
+                            if (emp < 4) {
+                                slot = g + emp;
+                                const KT prev = atomicCAS(&t_key[slot], EMPTYK, key);
+                                if (prev == EMPTYK) {
+                                    if (atomicAdd(&s_fill, 1u) + 1u >= fill_max) s_full = 1;
+                                    break;
+                                }
+                                if (prev == key) break;
+                                slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;   // taken by another key
+                                continue;
+                            }
+                            slot = (g + 4 >= (uint32_t)BR_SLOTS) ? 0u : g + 4;
+                        }
+                    } else {
                     for (; probe < BR_SLOTS; ++probe) {
                         if ((probe & 63) == 63 &&
                             __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
                             probe = BR_SLOTS;   // the class is being abandoned: give up the search
                             break;
                         }
-                        const unsigned long long cur =
-                            __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const KT cur = __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (cur == key) break;
-                        if (cur == BR_EMPTY) {
-                            const unsigned long long prev = atomicCAS(&t_key[slot], BR_EMPTY, key);
-                            if (prev == BR_EMPTY) {
+                        if (cur == EMPTYK) {
+                            const KT prev = atomicCAS(&t_key[slot], EMPTYK, key);
+                            if (prev == EMPTYK) {
                                 if (atomicAdd(&s_fill, 1u) + 1u >= fill_max) s_full = 1;
                                 break;
                             }
                             if (prev == key) break;
                         }
                         slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
+                    }
                     }
                     if (probe == BR_SLOTS) {
                         s_full = 1;
@@ -1747,7 +1796,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             if (st + STEP <= en) load_full(st, ka, va);
             else load_round(st, ka, va);
             for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
-                t_key[i] = BR_EMPTY;
+                t_key[i] = EMPTYK;
                 t_l0[i] = 0;
                 t_l1[i] = 0;
                 t_l2[i] = 0;
@@ -1823,8 +1872,8 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 em[r] = false;
                 yv[r] = 0.0;
                 dk[r] = 0;
-                const unsigned long long key = slot < BR_SLOTS ? t_key[slot] : BR_EMPTY;
-                if (key != BR_EMPTY) {
+                const KT key = slot < BR_SLOTS ? t_key[slot] : EMPTYK;
+                if (key != EMPTYK) {
                     ++uniq;
                     if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
                         const uint32_t tag = (t_tag[slot >> 5] >> (slot & 31)) & 1u;

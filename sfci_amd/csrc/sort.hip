@@ -752,6 +752,34 @@ constexpr int pthreads() {
     return ptile<R>() / PART_ITEMS;
 }
 
+// A tile of records in registers.
+template <typename R>
+struct TileRegs {
+    static constexpr int NW = sizeof(R) / 4;   // records held as 32-bit words
+    uint32_t x[PART_ITEMS][NW];
+    uint16_t d[PART_ITEMS];                    // hash bucket
+};
+
+template <typename R>
+__device__ __forceinline__ void tile_load(const R* __restrict__ rin, const uint16_t* __restrict__ bkt, size_t base,
+                                          uint32_t len, TileRegs<R>& t) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < PART_ITEMS; ++r) {
+        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
+        if (li < len) {
+            const uint32_t* src = (const uint32_t*)(rin + base + li);
+#pragma unroll
+            for (int q = 0; q < TileRegs<R>::NW; ++q) t.x[r][q] = src[q];
+            t.d[r] = bkt[base + li];
+        }
+    }
+}
+
+// First level: one persistent workgroup per CU (a tile's LDS staging takes most of the CU's
+// LDS, so one workgroup runs per CU): while a tile is ranked and written out, the next
+// tile's records are already loaded into registers, so the reads overlap the writes.
+constexpr size_t PART_PERSIST = 256;
 template <typename R>
 __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* __restrict__ rin,
                                                                       const uint16_t* __restrict__ bkt, size_t n,
@@ -766,67 +794,67 @@ __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* _
     __shared__ uint32_t s_lstart[RADIX], s_cnt[RADIX], s_ok[RADIX];
     __shared__ uint32_t s_wsum[RADIX / SFCI_WAVE];
     const int tid = threadIdx.x;
-    const size_t base = (size_t)blockIdx.x * ptile<R>();
-    const uint32_t len = (uint32_t)min((size_t)ptile<R>(), n - base);
     const uint32_t m2 = (1u << sh1) - 1u;
-    constexpr int NW = sizeof(R) / 4;   // records held as 32-bit words (kept in registers)
-    uint32_t x[PART_ITEMS][NW];
-    uint16_t bk[PART_ITEMS];
+    const size_t n_tiles = (n + ptile<R>() - 1) / ptile<R>();
+    constexpr int NW = TileRegs<R>::NW;
+    TileRegs<R> cur, nxt;
     uint32_t rank[PART_ITEMS];
+    size_t t = blockIdx.x;
+    if (t < n_tiles) tile_load(rin, bkt, t * ptile<R>(), (uint32_t)min((size_t)ptile<R>(), n - t * ptile<R>()), cur);
+    for (; t < n_tiles; t += gridDim.x) {
+        const size_t base = t * ptile<R>();
+        const uint32_t len = (uint32_t)min((size_t)ptile<R>(), n - base);
+        if (tid < RADIX) s_cnt[tid] = 0;
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
-        if (li < len) {
-            const uint32_t* src = (const uint32_t*)(rin + base + li);
-#pragma unroll
-            for (int q = 0; q < NW; ++q) x[r][q] = src[q];
-            bk[r] = bkt[base + li];
+        for (int r = 0; r < PART_ITEMS; ++r) {
+            const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
+            rank[r] = 0;
+            if (li < len) rank[r] = atomicAdd(&s_cnt[(uint32_t)cur.d[r] >> sh1], 1u);
         }
-    }
-    if (tid < RADIX) s_cnt[tid] = 0;
-    __syncthreads();
+        __syncthreads();
+        const uint32_t h = tid < RADIX ? s_cnt[tid] : 0u;
+        const uint32_t ls = digit_scan(h, s_wsum);
+        if (tid < RADIX) {
+            s_lstart[tid] = ls;
+            uint32_t ok = 1;
+            if (h) {
+                const unsigned long long g = atomicAdd(&cursor[tid], (unsigned long long)h);
+                s_gbase[tid] = g;
+                if (g + h > rend[tid]) {
+                    ok = 0;
+                    atomicOr(flags, FLAG_REGION_FULL);
+                }
+            }
+            s_ok[tid] = ok;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
-        rank[r] = 0;
-        if (li < len) rank[r] = atomicAdd(&s_cnt[(uint32_t)bk[r] >> sh1], 1u);
-    }
-    __syncthreads();
-    const uint32_t h = tid < RADIX ? s_cnt[tid] : 0u;
-    const uint32_t ls = digit_scan(h, s_wsum);
-    if (tid < RADIX) {
-        s_lstart[tid] = ls;
-        uint32_t ok = 1;
-        if (h) {
-            const unsigned long long g = atomicAdd(&cursor[tid], (unsigned long long)h);
-            s_gbase[tid] = g;
-            if (g + h > rend[tid]) {
-                ok = 0;
-                atomicOr(flags, FLAG_REGION_FULL);
+        for (int r = 0; r < PART_ITEMS; ++r) {
+            const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
+            if (li < len) {
+                const uint32_t p = s_lstart[(uint32_t)cur.d[r] >> sh1] + rank[r];
+                uint32_t* dst = (uint32_t*)(s_rec + p);
+#pragma unroll
+                for (int q = 0; q < NW; ++q) dst[q] = cur.x[r][q];
+                s_bk[p] = cur.d[r];
             }
         }
-        s_ok[tid] = ok;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < PART_ITEMS; ++r) {
-        const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
-        if (li < len) {
-            const uint32_t p = s_lstart[(uint32_t)bk[r] >> sh1] + rank[r];
-            uint32_t* dst = (uint32_t*)(s_rec + p);
-#pragma unroll
-            for (int q = 0; q < NW; ++q) dst[q] = x[r][q];
-            s_bk[p] = bk[r];
+        __syncthreads();
+        // the next tile's loads go out before this tile's stores
+        const size_t tn = t + gridDim.x;
+        if (tn < n_tiles)
+            tile_load(rin, bkt, tn * ptile<R>(), (uint32_t)min((size_t)ptile<R>(), n - tn * ptile<R>()), nxt);
+        for (uint32_t p = tid; p < len; p += pthreads<R>()) {
+            const uint32_t b = s_bk[p], d = b >> sh1;
+            if (s_ok[d]) {
+                const uint64_t at = s_gbase[d] + (p - s_lstart[d]);
+                rout[at] = s_rec[p];
+                dout[at] = (uint8_t)(b & m2);
+            }
         }
-    }
-    __syncthreads();
-    for (uint32_t p = tid; p < len; p += pthreads<R>()) {
-        const uint32_t b = s_bk[p], d = b >> sh1;
-        if (s_ok[d]) {
-            const uint64_t at = s_gbase[d] + (p - s_lstart[d]);
-            rout[at] = s_rec[p];
-            dout[at] = (uint8_t)(b & m2);
-        }
+        cur = nxt;
+        __syncthreads();   // LDS reuse by the next tile
     }
 }
 
@@ -917,7 +945,7 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
     const size_t n_tiles = (n + ptile<R>() - 1) / ptile<R>();
     uint8_t* dig = c->part_dig.get<uint8_t>(part_capacity(n));
     int t1 = kt_begin(c, K_PART_SCATTER1, (2.0 * sizeof(R) + 3.0) * n);
-    k_part_scatter_cur_c<R><<<(unsigned)n_tiles, pthreads<R>(), 0, c->stream>>>(
+    k_part_scatter_cur_c<R><<<(unsigned)std::min<size_t>(n_tiles, PART_PERSIST), pthreads<R>(), 0, c->stream>>>(
         A, bkt, n, B, sh1, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig);
     kt_end(c, t1);
     HIPCHK(hipGetLastError());
