@@ -653,48 +653,44 @@ struct SinkOf<float, true, false> {
 
 constexpr int B2_UNROLL = 4;
 
-// Single-excitation elements of one source, n_items of them, each evaluated by a group of
-// 2^lg lanes (2^lg >= occupied spin orbitals, groups never span a wavefront): lane l of a
-// group adds the Coulomb and exchange terms of occupied orbitals l, l + 2^lg, ... in limb
-// form, the limbs are combined exactly across the group, and the group's first lane adds
-// h_mn and writes the record.  Same value as single_elem (an exact sum is order-free).
-// item(t, n, m, key) gives item t's annihilated/created spin orbitals and record key.
+// spin-orbital integral (m n | p q) for spin-allowed index sets (the caller knows the spins
+// match), 32-bit indexing: four blocks of n_orb^4 <= 2^20 entries
+__device__ __forceinline__ double g_so_allowed(const HamDev& H, int m, int n, int p, int q) {
+    const uint32_t N = (uint32_t)H.n_orb;
+    const uint32_t blk = (uint32_t)(((m & 1) << 1) | (p & 1));
+    return H.g[blk * (uint32_t)H.g_stride +
+               ((((uint32_t)m >> 1) * N + ((uint32_t)n >> 1)) * N + ((uint32_t)p >> 1)) * N + ((uint32_t)q >> 1)];
+}
+
+// Single-excitation elements of one source, n_items of them, one lane each (a source has at
+// most k*v <= 256 of each spin): the lane adds h_mn and, over the occupied spin orbitals p != n,
+// (m n | p p) and, for p of m's spin, -(m p | p n), in limb form; the same exact sum as
+// single_elem.  item(t, n, m, key) gives item t's annihilated/created spin orbitals and
+// record key.
 template <typename T, typename Item, typename Sink>
-__device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, uint32_t n_items, int lg, double temp,
+__device__ __forceinline__ void emit_singles(const HamDev& H, const SrcLds& s, uint32_t n_items, double temp,
                                              uint64_t pos, Item item, Sink& sink, bool& range_bad, bool& wbad) {
-    const int L = 1 << lg;
-    const int gl = threadIdx.x & (L - 1);
-    const uint32_t grp = threadIdx.x >> lg, n_grp = SFCI_BLOCK >> lg;
     const int no = s.n_occ_so;
-    for (uint32_t t0 = 0; t0 < n_items; t0 += n_grp) {   // block-uniform trip count
-        const uint32_t t = t0 + grp;
-        const bool act = t < n_items;
+    for (uint32_t t = threadIdx.x; t < n_items; t += SFCI_BLOCK) {
         int n = 0, m = 0;
         uint64_t key = 0;
-        if (act) item(t, n, m, key);
+        item(t, n, m, key);
         FxLimbs acc = fxl_zero();
-        bool ok = true;
-        if (act) {
-            for (int a = gl; a < no; a += L) {
-                const int p = s.occ_so[a];
-                if (p == n) continue;
-                ok &= fxl_add(acc, g_so(H, m, n, p, p));
-                ok &= fxl_add(acc, -g_so(H, m, p, p, n));
-            }
+        bool ok = fxl_add(acc, h_so(H, m, n));
+        const int sm = m & 1;
+#pragma unroll 2
+        for (int a = 0; a < no; ++a) {
+            const int p = s.occ_so[a];
+            if (p == n) continue;
+            const double coul = g_so_allowed(H, m, n, p, p);
+            const double exch = (p & 1) == sm ? g_so_allowed(H, m, p, p, n) : 0.0;
+            ok &= fxl_add(acc, coul);
+            ok &= fxl_add(acc, -exch);
         }
-        for (int o = L >> 1; o > 0; o >>= 1) {   // at most 64 terms per limb: no overflow
-            acc.l0 += __shfl_xor(acc.l0, o, SFCI_WAVE);
-            acc.l1 += __shfl_xor(acc.l1, o, SFCI_WAVE);
-            acc.l2 += __shfl_xor(acc.l2, o, SFCI_WAVE);
-            ok = (__shfl_xor(ok ? 1 : 0, o, SFCI_WAVE) != 0) && ok;
-        }
-        if (act && gl == 0) {
-            ok &= fxl_add(acc, h_so(H, m, n));
-            const double hij = ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
-            const T val = cand_val<T>(hij, temp);
-            range_bad |= !fx_in_range((double)val);
-            wbad |= !sink.put(pos + t, key, val, 1u, s.key ^ (1ull << n) ^ (1ull << m));
-        }
+        const double hij = ok ? sign_between(s.key, m, n) * fx_to(fxl_join(acc)) : __builtin_nan("");
+        const T val = cand_val<T>(hij, temp);
+        range_bad |= !fx_in_range((double)val);
+        wbad |= !sink.put(pos + t, key, val, 1u, s.key ^ (1ull << n) ^ (1ull << m));
     }
 }
 
@@ -762,7 +758,6 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
         const double temp = s.temp;
-        const int lg_occ = s.n_occ_so <= 1 ? 0 : min(6, 32 - (int)__clz(s.n_occ_so - 1));
         uint64_t pos = off ? off[j] : (uint64_t)j * H.n_conn;
         if (s.row_in) {
             const uint64_t rowbase = s.rank[0] * H.dim - base_dense;
@@ -773,7 +768,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             }
             pos += 1;
             // A1
-            emit_singles<T>(H, s, nsa, lg_occ, temp, pos,
+            emit_singles<T>(H, s, nsa, temp, pos,
                             [&](uint32_t f, int& n, int& m, uint64_t& key) {
                                 const uint16_t dd = sdec[f];
                                 n = 2 * s.occ[1][dd & 0xff] + 1;
@@ -816,7 +811,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         }
         // B1
         const uint32_t na = (uint32_t)s.n_alist;
-        emit_singles<T>(H, s, na, lg_occ, temp, pos,
+        emit_singles<T>(H, s, na, temp, pos,
                         [&](uint32_t f, int& n, int& m, uint64_t& key) {
                             const uint32_t aw = s.aw[f];
                             n = 2 * (int)((aw >> 10) & 31u);
