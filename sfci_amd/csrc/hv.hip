@@ -1690,7 +1690,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         c16 = (uint32_t)(hf >> cls_shift) & 0xffffu;
                     }
                     // class cq of cm: the cm-th part of the 16-bit class value's range
-                    if ((uint32_t)((c16 * cm) >> 16) != cq) continue;
+                    if (cm > 1 && (uint32_t)((c16 * cm) >> 16) != cq) continue;
                     ++n_ins;
                     uint32_t slot = (uint32_t)((s16 * (uint32_t)BR_SLOTS) >> 16);
                     // most records meet a key already in the table (a target collects one
@@ -1714,7 +1714,10 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                             }
                             ++probe;
                             const uint32_t g = slot & ~3u, q0 = slot & 3u;
-                            const k4_t kk = *(volatile const k4_t*)(t_key + g);
+                            // read through an LDS-space pointer (ds_read_b128): a generic pointer
+                            // would make it a flat load that waits for all outstanding global loads
+                            const k4_t kk = *(volatile const __attribute__((address_space(3))) k4_t*)(
+                                (const __attribute__((address_space(3))) uint32_t*)(t_key + g));
                             uint32_t hit = 4, emp = 4;
 #pragma unroll
                             for (int q = 3; q >= 0; --q) {
@@ -1777,11 +1780,12 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         }
                         continue;
                     }
-                    // zero limbs are skipped (l2 is zero for every |term| < 1)
+                    // l0 and l1 are added unconditionally (a zero add costs less than the
+                    // branch); l2 is zero for every |term| < 1 and is skipped then
                     unsigned long long l0, l1, l2;
                     limb_split((double)v_[r], l0, l1, l2);
-                    if (l0) atomicAdd(&t_l0[slot], l0);
-                    if (l1) atomicAdd(&t_l1[slot], l1);
+                    atomicAdd(&t_l0[slot], l0);
+                    atomicAdd(&t_l1[slot], l1);
                     if (l2) atomicAdd(&t_l2[slot], l2);
                 }
             };
