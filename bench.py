@@ -84,27 +84,63 @@ def cpu_baseline(w, keys, vals, n_conn, target_s=15.0, max_threads=16):
                        "oracle H.v with %d OpenMP threads: %.2f s" % (s, len(keys), n_conn, threads, dt))
 
 
+def fixed_vector_precision(ctx, v, nd, cap):
+    """One H.v of the same vector in FP64 and in mixed precision: diag = <v|H v> of each
+    (the energy for a normalized v) and how far the mixed product's entries lie from the
+    FP64 ones.  No truncation path is involved: the difference is the FP32 rounding of the
+    candidate products alone."""
+    out = ctx.vector(cap)
+    res = {}
+    vals = {}
+    for tag, mixed in (("fp64", False), ("mixed", True)):
+        ctx.set_precision(mixed)
+        try:
+            d = ctx.hv(v, out, nd, 0.0, 1.0)
+        finally:
+            ctx.set_precision(False)
+        k, y = out.download()
+        res["energy_" + tag] = d
+        vals[tag] = (k, y)
+    (k64, y64), (k32, y32) = vals["fp64"], vals["mixed"]
+    common, i64, i32 = np.intersect1d(k64, k32, assume_unique=True, return_indices=True)
+    dy = np.abs(y32[i32] - y64[i64])
+    res.update(energy_mixed_minus_fp64=res["energy_mixed"] - res["energy_fp64"],
+               n_fp64=int(len(k64)), n_mixed=int(len(k32)), n_common=int(len(common)),
+               max_abs_entry_diff=float(dy.max()) if len(dy) else 0.0,
+               max_rel_entry_diff=float((dy / np.abs(y64[i64])).max()) if len(dy) else 0.0)
+    return res
+
+
 def time_to_e(ctx, w, nd, args):
     """The reference's ground-state solve (truncated Lanczos, restarts until the Ritz
     value drops by less than 1e-5 Ha) on the GPU; optionally the CPU oracle's solve."""
     mixed = args.precision == "mixed"
     w = dict(w, nl=args.nl or w["nl"], restarts=args.restarts or w["restarts"])
     fp64_ref = None
-    if mixed:
+    fixed = None
+    if mixed and args.fp64_energy is not None:
+        # the plain FP64 solve's energy from a separate run (a solve pair too long for one job)
+        fp64_ref = dict(e_final=args.fp64_energy, seconds=None, restarts=None, given=True)
+    elif mixed:
         # the plain FP64 solve first: the mixed solve's energy is judged against it
         t = time.perf_counter()
         r64 = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap,
                                e_prev_init=float("inf"), verbose=args.verbose)
         fp64_ref = dict(seconds=time.perf_counter() - t, e_final=r64["e_final"], restarts=r64["restarts"])
         log("fp64 solve: %.2f s, E = %.12f" % (fp64_ref["seconds"], r64["e_final"]))
+    if mixed:
         ctx.set_precision(True)
+    v_fin = ctx.vector(nd) if args.fixed_vector else None
     t = time.perf_counter()
     # the synthetic workloads have positive energies; starting the stopping test from
     # +inf (instead of the reference's 0.0) makes the solve run until two restarts agree
     r = ctx.ground_state(w["init_hf"], nd, w["nl"], w["restarts"], hpsi_cap=args.cap, e_prev_init=float("inf"),
-                         verbose=args.verbose)
+                         verbose=args.verbose, v_out=v_fin)
     dt = time.perf_counter() - t
     ctx.set_precision(False)
+    if v_fin is not None:
+        fixed = fixed_vector_precision(ctx, v_fin, nd, args.cap)
+        log("fixed vector: %s" % fixed)
     res = dict(metric="time-to-E (truncated Lanczos until two restarts' Ritz values differ by < 1e-5 Ha)",
                value=dt, unit="s",
                higher_is_better=False, dtype="f32/f64" if mixed else "f64", data="synthetic",
@@ -112,10 +148,14 @@ def time_to_e(ctx, w, nd, args):
                            % (w["n_orb"], w["n_so"], w["n_elec"], w["seed"], nd, w["nl"], w["restarts"])),
                restarts=r["restarts"], ritz=list(r["ritz"]), e_final=r["e_final"], n_hv=r["n_hv"],
                ms_hv_total=r["ms_hv"])
+    res["ms_restart"] = [float(x) for x in r["ms_restart"]]
+    res["e_trunc"] = [float(x) for x in r["e_trunc"]]
+    if fixed:
+        res["fixed_vector"] = fixed
     if fp64_ref:
         res["fp64_solve"] = fp64_ref
         res["energy_error_vs_fp64"] = r["e_final"] - fp64_ref["e_final"]
-        res["speedup_vs_fp64"] = fp64_ref["seconds"] / dt
+        res["speedup_vs_fp64"] = fp64_ref["seconds"] / dt if fp64_ref["seconds"] else None
     if args.cpu:
         from oracle import oracle as O
         orc = O.Oracle(w["in_path"], w["n_so"], w["n_elec"])
@@ -540,6 +580,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="one plain device copy of 1 GiB read + 1 GiB written (counter calibration)")
     ap.add_argument("--nl", type=int, default=None)
+    ap.add_argument("--fp64-energy", type=float, default=None,
+                    help="with --time-to-e --precision mixed: the plain FP64 solve's energy from a separate run")
+    ap.add_argument("--fixed-vector", action="store_true",
+                    help="with --time-to-e: FP64 and mixed H.v of the solve's final vector (precision alone)")
     ap.add_argument("--restarts", type=int, default=None)
     ap.add_argument("--nd-train", type=int, default=50_000)
     ap.add_argument("--samples-per-seed", type=int, default=400_000)
