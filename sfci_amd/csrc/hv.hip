@@ -601,7 +601,7 @@ struct ScreenSink {
         const bool spill = i >= SCR_LDS;
         const uint64_t sbal = __ballot(spill);
         if (!spill) {
-            lds[i] = r;
+            recs_store<3>(lds + i, r);
             return true;
         }
         const int sl = __ffsll((long long)sbal) - 1;
@@ -610,7 +610,7 @@ struct ScreenSink {
         gbase = __shfl(gbase, sl, SFCI_WAVE);
         const unsigned long long gi = gbase + popc64(sbal & lt);
         if (gi >= so.cap) return false;
-        so.out[gi] = r;
+        recs_store<1>(so.out + gi, r);
         return true;
     }
     // block-uniform: copy the list out when it holds SCR_FLUSH_AT or more
@@ -629,7 +629,7 @@ struct ScreenSink {
         bool ok = true;
         for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
             const uint64_t gi = *s_base + i;
-            if (gi < so.cap) so.out[gi] = lds[i];
+            if (gi < so.cap) recs_store<1>(so.out + gi, lds[i]);
             else ok = false;
         }
         __syncthreads();

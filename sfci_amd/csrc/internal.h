@@ -463,6 +463,18 @@ struct alignas(8) RecS {
     float val;
     uint32_t cls;
 };
+#if defined(__HIPCC__)
+// Stores of a RecS through pointers of a known address space (LDS: 3, global: 1): through
+// a generic pointer the compiler emits flat stores.
+template <int AS>
+__device__ __forceinline__ void recs_store(RecS* p, const RecS& r) {
+    __attribute__((address_space(AS))) RecS* q = (__attribute__((address_space(AS))) RecS*)p;
+    q->key = r.key;
+    q->det = r.det;
+    q->val = r.val;
+    q->cls = r.cls;
+}
+#endif
 
 // Screening feature of a candidate value: log2|v| piecewise linear in the mantissa,
 // exponent + mantissa fraction of the float (one IEEE rounding; -150 for zero).
