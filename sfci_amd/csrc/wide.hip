@@ -165,15 +165,16 @@ __device__ __forceinline__ double wide_val(double hij, double temp) {
 }
 
 // Passes generated together by one launch (each into its own region of the record buffer).
-constexpr int W_BATCH = 8;
+constexpr int W_BATCH = 32;   // target passes generated by one launch, at most
 
 // Where a source's records go.  Single pass: record `idx` of source j at j * n_conn + idx
 // (every connection is kept; no atomics).  Several passes: a launch generates the passes
 // [p_lo, p_lo + nb), pass p_lo + r into region r (cap records at r * cap); each source's
 // records of each region are counted first and reserved with one global atomic per region
-// (rbase), and each wavefront then takes its slots from the region's workgroup cursor in
-// LDS with one LDS atomic.  Each record goes with its hash bucket; both are written with
-// streaming stores (read back only by the partition).
+// (rbase), and each record then takes its slot from the region's workgroup cursor in LDS
+// with one LDS atomic (the order inside a region is free: every sum is exact).  Each record
+// goes with its hash bucket; both are written with streaming stores (read back only by the
+// partition).
 struct WEmit {
     RecW* rec;
     uint16_t* bkt;
@@ -182,7 +183,6 @@ struct WEmit {
     uint64_t base;                       // single pass: j * n_conn
     const unsigned long long* rbase;     // LDS, per region
     unsigned* cursor;                    // LDS, per region; nullptr in single-pass mode
-    int nb;
     uint32_t* flags;
     __device__ __forceinline__ void store(uint64_t at, uint64_t lo, uint64_t hi, double val, uint64_t h) const {
         __builtin_nontemporal_store(lo, &rec[at].lo);
@@ -190,32 +190,19 @@ struct WEmit {
         __builtin_nontemporal_store(val, &rec[at].val);
         __builtin_nontemporal_store((uint16_t)bucket_of(h, hb), &bkt[at]);
     }
-    // called by every lane of a wavefront that is still in the enclosing loop; r is the
-    // record's region (-1: not in this launch's passes), h its placement hash
+    // r is the record's region (-1: not in this launch's passes), h its placement hash
     __device__ __forceinline__ void put(int r, uint32_t idx, uint64_t lo, uint64_t hi, double val,
                                         uint64_t h) const {
+        if (r < 0) return;
         if (!cursor) {
-            if (r < 0) return;
             const uint64_t i = base + idx;
             if (i < cap) store(i, lo, hi, val, h);
             else atomicOr(flags, FLAG_GEN_OVERFLOW);
             return;
         }
-        const int lane = threadIdx.x & (SFCI_WAVE - 1);
-        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        for (int g = 0; g < nb; ++g) {
-            const bool mine = r == g;
-            const uint64_t bal = __ballot(mine);
-            if (bal == 0) continue;
-            const int leader = __ffsll((long long)bal) - 1;
-            unsigned off = 0;
-            if (lane == leader) off = atomicAdd(&cursor[g], (unsigned)__popcll(bal));
-            off = __shfl(off, leader, SFCI_WAVE);
-            if (!mine) continue;
-            const uint64_t i = rbase[g] + off + (uint64_t)__popcll(bal & lt);
-            if (i < cap) store((uint64_t)g * cap + i, lo, hi, val, h);
-            else atomicOr(flags, FLAG_GEN_OVERFLOW);
-        }
+        const uint64_t i = rbase[r] + atomicAdd(&cursor[r], 1u);
+        if (i < cap) store((uint64_t)r * cap + i, lo, hi, val, h);
+        else atomicOr(flags, FLAG_GEN_OVERFLOW);
     }
 };
 
@@ -407,35 +394,23 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
         const double temp = s.temp;
         const uint32_t n_conn = (uint32_t)H.n_conn;
         bool bad = false;
-        WEmit em{rec, bkt, hb, cap, (uint64_t)j * n_conn, s_base, nullptr, nb, flags};
+        WEmit em{rec, bkt, hb, cap, (uint64_t)j * n_conn, s_base, nullptr, flags};
         if (n_pass > 1) {
-            // count this source's connections of each region, reserve them with one global
-            // atomic per region
-            unsigned cnt[W_BATCH];
-#pragma unroll
-            for (int g = 0; g < W_BATCH; ++g) cnt[g] = 0;
-            if (tid == 0) {
-                const int r = region(s.hx);
-#pragma unroll
-                for (int g = 0; g < W_BATCH; ++g) cnt[g] += r == g ? 1u : 0u;
-            }
-            for_connections(H, s, t_orb, [&](uint32_t, uint64_t, uint64_t, uint64_t hx, int, int, int, int) {
-                const int r = region(hx);
-#pragma unroll
-                for (int g = 0; g < W_BATCH; ++g) cnt[g] += r == g ? 1u : 0u;
-            });
+            // count this source's connections of each region (LDS atomics), reserve them with
+            // one global atomic per region
             if (tid < W_BATCH) {
                 s_cnt[tid] = 0;
                 s_cur[tid] = 0;
             }
             __syncthreads();
-#pragma unroll
-            for (int g = 0; g < W_BATCH; ++g) {
-                if (g >= nb) break;
-                unsigned c = cnt[g];
-                for (int off = SFCI_WAVE / 2; off > 0; off >>= 1) c += __shfl_down(c, off, SFCI_WAVE);
-                if (lane == 0 && c) atomicAdd(&s_cnt[g], c);
+            if (tid == 0) {
+                const int r = region(s.hx);
+                if (r >= 0) atomicAdd(&s_cnt[r], 1u);
             }
+            for_connections(H, s, t_orb, [&](uint32_t, uint64_t, uint64_t, uint64_t hx, int, int, int, int) {
+                const int r = region(hx);
+                if (r >= 0) atomicAdd(&s_cnt[r], 1u);
+            });
             __syncthreads();
             if (tid < nb) s_base[tid] = s_cnt[tid] ? atomicAdd(&count[tid], (unsigned long long)s_cnt[tid]) : 0ull;
             __syncthreads();
