@@ -535,17 +535,28 @@ void atomic_flags_restore(Ctx* c, uint32_t f);   // put back flags read by flags
 // l0 = trunc(rest * 2^42).  Every limb has the sign of x and |l1|, |l0| < 2^42.  Each limb
 // is added to its own 64-bit LDS word with an atomic whose result is unused.  A target
 // receives fewer than 2^21 terms (at most one per source), so no limb sum overflows; the
-// limbs are recombined exactly when the target is finished.
+// limbs are recombined exactly when the target is finished.  An integer t with |t| < 2^51
+// converts to int64 as the bits of t + 1.5 * 2^52 minus those of 1.5 * 2^52 (the sum is exact
+// and its ulp is 1), one add and one integer subtract instead of the generic conversion; the
+// integer part is formed only when |x| >= 1.
 __device__ __forceinline__ void limb_split(double x, unsigned long long& l0, unsigned long long& l1,
                                            unsigned long long& l2) {
-    if (!(fabs(x) < 4398046511104.0)) x = 0.0;   // out of range: flagged where the term was made
-    const double t2 = trunc(x);
-    const double r1 = (x - t2) * 0x1p42;
+    constexpr double MAGIC = 0x1.8p52;
+    const long long mb = __double_as_longlong(MAGIC);
+    double r1;
+    l2 = 0;
+    if (fabs(x) < 1.0) {
+        r1 = x * 0x1p42;
+    } else {
+        if (!(fabs(x) < 4398046511104.0)) x = 0.0;   // out of range: flagged where the term was made
+        const double t2 = trunc(x);
+        l2 = (unsigned long long)(__double_as_longlong(t2 + MAGIC) - mb);
+        r1 = (x - t2) * 0x1p42;
+    }
     const double t1 = trunc(r1);
     const double t0 = trunc((r1 - t1) * 0x1p42);
-    l2 = (unsigned long long)(long long)t2;
-    l1 = (unsigned long long)(long long)t1;
-    l0 = (unsigned long long)(long long)t0;
+    l1 = (unsigned long long)(__double_as_longlong(t1 + MAGIC) - mb);
+    l0 = (unsigned long long)(__double_as_longlong(t0 + MAGIC) - mb);
 }
 
 __device__ __forceinline__ fx128 limb_join(unsigned long long l0, unsigned long long l1, unsigned long long l2) {
