@@ -151,7 +151,10 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         HIPCHK(hipMalloc(&c.d_g, g.size() * 8));
         HIPCHK(hipMalloc(&c.d_gt, gt.size() * 8));
         HIPCHK(hipMemcpy(c.d_gt, gt.data(), gt.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc(&c.d_binom, c.h_binom.size() * 8));
+        // the device's table: rows n <= 32 in 32 bits (C(32, 16) < 2^32)
+        std::vector<uint32_t> b32(33 * 33);
+        for (int i = 0; i < 33 * 33; ++i) b32[i] = (uint32_t)c.h_binom[i];
+        HIPCHK(hipMalloc(&c.d_binom, b32.size() * 4));
         HIPCHK(hipMalloc(&c.d_occ_pairs, op.size() * 2));
         HIPCHK(hipMalloc(&c.d_vir_pairs, vp.size() * 2));
         HIPCHK(hipMalloc(&c.d_sdec, sd.size() * 2));
@@ -161,7 +164,7 @@ int build_ctx(int n_so, int n_elec, const std::vector<Entry>& ent, int device, s
         HIPCHK(hipMalloc(&c.d_counter, 8 * 8));
         HIPCHK(hipMemcpy(c.d_h, h.data(), h.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c.d_g, g.data(), g.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c.d_binom, c.h_binom.data(), c.h_binom.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c.d_binom, b32.data(), b32.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c.d_occ_pairs, op.data(), op.size() * 2, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c.d_vir_pairs, vp.data(), vp.size() * 2, hipMemcpyHostToDevice));
         HIPCHK(hipMemset(c.d_acc, 0, 16 * 8));

@@ -237,8 +237,10 @@ __host__ __device__ __forceinline__ uint64_t target_hash(uint64_t t, int hb) {
 }
 
 // colex rank of a k-subset given as a bit string over spatial orbitals:
-// sum_t C(pos_t, t+1), pos_t ascending.  binom is a row-major [65][33] table.
-__host__ __device__ __forceinline__ uint64_t colex_rank(uint32_t s, const uint64_t* binom) {
+// sum_t C(pos_t, t+1), pos_t ascending.  binom is a row-major table with 33 columns (the
+// host's [65][33] in 64 bits, the device's [33][33] in 32 bits).
+template <typename BT>
+__host__ __device__ __forceinline__ uint64_t colex_rank(uint32_t s, const BT* binom) {
     uint64_t r = 0;
     int t = 1;
     while (s) {
@@ -251,7 +253,8 @@ __host__ __device__ __forceinline__ uint64_t colex_rank(uint32_t s, const uint64
 }
 
 // inverse of colex_rank for k-subsets of n_orb orbitals
-__host__ __device__ __forceinline__ uint32_t colex_unrank(uint64_t r, int k, int n_orb, const uint64_t* binom) {
+template <typename BT>
+__host__ __device__ __forceinline__ uint32_t colex_unrank(uint64_t r, int k, int n_orb, const BT* binom) {
     uint32_t s = 0;
     int pos = n_orb - 1;
     for (int t = k; t >= 1; --t) {

@@ -39,7 +39,7 @@ __device__ __forceinline__ double h_so(const HamDev& H, int m, int n) {
 constexpr int LDS_BINOM = SFCI_MAX_ORB * 33;
 constexpr int LDS_PAIRS = SFCI_MAX_ORB * (SFCI_MAX_ORB - 1) / 2;
 
-__device__ __forceinline__ HamDev stage_tables(const HamDev& H, uint64_t* s_binom, uint16_t* s_op, uint16_t* s_vp) {
+__device__ __forceinline__ HamDev stage_tables(const HamDev& H, uint32_t* s_binom, uint16_t* s_op, uint16_t* s_vp) {
     for (int i = threadIdx.x; i < H.n_orb * 33; i += blockDim.x) s_binom[i] = H.binom[i];
     const int nop = H.k * (H.k - 1) / 2, nvp = H.v * (H.v - 1) / 2;
     for (int i = threadIdx.x; i < nop; i += blockDim.x) s_op[i] = H.occ_pairs[i];
@@ -51,6 +51,12 @@ __device__ __forceinline__ HamDev stage_tables(const HamDev& H, uint64_t* s_bino
     L.vir_pairs = s_vp;
     return L;
 }
+
+// The generators wait on table reads and record stores most of their cycles: they run at
+// eight wavefronts per SIMD (at most 64 VGPRs; their LDS allows eight 256-thread workgroups
+// per CU), which the compiler otherwise trades for a few more registers.  (The screened
+// generator's LDS staging allows seven.)
+#define GEN_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
 
 constexpr int MAX_SINGLES = (SFCI_MAX_ORB / 2) * (SFCI_MAX_ORB / 2);   // k*v <= 16*16 for 32 orbitals
 
@@ -285,7 +291,7 @@ __device__ __forceinline__ int pass_of(const uint64_t* bounds, int n_pass, uint6
 __global__ __launch_bounds__(SFCI_BLOCK) void k_count_passes(HamDev Hg, const uint64_t* __restrict__ astr, size_t n_src,
                                                              const uint64_t* __restrict__ bounds_g, int n_pass,
                                                              uint32_t* __restrict__ cnt) {
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     __shared__ uint64_t bounds[MAX_PASSES + 1];
     __shared__ uint32_t s_cnt[MAX_PASSES];
@@ -347,7 +353,7 @@ template <int SP>
 __global__ __launch_bounds__(SFCI_BLOCK) void k_ss_table(HamDev Hg, const uint64_t* __restrict__ strs, size_t n_str,
                                                          const uint64_t* __restrict__ bounds_g, int n_pass,
                                                          SsEnt* __restrict__ tab, uint32_t* __restrict__ pstart) {
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     __shared__ uint64_t bounds[MAX_PASSES + 1];
     __shared__ uint32_t s_cnt[MAX_PASSES + 1];
@@ -430,7 +436,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_rowgroup_hist(HamDev Hg, const u
                                                               const uint64_t* __restrict__ first, size_t n_src,
                                                               uint64_t row_lo, uint64_t row_hi, int grp_shift, int n_grp,
                                                               unsigned long long* __restrict__ hist) {
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     extern __shared__ unsigned long long s_h[];
@@ -736,7 +742,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
     }
     __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_ccur;
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     const HamDev H = stage_tables(Hg, s_binom, s_op, s_vp);
     const int tid = threadIdx.x, lane = tid & (SFCI_WAVE - 1), w = tid / SFCI_WAVE;
@@ -1061,7 +1067,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
 }
 
 template <typename T>
-__global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
+__global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_generate(HamDev Hg, const uint64_t* __restrict__ sk,
                                                          const double* __restrict__ sv, size_t n_src, double lambda,
                                                          double coeff, uint64_t row_lo, uint64_t row_hi,
                                                          const uint64_t* __restrict__ off,
@@ -1075,7 +1081,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate(HamDev Hg, const uint64
 
 // the same into compact records and their hash buckets
 template <typename T>
-__global__ __launch_bounds__(SFCI_BLOCK) void k_generate_compact(HamDev Hg, const uint64_t* __restrict__ sk,
+__global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_generate_compact(HamDev Hg, const uint64_t* __restrict__ sk,
                                                                  const double* __restrict__ sv, size_t n_src,
                                                                  double lambda, double coeff, uint64_t row_lo,
                                                                  uint64_t row_hi, const uint64_t* __restrict__ off,
@@ -1120,7 +1126,7 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_energy(HamDev Hg, const uint64_t
                                                        unsigned long long* acc_out, uint32_t* flags) {
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     __shared__ uint16_t s_op[LDS_PAIRS], s_vp[LDS_PAIRS];
     __shared__ uint16_t e_sa[MAX_SINGLES];
     __shared__ uint32_t e_lo[MAX_SINGLES], e_hi[MAX_SINGLES];
@@ -1956,7 +1962,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
 // Staged outputs carry dense indices (spin-0 rank * dim + spin-1 rank); rewrite them as
 // determinant keys, with the binomial table in LDS.
 __global__ __launch_bounds__(SFCI_BLOCK) void k_dense_to_det(HamDev Hg, uint64_t* __restrict__ k, size_t n) {
-    __shared__ uint64_t s_binom[LDS_BINOM];
+    __shared__ uint32_t s_binom[LDS_BINOM];
     for (int i = threadIdx.x; i < Hg.n_orb * 33; i += blockDim.x) s_binom[i] = Hg.binom[i];
     __syncthreads();
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {

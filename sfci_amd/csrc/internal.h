@@ -222,7 +222,7 @@ struct HamDev {
     const double* gt;      // [2][n_orb^4] opposite-spin blocks as [a][i][j][b], b fastest: gt[0] holds
                            // (a i | b j) of block (0,1), gt[gt_stride] holds (b j | a i) of block (1,0)
     uint64_t gt_stride;
-    const uint64_t* binom; // [65][33]
+    const uint32_t* binom; // [33][33] C(n, t), n <= 32 (every entry is below 2^32)
     const uint16_t* occ_pairs;  // C(k,2) entries (lo byte, hi byte) = positions in the occupied list
     const uint16_t* vir_pairs;  // C(v,2) entries = positions in the virtual list
     double constant;
@@ -310,7 +310,7 @@ struct Ctx {
     double* d_h = nullptr;
     double* d_g = nullptr;
     double* d_gt = nullptr;
-    uint64_t* d_binom = nullptr;
+    uint32_t* d_binom = nullptr;
     uint16_t* d_occ_pairs = nullptr;
     uint16_t* d_vir_pairs = nullptr;
     uint16_t* d_sdec = nullptr;      // single excitation x -> (occupied position | virtual position << 8)
