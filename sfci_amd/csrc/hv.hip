@@ -1676,7 +1676,27 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     }
                 }
             };
+            typedef uint32_t k4_t __attribute__((ext_vector_type(4)));
+            // the aligned four-slot group of compact keys holding slot; read through an LDS-space
+            // pointer (ds_read_b128): a generic pointer would make it a flat load that waits for
+            // all outstanding global loads
+            auto read_group = [&](uint32_t slot) -> k4_t {
+                return *(volatile const __attribute__((address_space(3))) k4_t*)(
+                    (const __attribute__((address_space(3))) uint32_t*)(t_key + (slot & ~3u)));
+            };
             auto insert_round = [&](const uint64_t* k_, const T* v_) {
+                // compact keys: the first group of every record of the round is read before any
+                // is probed, so that the reads' latencies overlap
+                k4_t kf[BRB];
+                uint32_t sf[BRB];
+                if constexpr (CR) {
+#pragma unroll
+                    for (int r = 0; r < BRB; ++r) {
+                        const uint32_t t = (uint32_t)(k_[r] >> 1) * 0x9E3779B1u;
+                        sf[r] = (uint32_t)(((t >> 16) * (uint32_t)BR_SLOTS) >> 16);
+                        if (k_[r] != ~0ull) kf[r] = read_group(sf[r]);
+                    }
+                }
 #pragma unroll
                 for (int r = 0; r < BRB; ++r) {
                     if (k_[r] == ~0ull) continue;
@@ -1705,9 +1725,9 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     int probe = 0;
                     if constexpr (CR) {
                         // linear probing four slots at a time: one 128-bit read of the aligned
-                        // group holding `slot`; the first of (our key, an empty slot) at or
-                        // after `slot` in probe order decides
-                        typedef uint32_t k4_t __attribute__((ext_vector_type(4)));
+                        // group holding `slot` (the first one read above); the first of (our
+                        // key, an empty slot) at or after `slot` in probe order decides.  A group
+                        // read earlier may be stale: a key inserted since is met by the CAS.
                         for (;;) {
                             if ((probe & 15) == 15 &&
                                 __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
@@ -1718,12 +1738,9 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                                 probe = BR_SLOTS;
                                 break;
                             }
-                            ++probe;
                             const uint32_t g = slot & ~3u, q0 = slot & 3u;
-                            // read through an LDS-space pointer (ds_read_b128): a generic pointer
-                            // would make it a flat load that waits for all outstanding global loads
-                            const k4_t kk = *(volatile const __attribute__((address_space(3))) k4_t*)(
-                                (const __attribute__((address_space(3))) uint32_t*)(t_key + g));
+                            const k4_t kk = probe == 0 ? kf[r] : read_group(slot);
+                            ++probe;
                             uint32_t hit = 4, emp = 4;
 #pragma unroll
                             for (int q = 3; q >= 0; --q) {
