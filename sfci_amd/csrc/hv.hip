@@ -1684,6 +1684,13 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 return *(volatile const __attribute__((address_space(3))) k4_t*)(
                     (const __attribute__((address_space(3))) uint32_t*)(t_key + (slot & ~3u)));
             };
+            // the table counts as full once fill_max slots are taken (the fill counter's adds
+            // return nothing; it is read at round boundaries and every 16 probes) or a probe
+            // saw every group
+            auto table_full = [&]() -> bool {
+                return __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
+                       __hip_atomic_load(&s_fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= fill_max;
+            };
             auto insert_round = [&](const uint64_t* k_, const T* v_) {
                 // compact keys: the first group of every record of the round is read before any
                 // is probed, so that the reads' latencies overlap
@@ -1729,8 +1736,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         // key, an empty slot) at or after `slot` in probe order decides.  A group
                         // read earlier may be stale: a key inserted since is met by the CAS.
                         for (;;) {
-                            if ((probe & 15) == 15 &&
-                                __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                            if ((probe & 15) == 15 && table_full()) {
                                 probe = BR_SLOTS;   // the class is being abandoned: give up the search
                                 break;
                             }
@@ -1757,7 +1763,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                                 slot = g + emp;
                                 const KT prev = atomicCAS(&t_key[slot], EMPTYK, key);
                                 if (prev == EMPTYK) {
-                                    if (atomicAdd(&s_fill, 1u) + 1u >= fill_max) s_full = 1;
+                                    atomicAdd(&s_fill, 1u);
                                     break;
                                 }
                                 if (prev == key) break;
@@ -1768,8 +1774,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         }
                     } else {
                     for (; probe < BR_SLOTS; ++probe) {
-                        if ((probe & 63) == 63 &&
-                            __hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        if ((probe & 63) == 63 && table_full()) {
                             probe = BR_SLOTS;   // the class is being abandoned: give up the search
                             break;
                         }
@@ -1778,7 +1783,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         if (cur == EMPTYK) {
                             const KT prev = atomicCAS(&t_key[slot], EMPTYK, key);
                             if (prev == EMPTYK) {
-                                if (atomicAdd(&s_fill, 1u) + 1u >= fill_max) s_full = 1;
+                                atomicAdd(&s_fill, 1u);
                                 break;
                             }
                             if (prev == key) break;
@@ -1837,18 +1842,17 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             insert_round(ka, va);
             uint64_t i0 = st + STEP;
             // a full table ends the class early: each thread stops at its next round
-            for (; i0 + STEP <= en && !__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                 i0 += STEP) {
+            for (; i0 + STEP <= en && !table_full(); i0 += STEP) {
                 load_full(i0, ka, va);
                 insert_round(ka, va);
             }
-            if (i0 < en && !__hip_atomic_load(&s_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            if (i0 < en && !table_full()) {
                 load_round(i0, ka, va);
                 insert_round(ka, va);
                 i0 = en;
             }
             __syncthreads();
-            const bool full = s_full != 0;
+            const bool full = s_full != 0 || s_fill >= fill_max;
             if (full) {
                 // split this class into f subclasses (class cq of cm -> classes cq*f .. cq*f+f-1
                 // of cm*f, a refinement of its range).  Its targets are estimated from the
