@@ -101,7 +101,8 @@ int sfci_hv_stats(const sfci_ctx* ctx, sfci_hv_stats_t* st);
 /* How H.v groups candidates by target (no reference counterpart; a tuning and test knob):
  * mode 0 = automatic (hash buckets for large passes), 1 = full radix sort only,
  * 2 = hash buckets with exactly hash_bits bits (1..16), 3 = as 2 with each LDS table counted
- * full at 1/16 of its usual occupancy (forces nested class splits), 4 = as 2 with full-key
+ * full at 230 occupied slots (1/16 of a full-key table's usual occupancy; forces nested class
+ * splits), 4 = as 2 with full-key
  * records (no compact 12/8-byte records).  Results are identical in every mode. */
 int sfci_set_merge_mode(sfci_ctx* ctx, int mode, int hash_bits);
 /* How H.v evaluates same-spin doubles (no reference counterpart; a tuning and test knob):

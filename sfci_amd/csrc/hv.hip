@@ -1553,7 +1553,13 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_reduce(HamDev H, const uint64_t*
 // set of targets, so classes split further when they overflow in turn.  Only a class that
 // still overflows at 256 classes sets FLAG_BUCKET_FULL (the pass then takes the full sort).
 constexpr int BR_THREADS = 1024;
-constexpr int BR_SLOTS = 4608;                 // 4608 x 32 B = 144 KB of LDS per workgroup
+constexpr int BR_SLOTS = 4608;                 // full-key tables: 4608 x 32 B = 144 KB of LDS per workgroup
+constexpr int BR_SLOTS_C = 5504;               // compact-key tables: 5504 x 28 B = 150.5 KB (with the tag
+                                               // lists and class stacks about 158 of the CU's 160 KB)
+template <bool CR>
+constexpr int br_slots() {
+    return CR ? BR_SLOTS_C : BR_SLOTS;
+}
 // records per thread loaded together: as many as the 128 registers of a 1024-thread
 // workgroup hold without spilling
 template <typename T, bool CR>
@@ -1562,11 +1568,11 @@ constexpr int br_batch() {
 }
 constexpr int BR_TAGS = 512;                   // tagged targets per bucket finished from a list
 constexpr int BR_FILL = BR_SLOTS * 4 / 5;      // occupied slots at which a table counts as full
-constexpr int BR_TARGET = BR_SLOTS * 3 / 5;    // targets per class aimed at when a bucket splits
+constexpr int BR_FILL_C = BR_SLOTS_C * 4 / 5;
 constexpr int BR_STACK = 256;                  // pending classes of one bucket
 constexpr uint32_t BR_MAX_CLASSES = 4096;      // classes of one bucket beyond which the pass takes the full sort
 static_assert(BR_TAGS <= BR_THREADS, "the tag list is finished one entry per thread");
-static_assert(BR_SLOTS % 4 == 0, "compact-key tables are probed in aligned groups of four slots");
+static_assert(BR_SLOTS_C % 4 == 0, "compact-key tables are probed in aligned groups of four slots");
 constexpr unsigned long long BR_EMPTY = ~0ull;
 
 // Bucket b of a hash-partitioned candidate list (ascending in target_hash(key >> 1, hb))
@@ -1608,10 +1614,11 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     // per 128-bit LDS read); full-key records a 64-bit dense index
     using KT = typename std::conditional<CR, uint32_t, unsigned long long>::type;
     constexpr KT EMPTYK = CR ? (KT)0xFFFFFFFFu : (KT)BR_EMPTY;
-    __shared__ __align__(16) KT t_key[BR_SLOTS];
-    __shared__ unsigned long long t_l0[BR_SLOTS], t_l1[BR_SLOTS], t_l2[BR_SLOTS];
-    __shared__ uint32_t t_tag[(BR_SLOTS + 31) / 32];   // tagged, finished by search
-    __shared__ uint32_t t_tl[(BR_SLOTS + 31) / 32];    // tagged, finished from the tag list
+    constexpr int SLOTS = br_slots<CR>();
+    __shared__ __align__(16) KT t_key[SLOTS];
+    __shared__ unsigned long long t_l0[SLOTS], t_l1[SLOTS], t_l2[SLOTS];
+    __shared__ uint32_t t_tag[(SLOTS + 31) / 32];   // tagged, finished by search
+    __shared__ uint32_t t_tl[(SLOTS + 31) / 32];    // tagged, finished from the tag list
     __shared__ uint32_t s_tslot[BR_TAGS], s_tj[BR_TAGS];
     __shared__ int s_full, s_ntag, s_sp, s_bad;
     __shared__ uint32_t s_fill;
@@ -1700,7 +1707,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
 #pragma unroll
                     for (int r = 0; r < BRB; ++r) {
                         const uint32_t t = (uint32_t)(k_[r] >> 1) * 0x9E3779B1u;
-                        sf[r] = (uint32_t)(((t >> 16) * (uint32_t)BR_SLOTS) >> 16);
+                        sf[r] = (uint32_t)(((t >> 16) * (uint32_t)SLOTS) >> 16);
                         if (k_[r] != ~0ull) kf[r] = read_group(sf[r]);
                     }
                 }
@@ -1725,7 +1732,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                     // class cq of cm: the cm-th part of the 16-bit class value's range
                     if (cm > 1 && (uint32_t)((c16 * cm) >> 16) != cq) continue;
                     ++n_ins;
-                    uint32_t slot = (uint32_t)((s16 * (uint32_t)BR_SLOTS) >> 16);
+                    uint32_t slot = (uint32_t)((s16 * (uint32_t)SLOTS) >> 16);
                     // most records meet a key already in the table (a target collects one
                     // record per connected source): a plain read finds it, and only an empty
                     // slot costs a compare-and-swap.  A slot's key never changes once set.
@@ -1737,11 +1744,11 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                         // read earlier may be stale: a key inserted since is met by the CAS.
                         for (;;) {
                             if ((probe & 15) == 15 && table_full()) {
-                                probe = BR_SLOTS;   // the class is being abandoned: give up the search
+                                probe = SLOTS;   // the class is being abandoned: give up the search
                                 break;
                             }
-                            if (probe * 4 > BR_SLOTS + 4) {   // every group seen
-                                probe = BR_SLOTS;
+                            if (probe * 4 > SLOTS + 4) {   // every group seen
+                                probe = SLOTS;
                                 break;
                             }
                             const uint32_t g = slot & ~3u, q0 = slot & 3u;
@@ -1767,15 +1774,15 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                                     break;
                                 }
                                 if (prev == key) break;
-                                slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;   // taken by another key
+                                slot = (slot + 1 == (uint32_t)SLOTS) ? 0u : slot + 1;   // taken by another key
                                 continue;
                             }
-                            slot = (g + 4 >= (uint32_t)BR_SLOTS) ? 0u : g + 4;
+                            slot = (g + 4 >= (uint32_t)SLOTS) ? 0u : g + 4;
                         }
                     } else {
-                    for (; probe < BR_SLOTS; ++probe) {
+                    for (; probe < SLOTS; ++probe) {
                         if ((probe & 63) == 63 && table_full()) {
-                            probe = BR_SLOTS;   // the class is being abandoned: give up the search
+                            probe = SLOTS;   // the class is being abandoned: give up the search
                             break;
                         }
                         const KT cur = __hip_atomic_load(&t_key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1788,10 +1795,10 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                             }
                             if (prev == key) break;
                         }
-                        slot = (slot + 1 == (uint32_t)BR_SLOTS) ? 0u : slot + 1;
+                        slot = (slot + 1 == (uint32_t)SLOTS) ? 0u : slot + 1;
                     }
                     }
-                    if (probe == BR_SLOTS) {
+                    if (probe == SLOTS) {
                         s_full = 1;
                         continue;
                     }
@@ -1822,13 +1829,13 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             T va[BRB];
             if (st + STEP <= en) load_full(st, ka, va);
             else load_round(st, ka, va);
-            for (int i = tid; i < BR_SLOTS; i += BR_THREADS) {
+            for (int i = tid; i < SLOTS; i += BR_THREADS) {
                 t_key[i] = EMPTYK;
                 t_l0[i] = 0;
                 t_l1[i] = 0;
                 t_l2[i] = 0;
             }
-            for (int i = tid; i < (BR_SLOTS + 31) / 32; i += BR_THREADS) {
+            for (int i = tid; i < (SLOTS + 31) / 32; i += BR_THREADS) {
                 t_tag[i] = 0;
                 t_tl[i] = 0;
             }
@@ -1863,7 +1870,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 if (lane == 0 && r) atomicAdd(&s_nread, r);
                 __syncthreads();
                 if (tid == 0) {
-                    uint32_t f = split_count_for(s_fill, s_nread, BR_TARGET, BR_STACK);
+                    uint32_t f = split_count_for(s_fill, s_nread, SLOTS * 3 / 5, BR_STACK);
                     if (f < 2) f = 2;
                     if (s_sp + (int)f > BR_STACK) f = (uint32_t)(BR_STACK - s_sp);
                     if (f < 2 || cm * f > BR_MAX_CLASSES) {
@@ -1887,7 +1894,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             // finish: every slot's target, and the tag list's, is rounded and cut; the kept
             // ones (staged under their dense index) are written after one block-wide scan of
             // the per-thread counts, with one global reservation per class
-            constexpr int FR = (BR_SLOTS + BR_THREADS - 1) / BR_THREADS;
+            constexpr int FR = (SLOTS + BR_THREADS - 1) / BR_THREADS;
             bool em[FR + 1];
             double yv[FR + 1];
             uint64_t dk[FR + 1];
@@ -1898,7 +1905,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
                 em[r] = false;
                 yv[r] = 0.0;
                 dk[r] = 0;
-                const KT key = slot < BR_SLOTS ? t_key[slot] : EMPTYK;
+                const KT key = slot < SLOTS ? t_key[slot] : EMPTYK;
                 if (key != EMPTYK) {
                     ++uniq;
                     if (!((t_tl[slot >> 5] >> (slot & 31)) & 1u)) {
@@ -2039,7 +2046,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         // overflows is merged in hash classes, each rereading its records, and the load aimed
         // at adapts to how many buckets split in the previous call
         const double ratio = std::min(1.0, c->unique_ratio * 1.25);
-        const double bucket_cap = c->pass_load * BR_SLOTS * 65536.0 / ratio;
+        const double bucket_cap = c->pass_load * BR_SLOTS_C * 65536.0 / ratio;
         if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
     }
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
@@ -2269,9 +2276,9 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             // distinct targets expected in this pass; aim at a table load below 0.6, and drop
             // to a whole number of 8-bit radix digits when that keeps the load below 0.8
             double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.25);
-            while (hb < 16 && est > 0.6 * BR_SLOTS * (double)(1ull << hb)) ++hb;
+            while (hb < 16 && est > 0.6 * BR_SLOTS_C * (double)(1ull << hb)) ++hb;
             int down = hb & ~7;
-            if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS * (double)(1ull << down)) hb = down;
+            if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS_C * (double)(1ull << down)) hb = down;
         }
         bool hashed = hb > 0 && hb <= 16;
         // compact records (12 or 8 bytes) when the pass's dense range leaves at most 31 bits
@@ -2374,7 +2381,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             HIPCHK(hipMemsetAsync(d_split, 0, 8, c->stream));
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
-                const uint32_t fill = c->merge_mode == 3 ? BR_FILL / 16 : BR_FILL;
+                // (mode 3, a test knob: every table counts as full at BR_FILL / 16 slots)
+                const uint32_t fill = c->merge_mode == 3 ? BR_FILL / 16 : compact ? BR_FILL_C : BR_FILL;
                 if (compact) {
                     int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(RC) * n_pass);
                     k_bucket_reduce<T, true><<<256, BR_THREADS, 0, c->stream>>>(

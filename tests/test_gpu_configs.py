@@ -108,7 +108,7 @@ def test_c2_sample_hv_bit_exact(c2, case):
         ctx.set_workspace(8 << 30)            # one pass: tens of thousands of targets per bucket
         ctx.set_merge_mode(2, 9)
     elif case == "nested_split_12bits":
-        ctx.set_merge_mode(3, 12)             # tables counted full at 1/16: classes split again
+        ctx.set_merge_mode(3, 12)             # tables counted full at 230 slots: classes split again
     elif case == "full_key_records_16bits":
         ctx.set_merge_mode(4, 16)             # 16-byte records with the dense target (no compaction)
     log("c2 hv case %s" % case)

@@ -159,7 +159,7 @@ def test_hv_multipass_matches(sys1, orc1, ctx1):
 def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
     # full sort, hash buckets (compact records above 8 bits, full-key records in mode 4),
     # hash buckets too few to hold the targets (each bucket is split into hash classes
-    # merged one at a time), and tables counted full at 1/16 of their occupancy (classes
+    # merged one at a time), and tables counted full at 230 occupied slots (classes
     # split again and again) give the same bits
     k, v = random_input(sys1, 3000, 31)
     ctx1.set_merge_mode(mode, hbits)
