@@ -564,6 +564,8 @@ def main():
     ap.add_argument("--no-tte", action="store_true", help="leave out the time-to-E block of the default run")
     ap.add_argument("--full-key-records", action="store_true",
                     help="merge with 16-byte full-key records and 16 hash bits (no compact records)")
+    ap.add_argument("--workspace", type=float, default=None,
+                    help="GiB the H.v candidate buffers of one target pass may use (fewer: more passes)")
     ap.add_argument("--data", default=None, help="directory for the generated integral files")
     ap.add_argument("--time-to-e", action="store_true",
                     help="run the whole ground-state solve instead (GPU, and the CPU oracle with --cpu)")
@@ -629,6 +631,8 @@ def main():
     ctx.set_precision(mixed)
     if args.full_key_records:
         ctx.set_merge_mode(4, 16)
+    if args.workspace:
+        ctx.set_workspace(int(args.workspace * (1 << 30)))
     rec = 12.0 if mixed else 16.0        # bytes of one candidate record (key + value)
     for i in range(args.warmup):
         ctx.hv(v, out, nd, 0.0, 1.0)

@@ -2038,17 +2038,22 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     const uint64_t n_rows = H.dim;
     const size_t rec = sizeof(uint64_t) + sizeof(double);
     // two record buffers per pass and one digit byte per first-level slot
-    size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
-    if (c->merge_mode == 0) {
-        // keep the distinct targets of a pass within 2^16 LDS-table buckets at load
-        // c->pass_load (0.9 at most: more passes cost more generation than an occasional
-        // class split), using the distinct/candidate ratio of the previous call; a bucket that
-        // overflows is merged in hash classes, each rereading its records, and the load aimed
-        // at adapts to how many buckets split in the previous call
-        const double ratio = std::min(1.0, c->unique_ratio * 1.25);
-        const double bucket_cap = c->pass_load * BR_SLOTS_C * 65536.0 / ratio;
-        if (bucket_cap < (double)pass_cap) pass_cap = std::max<size_t>((size_t)bucket_cap, (size_t)1 << 20);
-    }
+    const size_t pass_cap = std::max<size_t>(c->ws_limit / (2 * rec + 1), (size_t)1 << 20);
+    // keep the distinct targets of a pass within 2^16 LDS-table buckets at load c->pass_load
+    // (0.9 at most: more passes cost more generation than an occasional class split).  A row
+    // segment's distinct targets are estimated from its candidates and the distinct/candidate
+    // ratio the previous call saw in that row range (x 1.25); a bucket that overflows anyway is
+    // merged in hash classes, each rereading its records, and the load aimed at adapts to how
+    // many buckets split in the previous call
+    const double dcap = c->merge_mode == 0 ? c->pass_load * BR_SLOTS_C * 65536.0 : INFINITY;
+    auto ratio_at = [&](uint64_t row) -> double {
+        double r = c->unique_ratio;
+        if (!c->rr_lo.empty()) {
+            const size_t i = (size_t)(std::upper_bound(c->rr_lo.begin(), c->rr_lo.end(), row) - c->rr_lo.begin());
+            r = c->rr_ratio[i ? i - 1 : 0];
+        }
+        return std::min(1.0, r * 1.25);
+    };
     uint64_t total_cand = (uint64_t)ns * H.n_conn;
     // distinct spin-0 strings of the sources (planning work depends only on them)
     uint64_t n_row_src = 0;
@@ -2078,9 +2083,11 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     };
     std::vector<std::pair<uint64_t, uint64_t>> passes;
     std::vector<uint64_t> pass_count;   // candidates of each pass
-    if (total_cand <= pass_cap) {
+    std::vector<double> pass_dist;      // distinct targets estimated for each pass
+    if (total_cand <= pass_cap && (double)total_cand * std::min(1.0, c->unique_ratio * 1.25) <= dcap) {
         passes.push_back({0, n_rows});
         pass_count.push_back(total_cand);
+        pass_dist.push_back((double)total_cand * std::min(1.0, c->unique_ratio * 1.25));
     } else {
         // candidates per row segment: a histogram over at most 4096 row groups, refined
         // inside any group that alone exceeds the pass capacity, down to single rows
@@ -2108,12 +2115,16 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                              hh[g]};
             return out;
         };
+        // a segment too large for one pass by either count
+        auto too_big = [&](const Seg& sg) {
+            return sg.count > pass_cap || (double)sg.count * ratio_at(sg.lo) > dcap;
+        };
         std::vector<Seg> segs = hist_of(0, n_rows);
         for (int round = 0; round < 8; ++round) {
             bool refined = false;
             std::vector<Seg> next;
             for (const Seg& sg : segs) {
-                if (sg.count > pass_cap && sg.hi - sg.lo > 1) {
+                if (too_big(sg) && sg.hi - sg.lo > 1) {
                     std::vector<Seg> sub = hist_of(sg.lo, sg.hi);
                     next.insert(next.end(), sub.begin(), sub.end());
                     refined = true;
@@ -2125,19 +2136,25 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             if (!refined) break;
         }
         uint64_t lo = 0, acc = 0;
+        double acc_d = 0.0;   // distinct targets estimated for the pass being filled
         for (const Seg& sg : segs) {
             if (sg.count > pass_cap)
                 throw HipError("one target row exceeds the pass capacity; raise the workspace limit", -4);
-            if (acc + sg.count > pass_cap) {
+            const double d = (double)sg.count * ratio_at(sg.lo);
+            if (acc > 0 && (acc + sg.count > pass_cap || acc_d + d > dcap)) {
                 passes.push_back({lo, sg.lo});
                 pass_count.push_back(acc);
+                pass_dist.push_back(acc_d);
                 lo = sg.lo;
                 acc = 0;
+                acc_d = 0.0;
             }
             acc += sg.count;
+            acc_d += d;
         }
         passes.push_back({lo, n_rows});
         pass_count.push_back(acc);
+        pass_dist.push_back(acc_d);
     }
     st.n_passes = passes.size();
     uint32_t* pcnt = nullptr;
@@ -2245,6 +2262,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     HIPCHK(hipMemsetAsync(c->d_counter, 0, 2 * sizeof(uint64_t), c->stream));
     uint64_t out_count_host = 0;
     uint64_t buckets_merged = 0;   // hash buckets of the passes merged through LDS tables
+    std::vector<uint64_t> rr_lo;   // this call's distinct/candidate ratio per pass, for the next
+    std::vector<double> rr_ratio;
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const uint64_t row_lo = passes[pi].first, row_hi = passes[pi].second;
         if (row_lo >= row_hi) continue;
@@ -2275,7 +2294,7 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         } else if (c->merge_mode == 0 && n_pass >= ((size_t)1 << 20)) {
             // distinct targets expected in this pass; aim at a table load below 0.6, and drop
             // to a whole number of 8-bit radix digits when that keeps the load below 0.8
-            double est = (double)n_pass * std::min(1.0, c->unique_ratio * 1.25);
+            const double est = pass_dist[pi];
             while (hb < 16 && est > 0.6 * BR_SLOTS_C * (double)(1ull << hb)) ++hb;
             int down = hb & ~7;
             if (down > 0 && down < hb && est <= 0.8 * BR_SLOTS_C * (double)(1ull << down)) hb = down;
@@ -2434,6 +2453,8 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
             }
             if (cnts[0] <= stage_cap) {
                 out_count_host = cnts[0];
+                rr_lo.push_back(row_lo);
+                rr_ratio.push_back(std::max(1e-6, (double)(cnts[1] - uniq_host) / (double)n_pass));
                 uniq_host = cnts[1];
                 st.n_split += cnts[2];
                 if (hashed) buckets_merged += (uint64_t)1 << hb;
@@ -2476,6 +2497,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
     if (fl & FLAG_BAD_INPUT) throw HipError("input vector keys are not sorted and unique", -1);
     st.n_unique = uniq;
     if (st.n_candidates) c->unique_ratio = std::max(1e-6, (double)uniq / (double)st.n_candidates);
+    if (rr_lo.size() > 1 && rr_lo[0] == 0) {
+        c->rr_lo.swap(rr_lo);
+        c->rr_ratio.swap(rr_ratio);
+    } else {
+        c->rr_lo.clear();
+        c->rr_ratio.clear();
+    }
     if (c->merge_mode == 0 && buckets_merged) {
         // the distinct-target count of a pass varies with its row range: aim lower while more
         // than 1% of the buckets split, back up while none does

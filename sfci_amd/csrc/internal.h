@@ -337,6 +337,12 @@ struct Ctx {
     DevBuf bucket_bounds;
     double unique_ratio = 0.25;
     double pass_load = 0.9;          // distinct targets per LDS-table bucket aimed at, in table slots
+    // distinct/candidate ratio of each target-row range (one per pass) of the previous H.v:
+    // range lower bounds (ascending, the first 0) and ratios; the next call's planning weighs
+    // each row segment by the ratio of its range (distinct targets are denser near the rows
+    // of the largest entries)
+    std::vector<uint64_t> rr_lo;
+    std::vector<double> rr_ratio;
     bool g_one_block = false;        // the four two-electron spin blocks share one table
     int precision = 0;               // 0: FP64 candidate values; 1: FP32 values (mixed precision)
     int merge_mode = 0;              // 0 auto, 1 full sort only, 2 hash buckets with merge_hash_bits
