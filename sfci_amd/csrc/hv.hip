@@ -52,10 +52,11 @@ __device__ __forceinline__ HamDev stage_tables(const HamDev& H, uint32_t* s_bino
     return L;
 }
 
-// The generators wait on table reads and record stores most of their cycles: they run at
-// eight wavefronts per SIMD (at most 64 VGPRs; their LDS allows eight 256-thread workgroups
-// per CU), which the compiler otherwise trades for a few more registers.  (The screened
-// generator's LDS staging allows seven.)
+// The generators and the energy pull wait on table reads, lookups and record stores most of
+// their cycles: they run at eight wavefronts per SIMD (at most 64 VGPRs; their LDS allows
+// eight 256-thread workgroups per CU), which the compiler otherwise trades for a few more
+// registers (energy pull of the c2 vector: 301 ms at six wavefronts, 201 ms at eight).  (The
+// screened generator's LDS staging allows seven.)
 #define GEN_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
 
 constexpr int MAX_SINGLES = (SFCI_MAX_ORB / 2) * (SFCI_MAX_ORB / 2);   // k*v <= 16*16 for 32 orbitals
@@ -1118,7 +1119,7 @@ __device__ __forceinline__ uint32_t seg_find(const uint32_t* __restrict__ r1, ui
     return lo;
 }
 
-__global__ __launch_bounds__(SFCI_BLOCK) void k_energy(HamDev Hg, const uint64_t* __restrict__ sk,
+__global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_energy(HamDev Hg, const uint64_t* __restrict__ sk,
                                                        const double* __restrict__ sv, size_t n_src,
                                                        const uint32_t* __restrict__ row_off,
                                                        const uint32_t* __restrict__ r1s, const double* __restrict__ cs,
