@@ -787,7 +787,8 @@ __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* _
                                                                       unsigned long long* __restrict__ cursor,
                                                                       const uint64_t* __restrict__ rend,
                                                                       uint32_t* __restrict__ flags,
-                                                                      uint8_t* __restrict__ dout) {
+                                                                      uint8_t* __restrict__ dout,
+                                                                      unsigned long long* __restrict__ tile_next) {
     __shared__ R s_rec[ptile<R>()];
     __shared__ uint16_t s_bk[ptile<R>()];
     __shared__ uint64_t s_gbase[RADIX];
@@ -799,13 +800,21 @@ __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* _
     constexpr int NW = TileRegs<R>::NW;
     TileRegs<R> cur, nxt;
     uint32_t rank[PART_ITEMS];
+    // tiles are taken from a global counter (a workgroup's first tile is its index), so a
+    // workgroup that runs slower takes fewer tiles; the tile after next is claimed while this
+    // one is ranked, and published in s_tn at the barrier before the write-out
+    __shared__ unsigned long long s_tn;
     size_t t = blockIdx.x;
+    if (tid == 0) s_tn = gridDim.x + atomicAdd(tile_next, 1ull);
     if (t < n_tiles) tile_load(rin, bkt, t * ptile<R>(), (uint32_t)min((size_t)ptile<R>(), n - t * ptile<R>()), cur);
-    for (; t < n_tiles; t += gridDim.x) {
+    while (t < n_tiles) {
         const size_t base = t * ptile<R>();
         const uint32_t len = (uint32_t)min((size_t)ptile<R>(), n - base);
         if (tid < RADIX) s_cnt[tid] = 0;
         __syncthreads();
+        const size_t tn = (size_t)s_tn;   // this workgroup's next tile
+        unsigned long long tnn = 0;
+        if (tid == 0) tnn = gridDim.x + atomicAdd(tile_next, 1ull);   // and the one after
 #pragma unroll
         for (int r = 0; r < PART_ITEMS; ++r) {
             const uint32_t li = (uint32_t)r * pthreads<R>() + tid;
@@ -840,9 +849,9 @@ __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* _
                 s_bk[p] = cur.d[r];
             }
         }
+        if (tid == 0) s_tn = tnn;   // read by every thread after the next iteration's first barrier
         __syncthreads();
         // the next tile's loads go out before this tile's stores
-        const size_t tn = t + gridDim.x;
         if (tn < n_tiles)
             tile_load(rin, bkt, tn * ptile<R>(), (uint32_t)min((size_t)ptile<R>(), n - tn * ptile<R>()), nxt);
         for (uint32_t p = tid; p < len; p += pthreads<R>()) {
@@ -854,6 +863,7 @@ __global__ __launch_bounds__(pthreads<R>()) void k_part_scatter_cur_c(const R* _
             }
         }
         cur = nxt;
+        t = tn;
         __syncthreads();   // LDS reuse by the next tile
     }
 }
@@ -935,18 +945,20 @@ bool hash_partition_c(Ctx* c, R* A, const uint16_t* bkt, R* B, size_t n, int hb,
     const int sh1 = hb - 8;
     // pass 1: the bucket's top 8 bits, fixed-capacity regions of B
     const size_t cap1 = part_capacity(n) / RADIX;
-    std::vector<uint64_t> cur(2 * RADIX);
+    // region cursors, region ends, and the tile counter
+    std::vector<uint64_t> cur(2 * RADIX + 1, 0);
     for (int d = 0; d < RADIX; ++d) {
         cur[d] = (uint64_t)d * cap1;
         cur[RADIX + d] = (uint64_t)(d + 1) * cap1;
     }
-    uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX);
-    HIPCHK(hipMemcpyAsync(dcur, cur.data(), 2 * RADIX * 8, hipMemcpyHostToDevice, c->stream));
+    uint64_t* dcur = c->part_meta[1].get<uint64_t>(2 * RADIX + 1);
+    HIPCHK(hipMemcpyAsync(dcur, cur.data(), (2 * RADIX + 1) * 8, hipMemcpyHostToDevice, c->stream));
     const size_t n_tiles = (n + ptile<R>() - 1) / ptile<R>();
     uint8_t* dig = c->part_dig.get<uint8_t>(part_capacity(n));
     int t1 = kt_begin(c, K_PART_SCATTER1, (2.0 * sizeof(R) + 3.0) * n);
     k_part_scatter_cur_c<R><<<(unsigned)std::min<size_t>(n_tiles, PART_PERSIST), pthreads<R>(), 0, c->stream>>>(
-        A, bkt, n, B, sh1, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig);
+        A, bkt, n, B, sh1, (unsigned long long*)dcur, dcur + RADIX, c->d_flags, dig,
+        (unsigned long long*)dcur + 2 * RADIX);
     kt_end(c, t1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(cur.data(), dcur, RADIX * 8, hipMemcpyDeviceToHost, c->stream));
