@@ -1610,7 +1610,7 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
     double cut,
     uint64_t* __restrict__ sk_out, double* __restrict__ sv_out, unsigned long long* out_count, size_t out_cap,
     unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags,
-    unsigned long long* split_count, uint32_t fill_max) {
+    unsigned long long* split_count, uint32_t fill_max, unsigned long long* bucket_next) {
     // table keys: compact records carry a 31-bit key, stored as one 32-bit word (four slots
     // per 128-bit LDS read); full-key records a 64-bit dense index
     using KT = typename std::conditional<CR, uint32_t, unsigned long long>::type;
@@ -1964,21 +1964,39 @@ __global__ __launch_bounds__(BR_THREADS) void k_bucket_reduce(
             }
         }
     };
-    uint64_t st = 0, en = 0;
+    // buckets are taken from a global counter (a workgroup's first bucket is its index), so a
+    // workgroup slowed by class splits takes fewer; the bucket after next is claimed while this
+    // one is merged, and the next bucket's bounds are loaded meanwhile
+    __shared__ unsigned long long s_nb;
+    if (tid == 0) s_nb = gridDim.x + atomicAdd(bucket_next, 1ull);
+    __syncthreads();
+    uint64_t nb = s_nb;
+    uint64_t st = 0, en = 0, nst = 0, nen = 0;
     if (b < n_buckets) {
         st = bstart[b];
         en = bend[b];
     }
-    for (; b < n_buckets; b += gridDim.x) {
-        // the next bucket's bounds are loaded while this one is merged
-        uint64_t nst = 0, nen = 0;
-        if (b + gridDim.x < n_buckets) {
-            nst = bstart[b + gridDim.x];
-            nen = bend[b + gridDim.x];
-        }
+    if (nb < n_buckets) {
+        nst = bstart[nb];
+        nen = bend[nb];
+    }
+    while (b < n_buckets) {
+        unsigned long long nn = 0;
+        if (tid == 0) nn = gridDim.x + atomicAdd(bucket_next, 1ull);
         if (st < en) merge_bucket(st, en);
+        __syncthreads();   // every thread has read s_nb (an empty bucket has no barrier of its own)
+        if (tid == 0) s_nb = nn;
+        __syncthreads();
+        const uint64_t nnb = s_nb;
+        b = nb;
         st = nst;
         en = nen;
+        nb = nnb;
+        nst = nen = 0;
+        if (nb < n_buckets) {
+            nst = bstart[nb];
+            nen = bend[nb];
+        }
     }
     if (bad_input) atomicOr(flags, FLAG_BAD_INPUT);
     if (overflow) atomicOr(flags, FLAG_OUT_OVERFLOW);
@@ -2394,11 +2412,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         };
         if (!hashed) sort_path();
         unsigned long long* d_split = (unsigned long long*)c->d_counter + 3;
+        unsigned long long* d_bnext = (unsigned long long*)c->d_counter + 6;   // the merge's bucket counter
         for (int attempt = 0;; ++attempt) {
             PhaseTimer t(c);
             uint64_t before = out_count_host;
             acc_clear(c, 1);
             HIPCHK(hipMemsetAsync(d_split, 0, 8, c->stream));
+            HIPCHK(hipMemsetAsync(d_bnext, 0, 8, c->stream));
             if (hashed) {
                 const uint64_t nb = (uint64_t)1 << hb;
                 // (mode 3, a test knob: every table counts as full at BR_FILL / 16 slots)
@@ -2407,13 +2427,13 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
                     int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(RC) * n_pass);
                     k_bucket_reduce<T, true><<<256, BR_THREADS, 0, c->stream>>>(
                         H, A, hp, bstart, bend, nb, hb, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
-                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill);
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill, d_bnext);
                     kt_end(c, tr);
                 } else {
                     int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(R) * n_pass);
                     k_bucket_reduce<T, false><<<256, BR_THREADS, 0, c->stream>>>(
                         H, rd, hp, bstart, bend, nb, hb, row_lo * H.dim, sk, sv, dterm, ns, cut_eff, stk, stv,
-                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill);
+                        d_out_count, stage_cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill, d_bnext);
                     kt_end(c, tr);
                 }
             } else {
