@@ -721,7 +721,8 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
                                               const uint16_t* __restrict__ sdec_g,
                                               typename RecOf<T>::type* __restrict__ out, uint64_t out_cap,
                                               double* __restrict__ dterm, uint32_t* flags, const ScreenOut& screen,
-                                              const CompactOut& cout = CompactOut{}, const SsTabs& tabs = SsTabs{}) {
+                                              const CompactOut& cout = CompactOut{}, const SsTabs& tabs = SsTabs{},
+                                              unsigned long long* src_next = nullptr) {
     __shared__ SrcLds s;
     __shared__ RecS s_scr[SCR ? SCR_LDS : 1];
     __shared__ uint32_t s_scr_n;
@@ -759,9 +760,24 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         while (m1 < H.n_orb - 1 && cls_hi(m1) <= row_lo) ++m1;
         while (m2 > 0 && H.binom[m2 * 33 + H.k] >= row_hi) --m2;
     }
+    // with src_next, sources are taken from a global counter (a workgroup's first source is its
+    // index), so a workgroup on a slower CU takes fewer; the source after next is claimed when
+    // a source starts and published at its end.  Without, workgroup b takes b, b + grid, ...
+    __shared__ unsigned long long s_nj;
+    unsigned long long nn = 0;
+    if (src_next && threadIdx.x == 0) s_nj = gridDim.x + atomicAdd(src_next, 1ull);
+    auto next_source = [&](size_t j) -> size_t {
+        if (!src_next) return j + gridDim.x;
+        __syncthreads();
+        const size_t jn = (size_t)s_nj;
+        __syncthreads();   // every thread has read s_nj
+        if (threadIdx.x == 0) s_nj = nn;
+        return jn;
+    };
     __syncthreads();
     bool range_bad = false, wbad = false;
-    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+    for (size_t j = blockIdx.x; j < n_src; j = next_source(j)) {
+        if (src_next && threadIdx.x == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
         const double temp = s.temp;
@@ -1075,9 +1091,10 @@ __global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_generate(HamDev Hg
                                                          const uint16_t* __restrict__ sdec_g,
                                                          typename RecOf<T>::type* __restrict__ out,
                                                          uint64_t out_cap, double* __restrict__ dterm,
-                                                         uint32_t* flags, SsTabs tabs) {
+                                                         uint32_t* flags, SsTabs tabs,
+                                                         unsigned long long* src_next) {
     generate_body<T, false>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, out, out_cap, dterm, flags,
-                            ScreenOut{}, CompactOut{}, tabs);
+                            ScreenOut{}, CompactOut{}, tabs, src_next);
 }
 
 // the same into compact records and their hash buckets
@@ -1088,9 +1105,10 @@ __global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_generate_compact(H
                                                                  uint64_t row_hi, const uint64_t* __restrict__ off,
                                                                  const uint16_t* __restrict__ sdec_g, CompactOut cout,
                                                                  uint64_t out_cap, double* __restrict__ dterm,
-                                                                 uint32_t* flags, SsTabs tabs) {
+                                                                 uint32_t* flags, SsTabs tabs,
+                                                                 unsigned long long* src_next) {
     generate_body<T, false, true>(Hg, sk, sv, n_src, lambda, coeff, row_lo, row_hi, off, sdec_g, nullptr, out_cap,
-                                  dterm, flags, ScreenOut{}, cout, tabs);
+                                  dterm, flags, ScreenOut{}, cout, tabs, src_next);
 }
 
 // all target rows, screened (mixed-precision values)
@@ -2340,20 +2358,22 @@ static size_t dev_hv_t(Ctx* c, const uint64_t* ik, const double* iv, size_t n_in
         const size_t capB = part_capacity(n_pass);
         R* A = (R*)c->cand_k[0].get<uint8_t>(n_pass * sizeof(R));
         R* B = (R*)c->cand_k[1].get<uint8_t>(std::max(capB * sizeof(R), n_pass * (sizeof(uint64_t) + sizeof(T))));
+        unsigned long long* d_snext = (unsigned long long*)c->d_counter + 7;   // the generator's source counter
         auto generate = [&](bool cr) {
             PhaseTimer t(c);
+            HIPCHK(hipMemsetAsync(d_snext, 0, 8, c->stream));
             if (cr) {
                 uint16_t* bk = c->cand_v[0].get<uint16_t>(n_pass);
                 int tg = kt_begin(c, K_GENERATE, ((double)sizeof(RC) + 2.0) * n_pass + 16.0 * ns);
                 k_generate_compact<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(
                     H, sk, sv, ns, lambda, coeff, row_lo, row_hi, off, c->d_sdec, CompactOut{A, bk, hp}, n_pass,
-                    dterm, c->d_flags, tabs);
+                    dterm, c->d_flags, tabs, d_snext);
                 kt_end(c, tg);
             } else {
                 int tg = kt_begin(c, K_GENERATE, (double)sizeof(R) * n_pass + 16.0 * ns);
                 k_generate<T><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, sv, ns, lambda, coeff, row_lo,
                                                                             row_hi, off, c->d_sdec, A, n_pass,
-                                                                            dterm, c->d_flags, tabs);
+                                                                            dterm, c->d_flags, tabs, d_snext);
                 kt_end(c, tg);
             }
             HIPCHK(hipGetLastError());
@@ -2589,7 +2609,8 @@ void dev_connection_records(Ctx* c, const uint64_t* sk, size_t ns, const Rec64**
     double* dt = c->dterm.get<double>(ns);
     k_fill<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(ones, ns, 1.0);
     k_generate<double><<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(H, sk, ones, ns, 0.0, 1.0, 0, H.dim, nullptr,
-                                                                      c->d_sdec, r0, n, dt, c->d_flags, SsTabs{});
+                                                                      c->d_sdec, r0, n, dt, c->d_flags, SsTabs{},
+                                                                      nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     const uint32_t fl = flags_read_clear(c);
