@@ -1142,7 +1142,8 @@ __global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_energy(HamDev Hg, 
                                                        const uint32_t* __restrict__ row_off,
                                                        const uint32_t* __restrict__ r1s, const double* __restrict__ cs,
                                                        const uint16_t* __restrict__ sdec_g,
-                                                       unsigned long long* acc_out, uint32_t* flags) {
+                                                       unsigned long long* acc_out, uint32_t* flags,
+                                                       unsigned long long* src_next) {
     __shared__ SrcLds s;
     __shared__ uint16_t sdec[MAX_SINGLES];
     __shared__ uint32_t s_binom[LDS_BINOM];
@@ -1161,7 +1162,20 @@ __global__ __launch_bounds__(SFCI_BLOCK) GEN_OCCUPANCY void k_energy(HamDev Hg, 
     __syncthreads();
     bool bad = false;
     fx128 total = fx_zero();   // thread 0: sum of fx(c_i E_i) over this block's sources
-    for (size_t j = blockIdx.x; j < n_src; j += gridDim.x) {
+    // sources from a global counter, as in the generators (the sums are exact: any order)
+    __shared__ unsigned long long s_nj;
+    unsigned long long nn = 0;
+    if (tid == 0) s_nj = gridDim.x + atomicAdd(src_next, 1ull);
+    __syncthreads();
+    auto next_source = [&]() -> size_t {
+        __syncthreads();
+        const size_t jn = (size_t)s_nj;
+        __syncthreads();   // every thread has read s_nj
+        if (tid == 0) s_nj = nn;
+        return jn;
+    };
+    for (size_t j = blockIdx.x; j < n_src; j = next_source()) {
+        if (tid == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
         load_source(H, s, sdec, sk[j], sv[j], 0.0, 0, H.dim, flags);
         // spin-0 singles whose target row holds members of v (nsa <= SFCI_BLOCK)
         {
@@ -2638,9 +2652,11 @@ double dev_energy(Ctx* c, const uint64_t* k, const double* v, size_t n) {
     k_row_index<<<grid_for(c, std::max<size_t>(n, H.dim + 1), 16), SFCI_BLOCK, 0, c->stream>>>(sk, n, H.dim, row_off,
                                                                                                 r1s);
     acc_clear(c, 4);
+    unsigned long long* d_snext = (unsigned long long*)c->d_counter + 7;   // the pull's source counter
+    HIPCHK(hipMemsetAsync(d_snext, 0, 8, c->stream));
     int te = kt_begin(c, K_ENERGY, 16.0 * n);
     k_energy<<<grid_for(c, n), SFCI_BLOCK, 0, c->stream>>>(H, k, v, n, row_off, r1s, sv, c->d_sdec, c->d_acc + 8,
-                                                           c->d_flags);
+                                                           c->d_flags, d_snext);
     kt_end(c, te);
     HIPCHK(hipGetLastError());
     double e = acc_read(c, 4);

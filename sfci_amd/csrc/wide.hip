@@ -371,7 +371,8 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
                                                            double coeff, uint32_t n_pass, uint32_t p_lo, int nb,
                                                            RecW* __restrict__ rec, uint16_t* __restrict__ bkt, int hb,
                                                            uint64_t cap, unsigned long long* count,
-                                                           double* __restrict__ dterm, uint32_t* flags) {
+                                                           double* __restrict__ dterm, uint32_t* flags,
+                                                           unsigned long long* src_next) {
     __shared__ WSrc s;
     __shared__ unsigned s_cnt[W_BATCH], s_cur[W_BATCH];
     __shared__ unsigned long long s_base[W_BATCH];
@@ -383,7 +384,21 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_generate(HamDev H, int n_so, c
         const uint32_t p = pass_of(h, n_pass) - p_lo;   // wraps for p < p_lo
         return p < (uint32_t)nb ? (int)p : -1;
     };
-    for (size_t j = blockIdx.x; j < ns; j += gridDim.x) {
+    // sources from a global counter (a workgroup's first source is its index): the source after
+    // next is claimed when one starts and published at its end
+    __shared__ unsigned long long s_nj;
+    unsigned long long nn = 0;
+    if (tid == 0) s_nj = gridDim.x + atomicAdd(src_next, 1ull);
+    __syncthreads();
+    auto next_source = [&]() -> size_t {
+        __syncthreads();
+        const size_t jn = (size_t)s_nj;
+        __syncthreads();   // every thread has read s_nj
+        if (tid == 0) s_nj = nn;
+        return jn;
+    };
+    for (size_t j = blockIdx.x; j < ns; j = next_source()) {
+        if (tid == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
         __syncthreads();   // the previous source's lists are no longer read
         if (wv == 0) stage_wsrc(H, n_so, slo[j], shi[j], coeff * sv[j], t_orb, s, flags);
         __syncthreads();
@@ -645,7 +660,7 @@ __global__ __launch_bounds__(WR_THREADS) void k_wide_bucket_reduce(
     const double* __restrict__ sv, const double* __restrict__ dterm, size_t ns, double cut,
     uint64_t* __restrict__ olo, uint64_t* __restrict__ ohi, double* __restrict__ oy, unsigned long long* out_count,
     size_t out_cap, unsigned long long* unique_count, unsigned long long* diag_acc, uint32_t* flags,
-    unsigned long long* split_count, uint32_t fill_max) {
+    unsigned long long* split_count, uint32_t fill_max, unsigned long long* bucket_next) {
     __shared__ unsigned long long t_lo[WR_SLOTS], t_hi[WR_SLOTS];
     __shared__ unsigned long long t_l0[WR_SLOTS], t_l1[WR_SLOTS], t_l2[WR_SLOTS];
     __shared__ uint32_t t_tag[(WR_SLOTS + 31) / 32];   // tagged, finished by search
@@ -910,22 +925,39 @@ __global__ __launch_bounds__(WR_THREADS) void k_wide_bucket_reduce(
             }
         }
     };
+    // buckets from a global counter (a workgroup's first bucket is its index); the bucket after
+    // next is claimed while one is merged, and the next bucket's bounds are loaded meanwhile
+    __shared__ unsigned long long s_nb;
     uint64_t b = blockIdx.x;
-    uint64_t st = 0, en = 0;
+    if (threadIdx.x == 0) s_nb = gridDim.x + atomicAdd(bucket_next, 1ull);
+    __syncthreads();
+    uint64_t nb = s_nb;
+    uint64_t st = 0, en = 0, nst = 0, nen = 0;
     if (b < n_buckets) {
         st = bstart[b];
         en = bend[b];
     }
-    for (; b < n_buckets; b += gridDim.x) {
-        // the next bucket's bounds are loaded while this one is merged
-        uint64_t nst = 0, nen = 0;
-        if (b + gridDim.x < n_buckets) {
-            nst = bstart[b + gridDim.x];
-            nen = bend[b + gridDim.x];
-        }
+    if (nb < n_buckets) {
+        nst = bstart[nb];
+        nen = bend[nb];
+    }
+    while (b < n_buckets) {
+        unsigned long long nn = 0;
+        if (threadIdx.x == 0) nn = gridDim.x + atomicAdd(bucket_next, 1ull);
         if (st < en) merge_bucket(st, en);
+        __syncthreads();   // every thread has read s_nb (an empty bucket has no barrier of its own)
+        if (threadIdx.x == 0) s_nb = nn;
+        __syncthreads();
+        const uint64_t nnb = s_nb;
+        b = nb;
         st = nst;
         en = nen;
+        nb = nnb;
+        nst = nen = 0;
+        if (nb < n_buckets) {
+            nst = bstart[nb];
+            nen = bend[nb];
+        }
     }
     if (bad_input) atomicOr(flags, FLAG_BAD_INPUT);
     if (overflow) atomicOr(flags, FLAG_OUT_OVERFLOW);
@@ -1215,6 +1247,8 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
     unsigned long long* d_out = (unsigned long long*)c->d_counter;
     unsigned long long* d_uniq = d_out + 1;
     unsigned long long* d_split = d_out + 3;
+    unsigned long long* d_bnext = d_out + 6;   // the merge's bucket counter
+    unsigned long long* d_snext = d_out + 7;   // the generator's source counter
     unsigned long long* d_gen = (unsigned long long*)c->src_cnt.get<uint64_t>(W_BATCH);
     const uint32_t fill = c->wide_fill ? (uint32_t)c->wide_fill : (uint32_t)WR_FILL;
     fx128 dsum = fx_zero();
@@ -1244,11 +1278,12 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
             const int nbh = (int)std::min<uint32_t>((uint32_t)nb, n_pass - p0);
             // generate passes [p0, p0 + nbh)
             HIPCHK(hipMemsetAsync(d_gen, 0, W_BATCH * 8, c->stream));
+            HIPCHK(hipMemsetAsync(d_snext, 0, 8, c->stream));
             const int gg = (int)std::min<size_t>(ns, 256 * 8);
             int tg = kt_begin(c, K_GENERATE, (double)rec_b * (double)(total_cand / n_pass) * nbh);
             k_wide_generate<MIXED><<<gg, W_BLOCK, 0, c->stream>>>(H, n_so, slo, shi, sv, ns, lambda, coeff, n_pass,
                                                                  p0, nbh, A, bkt, hb, cap, d_gen, dterm,
-                                                                 c->d_flags);
+                                                                 c->d_flags, d_snext);
             kt_end(c, tg);
             HIPCHK(hipGetLastError());
             uint64_t gen_n[W_BATCH] = {};
@@ -1290,10 +1325,11 @@ size_t wide_hv_t(Ctx* c, int n_so, const uint64_t* ik2, const double* iv, size_t
                     const uint64_t before = stg.n;
                     acc_clear(c, 1);
                     HIPCHK(hipMemsetAsync(d_split, 0, 8, c->stream));
+                    HIPCHK(hipMemsetAsync(d_bnext, 0, 8, c->stream));
                     int tr = kt_begin(c, K_BUCKET_REDUCE, (double)sizeof(RecW) * n);
                     k_wide_bucket_reduce<<<256, WR_THREADS, 0, c->stream>>>(
                         Ar, bstart, bend, nbk, hb, slo, shi, sv, dterm, ns, c->cut, stg.lo, stg.hi, stg.y, d_out,
-                        stg.cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill);
+                        stg.cap, d_uniq, c->d_acc + 2, c->d_flags, d_split, fill, d_bnext);
                     kt_end(c, tr);
                     HIPCHK(hipGetLastError());
                     uint64_t cnts[3];
