@@ -1117,9 +1117,9 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_generate_screened(HamDev Hg, con
                                                                   double lambda, double coeff,
                                                                   const uint16_t* __restrict__ sdec_g,
                                                                   double* __restrict__ dterm, uint32_t* flags,
-                                                                  ScreenOut screen) {
+                                                                  ScreenOut screen, unsigned long long* src_next) {
     generate_body<float, true>(Hg, sk, sv, n_src, lambda, coeff, 0, Hg.dim, nullptr, sdec_g, nullptr, 0, dterm, flags,
-                               screen);
+                               screen, CompactOut{}, SsTabs{}, src_next);
 }
 
 // ---- energy readout: calc_energy (src/functions.cpp:1272-1313) ---------------------
@@ -2762,9 +2762,12 @@ size_t screened_generate(Ctx* c, const uint64_t* sk, const double* sv, size_t ns
     unsigned long long* cnt = (unsigned long long*)c->d_counter + 2;
     for (int attempt = 0;; ++attempt) {
         HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+        unsigned long long* d_snext = (unsigned long long*)c->d_counter + 7;   // the generator's source counter
+        HIPCHK(hipMemsetAsync(d_snext, 0, 8, c->stream));
         int tg = kt_begin(c, K_GENERATE, 16.0 * ns);
         k_generate_screened<<<grid_for(c, ns), SFCI_BLOCK, 0, c->stream>>>(
-            H, sk, sv, ns, lambda, coeff, c->d_sdec, dterm, c->d_flags, ScreenOut{recs, cnt, (uint64_t)cap, (float)eps});
+            H, sk, sv, ns, lambda, coeff, c->d_sdec, dterm, c->d_flags, ScreenOut{recs, cnt, (uint64_t)cap, (float)eps},
+            d_snext);
         kt_end(c, tg);
         HIPCHK(hipGetLastError());
         unsigned long long m = 0;

@@ -1008,7 +1008,8 @@ __device__ __forceinline__ uint64_t member_find(const unsigned long long* __rest
 __global__ __launch_bounds__(W_BLOCK) void k_wide_energy(HamDev H, int n_so, const uint64_t* __restrict__ k2,
                                                          const double* __restrict__ v, size_t n,
                                                          const unsigned long long* __restrict__ table, int tb,
-                                                         unsigned long long* acc_out, uint32_t* flags) {
+                                                         unsigned long long* acc_out, uint32_t* flags,
+                                                         unsigned long long* src_next) {
     __shared__ WSrc s;
     __shared__ uint64_t t_orb[W_ORBS];
     __shared__ unsigned long long s_rlo[W_BLOCK / SFCI_WAVE], s_rhi[W_BLOCK / SFCI_WAVE];
@@ -1016,7 +1017,20 @@ __global__ __launch_bounds__(W_BLOCK) void k_wide_energy(HamDev H, int n_so, con
     load_orb_words(t_orb);
     bool bad = false;
     fx128 total = fx_zero();
-    for (size_t i = blockIdx.x; i < n; i += gridDim.x) {
+    // entries from a global counter, as in the generator (the sums are exact: any order)
+    __shared__ unsigned long long s_nj;
+    unsigned long long nn = 0;
+    if (tid == 0) s_nj = gridDim.x + atomicAdd(src_next, 1ull);
+    __syncthreads();
+    auto next_entry = [&]() -> size_t {
+        __syncthreads();
+        const size_t jn = (size_t)s_nj;
+        __syncthreads();   // every thread has read s_nj
+        if (tid == 0) s_nj = nn;
+        return jn;
+    };
+    for (size_t i = blockIdx.x; i < n; i = next_entry()) {
+        if (tid == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
         __syncthreads();
         if (wv == 0) stage_wsrc(H, n_so, k2[2 * i], k2[2 * i + 1], v[i], t_orb, s, flags);
         __syncthreads();
@@ -1505,7 +1519,10 @@ double dev_wide_energy(Ctx* c, const uint64_t* k2, const double* v, size_t n) {
     k_wide_member_build<<<grid_of_w(n), 256, 0, c->stream>>>(k2, n, table, tb);
     acc_clear(c, 1);
     const int g = (int)std::min<size_t>(n, 256 * 8);
-    k_wide_energy<<<g, W_BLOCK, 0, c->stream>>>(c->ham, c->n_so, k2, v, n, table, tb, c->d_acc + 2, c->d_flags);
+    unsigned long long* d_snext = (unsigned long long*)c->d_counter + 7;   // the pull's entry counter
+    HIPCHK(hipMemsetAsync(d_snext, 0, 8, c->stream));
+    k_wide_energy<<<g, W_BLOCK, 0, c->stream>>>(c->ham, c->n_so, k2, v, n, table, tb, c->d_acc + 2, c->d_flags,
+                                                d_snext);
     HIPCHK(hipGetLastError());
     const double e = acc_read(c, 1);
     const uint32_t f = flags_read_clear(c);
