@@ -760,24 +760,28 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         while (m1 < H.n_orb - 1 && cls_hi(m1) <= row_lo) ++m1;
         while (m2 > 0 && H.binom[m2 * 33 + H.k] >= row_hi) --m2;
     }
-    // with src_next, sources are taken from a global counter (a workgroup's first source is its
-    // index), so a workgroup on a slower CU takes fewer; the source after next is claimed when
-    // a source starts and published at its end.  Without, workgroup b takes b, b + grid, ...
+    // with src_next, sources are taken from a global counter in runs of GEN_CLAIM consecutive
+    // sources (a workgroup's first run is its index), so a workgroup on a slower CU takes fewer;
+    // the run after next is claimed when a run starts and published when it ends (one atomic
+    // per run of 8 when there are at least 64 sources per workgroup: one per source made every
+    // workgroup wait on one contended address).  Without, workgroup b takes b, b + grid, ...
+    const size_t GEN_CLAIM = n_src >= (size_t)gridDim.x * 64 ? 8 : 1;
     __shared__ unsigned long long s_nj;
     unsigned long long nn = 0;
     if (src_next && threadIdx.x == 0) s_nj = gridDim.x + atomicAdd(src_next, 1ull);
     auto next_source = [&](size_t j) -> size_t {
         if (!src_next) return j + gridDim.x;
+        if ((j + 1) % GEN_CLAIM != 0) return j + 1;   // block-uniform
         __syncthreads();
-        const size_t jn = (size_t)s_nj;
+        const size_t jn = (size_t)s_nj * GEN_CLAIM;
         __syncthreads();   // every thread has read s_nj
         if (threadIdx.x == 0) s_nj = nn;
         return jn;
     };
     __syncthreads();
     bool range_bad = false, wbad = false;
-    for (size_t j = blockIdx.x; j < n_src; j = next_source(j)) {
-        if (src_next && threadIdx.x == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
+    for (size_t j = src_next ? blockIdx.x * GEN_CLAIM : blockIdx.x; j < n_src; j = next_source(j)) {
+        if (src_next && threadIdx.x == 0 && j % GEN_CLAIM == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
         const double temp = s.temp;
