@@ -155,6 +155,27 @@ def test_hv_multipass_matches(sys1, orc1, ctx1):
     assert bits([gd])[0] == bits([od])[0]
 
 
+@pytest.mark.parametrize("workspace", [8 << 30, 1 << 30])
+def test_hv_many_sources_matches(sys1, orc1, ctx1, workspace):
+    # 140,000 sources (2.0e8 candidates): at least 64 per generator workgroup, so the
+    # generators claim their sources from the global counter in runs of eight; one target
+    # pass, and several with a 1 GiB workspace
+    k, v = random_input(sys1, 140_000, 11)
+    ctx1.set_workspace(workspace)
+    try:
+        gk, gv, gd = gpu_hv(ctx1, k, v, len(k), 0.1, 1.0, 60_000_000)
+        st = ctx1.stats()
+    finally:
+        ctx1.set_workspace(8 << 30)
+    log("many sources: %d candidates, %d passes, %d out" % (st["n_candidates"], st["n_passes"], len(gk)))
+    assert st["n_src"] == len(k)
+    if workspace < (8 << 30):
+        assert st["n_passes"] > 1
+    ok, ov, od, _ = orc1.hv(k, v, nd=len(k), lam=0.1, coeff=1.0, cut=1e-5)
+    assert_same_vec(gk, gv, ok, ov)
+    assert bits([gd])[0] == bits([od])[0]
+
+
 @pytest.mark.parametrize("mode,hbits", [(1, 0), (2, 3), (2, 1), (2, 14), (2, 10), (4, 10), (3, 4), (3, 10)])
 def test_hv_merge_modes_identical(sys1, orc1, ctx1, mode, hbits):
     # full sort, hash buckets (compact records above 8 bits, full-key records in mode 4),
