@@ -577,10 +577,24 @@ __global__ __launch_bounds__(SFCI_BLOCK) void k_part_hist_bytes(const uint8_t* _
         const uint64_t st = ch.start[c];
         const uint32_t len = ch.len[c];
         const uint32_t* w = (const uint32_t*)(din + st);
-        for (uint32_t i = tid; 4 * i < len; i += SFCI_BLOCK) {
-            const uint32_t x = w[i];
-            const uint32_t m = min(4u, len - 4 * i);
-            for (uint32_t e = 0; e < m; ++e) atomicAdd(&h[cc][(x >> (8 * e)) & 255u], 1u);
+        // HIST_UNROLL words per thread loaded before any is counted
+        constexpr int HIST_UNROLL = 8;
+        const uint32_t nw = (len + 3) / 4;
+        for (uint32_t i0 = 0; i0 < nw; i0 += HIST_UNROLL * SFCI_BLOCK) {
+            uint32_t x[HIST_UNROLL];
+#pragma unroll
+            for (int u = 0; u < HIST_UNROLL; ++u) {
+                const uint32_t i = i0 + (uint32_t)u * SFCI_BLOCK + tid;
+                x[u] = i < nw ? w[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < HIST_UNROLL; ++u) {
+                const uint32_t i = i0 + (uint32_t)u * SFCI_BLOCK + tid;
+                if (i < nw) {
+                    const uint32_t m = min(4u, len - 4 * i);
+                    for (uint32_t e = 0; e < m; ++e) atomicAdd(&h[cc][(x[u] >> (8 * e)) & 255u], 1u);
+                }
+            }
         }
     }
     __syncthreads();
