@@ -782,6 +782,18 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
     bool range_bad = false, wbad = false;
     for (size_t j = src_next ? blockIdx.x * GEN_CLAIM : blockIdx.x; j < n_src; j = next_source(j)) {
         if (src_next && threadIdx.x == 0 && j % GEN_CLAIM == 0) nn = gridDim.x + atomicAdd(src_next, 1ull);
+        // the string-table indices and this pass's range of the spin-0 table (they depend only
+        // on j) are loaded before the source is staged, so their latency overlaps the staging
+        uint32_t ta_e0 = 0, ta_e1 = ndaa, ta_u = 0, tb_u = 0;
+        if (!SCR && tabs.a != nullptr) {
+            ta_u = tabs.a_idx[j];
+            if (tabs.n_pass > 1) {
+                const uint32_t* ps = tabs.a_start + (size_t)ta_u * (tabs.n_pass + 1) + tabs.pass;
+                ta_e0 = ps[0];
+                ta_e1 = ps[1];
+            }
+        }
+        if (!SCR && tabs.b != nullptr) tb_u = tabs.b_idx[j];
         load_source(H, s, sdec, sk[j], coeff * sv[j], lambda, row_lo, row_hi, flags);
         const uint64_t d = s.key;
         const double temp = s.temp;
@@ -807,7 +819,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
             // A2
             if (!SCR && tabs.b != nullptr) {
                 // from the table of the source's spin-1 string: sign by the spin-0 electrons
-                const SsEnt* E = tabs.b + (size_t)tabs.b_idx[j] * ndaa;
+                const SsEnt* E = tabs.b + (size_t)tb_u * ndaa;
                 const uint32_t s0 = s.str[0];
                 const uint32_t dlt = (uint32_t)(pos & 63u);   // wavefront spans start at a multiple of 64 records
                 for (uint32_t f1 = tid; f1 < ndaa + dlt; f1 += SFCI_BLOCK) {
@@ -917,13 +929,7 @@ __device__ __forceinline__ void generate_body(const HamDev& Hg, const uint64_t* 
         if (!SCR && tabs.a != nullptr) {
             // from the table of the source's spin-0 string: this pass's entries, signed by the
             // spin-1 electrons
-            const uint32_t u = tabs.a_idx[j];
-            uint32_t e0 = 0, e1 = ndaa;
-            if (tabs.n_pass > 1) {
-                const uint32_t* ps = tabs.a_start + (size_t)u * (tabs.n_pass + 1) + tabs.pass;
-                e0 = ps[0];
-                e1 = ps[1];
-            }
+            const uint32_t u = ta_u, e0 = ta_e0, e1 = ta_e1;
             const SsEnt* E = tabs.a + (size_t)u * ndaa + e0;
             const uint32_t ne = e1 - e0, s1 = s.str[1];
             const uint64_t kb = s.rank[1] - base_dense;   // wraps; the sum with r * dim does not
